@@ -1,0 +1,457 @@
+"""holo_amd -- MI355X (gfx950) Huygens point-source superposition.
+
+Python side of ``libholo.so`` (C ABI in ``include/holo.h``).  Arrays are
+NumPy (host memory: the library stages them) or torch tensors on the GPU
+(device pointers, ordered on torch's current stream).  There is no CPU
+fallback: every call fails loudly when ``libholo.so`` is missing or no
+gfx950 device is present.
+
+Mirrors of the reference interface:
+  * ``transform_full`` / ``time_transform``  -- cuda/transform.cu:147-340,
+    346-406 (the C++ templates ``transform_full<Direction, Algorithm,
+    shared_memory>`` and ``time_transform<Direction, ...>``).
+  * ``transform`` -- the Monte Carlo estimator named in README.md:127.
+  * ``Direction`` -- cuda/superposition.cu:19; ``LAMBDA`` -- cuda/macros.h:23-25.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libholo.so")
+
+LAMBDA = 0.65e-6            # cuda/macros.h:23-25 default wavelength (m)
+FLOP_PER_POINT = 29         # cuda/macros.h:71-80, the reference's pair cost
+FP64 = 0
+FP32_SPLIT = 1
+FORWARDS = +1
+BACKWARDS = -1
+DEVICE_POINTERS = 0x1
+
+
+class Direction(enum.IntEnum):
+    """cuda/superposition.cu:19: Forwards adds 2 pi d / lambda, Backwards subtracts."""
+    Forwards = +1
+    Backwards = -1
+
+
+class Algorithm(enum.IntEnum):
+    """cuda/main.h:7 / input.h:113 (-p 1, 2, 3).  All three compute the same
+    sum; here they are accepted as hints and select nothing."""
+    Naive = 1
+    Reduced = 2
+    ReducedSharedMemory = 3
+
+
+class HoloError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"holo status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+_c_double_p = ctypes.c_void_p
+
+
+def lib() -> ctypes.CDLL:
+    """Load libholo.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HoloError(3, f"{LIB_PATH} is missing; run build.py (or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i, u32, u64, d = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                              ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
+    L.holo_abi_version.restype = i
+    L.holo_status_string.restype = ctypes.c_char_p
+    L.holo_status_string.argtypes = [i]
+    L.holo_last_error.restype = ctypes.c_char_p
+    L.holo_create.argtypes = [ctypes.POINTER(vp), i, vp]
+    L.holo_destroy.argtypes = [vp]
+    L.holo_destroy.restype = None
+    L.holo_set_stream.argtypes = [vp, vp]
+    L.holo_synchronize.argtypes = [vp]
+    L.holo_set_option.argtypes = [ctypes.c_char_p, ctypes.c_long]
+    L.holo_get_option.argtypes = [ctypes.c_char_p]
+    L.holo_get_option.restype = ctypes.c_long
+    L.holo_transform_full.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, vp, ctypes.c_uint]
+    L.holo_transform_mc.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, u32, u64, vp,
+                                    ctypes.c_uint]
+    L.holo_mc_indices.argtypes = [vp, u64, u64, sz, sz, u32, vp, ctypes.c_uint]
+    L.holo_normalize_polar.argtypes = [vp, vp, sz, vp, vp, vp, ctypes.c_uint]
+    L.holo_plan_create.argtypes = [vp, ctypes.POINTER(vp), vp, sz, ctypes.c_uint, ctypes.c_uint]
+    L.holo_plan_destroy.argtypes = [vp]
+    L.holo_plan_destroy.restype = None
+    L.holo_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                 ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(d)]
+    L.holo_plan_transform.argtypes = [vp, vp, i, i, d, vp, vp, vp, sz, vp, ctypes.c_uint]
+    L.holo_plan_transform_complex.argtypes = [vp, vp, i, i, d, vp, vp, sz, vp, ctypes.c_uint]
+    L.holo_plan_adjoint.argtypes = [vp, vp, i, i, d, vp, vp, vp, vp, sz, vp, vp, ctypes.c_uint]
+    L.holo_read_dat.argtypes = [ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)]
+    L.holo_write_dat.argtypes = [ctypes.c_char_p, vp, sz]
+    for name in ("holo_create", "holo_set_stream", "holo_synchronize", "holo_set_option",
+                 "holo_transform_full", "holo_transform_mc", "holo_mc_indices",
+                 "holo_normalize_polar", "holo_plan_create", "holo_plan_info",
+                 "holo_plan_transform", "holo_plan_transform_complex", "holo_plan_adjoint",
+                 "holo_read_dat", "holo_write_dat"):
+        getattr(L, name).restype = i
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = lib().holo_last_error().decode(errors="replace")
+        raise HoloError(status, msg)
+
+
+def set_option(name: str, value: int) -> None:
+    check(lib().holo_set_option(name.encode(), int(value)))
+
+
+def get_option(name: str) -> int:
+    return int(lib().holo_get_option(name.encode()))
+
+
+# ---------------------------------------------------------------------------
+# array marshalling
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _host(x, dtype=np.float64) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=dtype)
+    return a
+
+
+def _ptr(a) -> Optional[int]:
+    if a is None:
+        return None
+    if _is_torch(a):
+        return a.data_ptr() if a.numel() else None
+    return a.ctypes.data if a.size else None
+
+
+def _prep(arrays, device: bool):
+    """Validate that arrays are all host or all device; return contiguous views."""
+    out = []
+    for a in arrays:
+        if a is None:
+            out.append(None)
+            continue
+        if device:
+            import torch
+            if not (_is_torch(a) and a.is_cuda):
+                raise HoloError(1, "mixing device tensors and host arrays")
+            if a.dtype == torch.complex128:
+                a = torch.view_as_real(a)
+            if a.dtype != torch.float64:
+                raise HoloError(1, f"device tensors must be float64/complex128, got {a.dtype}")
+            out.append(a.contiguous())
+        else:
+            if _is_torch(a):
+                a = a.detach().cpu().numpy()
+            a = np.asarray(a)
+            if np.iscomplexobj(a):
+                a = np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+            out.append(np.ascontiguousarray(a, dtype=np.float64))
+    return out
+
+
+def _n(a, per: int) -> int:
+    if a is None:
+        return 0
+    numel = a.numel() if _is_torch(a) else a.size
+    if numel % per:
+        raise HoloError(1, f"array of {numel} values is not a multiple of {per}")
+    return numel // per
+
+
+class Context:
+    """A libholo context: one device, one stream, reusable device buffers."""
+
+    def __init__(self, device: int = 0, stream=None):
+        self._h = ctypes.c_void_p()
+        s = None if stream is None else int(stream)
+        check(lib().holo_create(ctypes.byref(self._h), int(device), s))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().holo_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream) -> None:
+        check(lib().holo_set_stream(self._h, int(stream)))
+
+    def synchronize(self) -> None:
+        check(lib().holo_synchronize(self._h))
+
+    def _torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+
+    # -- brute force ---------------------------------------------------------
+    def transform_full(self, amp, phase, u, v, sign: int = FORWARDS, lam: float = LAMBDA,
+                       precision: int = FP64):
+        """y[m] = sum_n amp_n/d exp(i(phase_n + sign 2 pi d/lam)); complex (M,)."""
+        device = _is_torch(amp) and amp.is_cuda
+        amp, phase, u, v = _prep([amp, phase, u, v], device)
+        N, M = _n(amp, 1), _n(v, 3)
+        if _n(phase, 1) != N or _n(u, 3) != N:
+            raise HoloError(1, "amp, phase and u must describe the same N sources")
+        if device:
+            import torch
+            self._torch_stream()
+            y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
+            check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
+                                            _ptr(u), N, _ptr(v), M, _ptr(y), DEVICE_POINTERS))
+            return torch.view_as_complex(y)
+        y = np.empty(2 * M)
+        check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
+                                        _ptr(u), N, _ptr(v), M, _ptr(y), 0))
+        return y.view(np.complex128)
+
+    # -- Monte Carlo -----------------------------------------------------------
+    def transform_mc(self, amp, phase, u, v, K: int, seed: int = 0, sign: int = FORWARDS,
+                     lam: float = LAMBDA, precision: int = FP32_SPLIT):
+        device = _is_torch(amp) and amp.is_cuda
+        amp, phase, u, v = _prep([amp, phase, u, v], device)
+        N, M = _n(amp, 1), _n(v, 3)
+        if _n(phase, 1) != N or _n(u, 3) != N:
+            raise HoloError(1, "amp, phase and u must describe the same N sources")
+        if device:
+            import torch
+            self._torch_stream()
+            y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
+            check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
+                                          _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y),
+                                          DEVICE_POINTERS))
+            return torch.view_as_complex(y)
+        y = np.empty(2 * M)
+        check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
+                                      _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), 0))
+        return y.view(np.complex128)
+
+    def mc_indices(self, seed: int, N: int, m0: int, count: int, K: int) -> np.ndarray:
+        idx = np.empty(count * K, dtype=np.int64)
+        check(lib().holo_mc_indices(self._h, int(seed), int(N), int(m0), int(count), int(K),
+                                    _ptr(idx), 0))
+        return idx.reshape(count, K)
+
+    # -- time_transform post-processing -------------------------------------------
+    def normalize_polar(self, y):
+        device = _is_torch(y) and y.is_cuda
+        (yv,) = _prep([y], device)
+        M = _n(yv, 2)
+        if device:
+            import torch
+            self._torch_stream()
+            amp = torch.empty(M, dtype=torch.float64, device=yv.device)
+            ph = torch.empty(M, dtype=torch.float64, device=yv.device)
+            mx = torch.zeros(1, dtype=torch.float64, device=yv.device)
+            check(lib().holo_normalize_polar(self._h, _ptr(yv), M, _ptr(amp), _ptr(ph), _ptr(mx),
+                                             DEVICE_POINTERS))
+            return amp, ph, mx
+        amp = np.empty(M)
+        ph = np.empty(M)
+        mx = ctypes.c_double(0)
+        check(lib().holo_normalize_polar(self._h, _ptr(yv), M, _ptr(amp), _ptr(ph),
+                                         ctypes.addressof(mx), 0))
+        return amp, ph, mx.value
+
+    def plan(self, u, tile: int = 16) -> "Plan":
+        return Plan(self, u, tile)
+
+
+class Plan:
+    """Source geometry resident on the device (and tiled for FP32-split)."""
+
+    def __init__(self, ctx: Context, u, tile: int = 16):
+        self.ctx = ctx
+        device = _is_torch(u) and u.is_cuda
+        (uu,) = _prep([u], device)
+        self.N = _n(uu, 3)
+        self._h = ctypes.c_void_p()
+        if device:
+            ctx._torch_stream()
+        check(lib().holo_plan_create(ctx.handle, ctypes.byref(self._h), _ptr(uu), self.N,
+                                     int(tile), DEVICE_POINTERS if device else 0))
+
+    def close(self):
+        if self._h:
+            lib().holo_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        n, t, tile, off = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_uint(), ctypes.c_double()
+        check(lib().holo_plan_info(self._h, ctypes.byref(n), ctypes.byref(t), ctypes.byref(tile),
+                                   ctypes.byref(off)))
+        return {"n_sources": n.value, "n_tiles": t.value, "tile": tile.value,
+                "max_offset": off.value}
+
+    def _out(self, like, M, device):
+        if device:
+            import torch
+            return torch.empty((M, 2), dtype=torch.float64, device=like.device)
+        return np.empty(2 * M)
+
+    def transform(self, amp, phase, v, sign: int = FORWARDS, lam: float = LAMBDA,
+                  precision: int = FP64, out=None):
+        device = _is_torch(v) and v.is_cuda
+        amp, phase, v = _prep([amp, phase, v], device)
+        M = _n(v, 3)
+        if _n(amp, 1) != self.N or _n(phase, 1) != self.N:
+            raise HoloError(1, "weights do not match the plan's sources")
+        y = out if out is not None else self._out(v, M, device)
+        if device:
+            self.ctx._torch_stream()
+        check(lib().holo_plan_transform(self.ctx.handle, self._h, precision, sign, lam,
+                                        _ptr(amp), _ptr(phase), _ptr(v), M, _ptr(y),
+                                        DEVICE_POINTERS if device else 0))
+        if device:
+            import torch
+            return torch.view_as_complex(y) if y.dim() == 2 else y
+        return y.view(np.complex128)
+
+    def transform_complex(self, w, v, sign: int = FORWARDS, lam: float = LAMBDA,
+                          precision: int = FP64):
+        device = _is_torch(v) and v.is_cuda
+        w, v = _prep([w, v], device)
+        M = _n(v, 3)
+        if _n(w, 2) != self.N:
+            raise HoloError(1, "weights do not match the plan's sources")
+        y = self._out(v, M, device)
+        if device:
+            self.ctx._torch_stream()
+        check(lib().holo_plan_transform_complex(self.ctx.handle, self._h, precision, sign, lam,
+                                                _ptr(w), _ptr(v), M, _ptr(y),
+                                                DEVICE_POINTERS if device else 0))
+        if device:
+            import torch
+            return torch.view_as_complex(y)
+        return y.view(np.complex128)
+
+    def adjoint(self, g, amp, theta, u, sign: int = FORWARDS, lam: float = LAMBDA,
+                precision: int = FP64):
+        """(dL/damp, dL/dtheta) for z = transform(amp, theta; u -> plan points)."""
+        device = _is_torch(u) and u.is_cuda
+        g, amp, theta, u = _prep([g, amp, theta, u], device)
+        N = _n(u, 3)
+        if _n(g, 2) != self.N or _n(amp, 1) != N or _n(theta, 1) != N:
+            raise HoloError(1, "shapes of g / amp / theta / u do not match")
+        if device:
+            import torch
+            self.ctx._torch_stream()
+            ga = torch.empty(N, dtype=torch.float64, device=u.device)
+            gt = torch.empty(N, dtype=torch.float64, device=u.device)
+        else:
+            ga, gt = np.empty(N), np.empty(N)
+        check(lib().holo_plan_adjoint(self.ctx.handle, self._h, precision, sign, lam, _ptr(g),
+                                      _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt),
+                                      DEVICE_POINTERS if device else 0))
+        return ga, gt
+
+
+# ---------------------------------------------------------------------------
+# .dat files (cuda/util.h:248-337): headerless little-endian float64
+
+
+def read_dat(path: str) -> np.ndarray:
+    n = ctypes.c_size_t()
+    check(lib().holo_read_dat(path.encode(), None, 0, ctypes.byref(n)))
+    buf = np.empty(n.value)
+    check(lib().holo_read_dat(path.encode(), _ptr(buf), n.value, ctypes.byref(n)))
+    return buf
+
+
+def write_dat(path: str, values) -> None:
+    a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+    check(lib().holo_write_dat(path.encode(), _ptr(a), a.size))
+
+
+# ---------------------------------------------------------------------------
+# Mirrors of the reference's C++ templates (cuda/transform.cu)
+
+_default_ctx: Optional[Context] = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def transform_full(x, u, v, direction: Direction = Direction.Forwards,
+                   algorithm: Algorithm = Algorithm.Reduced, shared_memory: bool = False,
+                   lam: float = LAMBDA, precision: int = FP64,
+                   ctx: Optional[Context] = None) -> np.ndarray:
+    """transform_full<direction, algorithm, shared_memory>(x, u, v, p)
+    (cuda/transform.cu:147-151).  x: (N, 2) polar [amp, phase]; u: 3N; v: 3M.
+    Returns the unnormalised complex field at the M targets.  The reference
+    also returns padded batch entries; batching here is internal, so none
+    are appended (the C++ mirror in include/holo/transform.hpp keeps them).
+    The reference asserts u[2] != v[2] (cuda/transform.cu:193)."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 2)
+    u = np.asarray(u, dtype=np.float64).reshape(-1)
+    v = np.asarray(v, dtype=np.float64).reshape(-1)
+    if u.size and v.size and u[2] == v[2]:
+        raise HoloError(1, "first source and first target share a z offset "
+                           "(cuda/transform.cu:193 asserts u[2] != v[2])")
+    c = ctx or default_context()
+    return c.transform_full(x[:, 0], x[:, 1], u, v, int(direction), lam, precision)
+
+
+def time_transform(x, u, v, direction: Direction = Direction.Forwards,
+                   algorithm: Algorithm = Algorithm.Reduced, lam: float = LAMBDA,
+                   precision: int = FP64, ctx: Optional[Context] = None
+                   ) -> Tuple[np.ndarray, float]:
+    """time_transform<direction>(...) (cuda/transform.cu:346-406) without the
+    reference wave: returns ((M, 2) polar [amp, phase] normalised to max 1,
+    runtime in seconds)."""
+    import time
+    t0 = time.perf_counter()
+    y = transform_full(x, u, v, direction, algorithm, lam=lam, precision=precision, ctx=ctx)
+    amp, phase, _ = (ctx or default_context()).normalize_polar(y)
+    dt = time.perf_counter() - t0
+    return np.stack([amp, phase], axis=1), dt
+
+
+def transform(x, u, v, K: int, seed: int = 0, direction: Direction = Direction.Forwards,
+              lam: float = LAMBDA, precision: int = FP32_SPLIT,
+              ctx: Optional[Context] = None) -> np.ndarray:
+    """Monte Carlo transform(): K uniform source draws per target, scaled N/K."""
+    x = np.asarray(x, dtype=np.float64).reshape(-1, 2)
+    c = ctx or default_context()
+    return c.transform_mc(x[:, 0], x[:, 1], u, v, K, seed, int(direction), lam, precision)

@@ -1,0 +1,736 @@
+// C ABI of libholo.so (declared in include/holo.h).
+//
+// Host-side orchestration only: argument checks, device buffers owned by the
+// context, host<->device staging, the FP32-split tiling of source positions,
+// and launches of the kernels in holo_kernels.hip.  Every failure returns a
+// status and records a message; nothing exits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/holo.h"
+#include "holo_internal.hpp"
+
+namespace {
+
+thread_local std::string g_error;
+
+holo_status fail(holo_status s, const std::string& msg) {
+  g_error = msg;
+  return s;
+}
+
+#define HOLO_HIP(expr)                                                     \
+  do {                                                                     \
+    hipError_t e_ = (expr);                                                \
+    if (e_ != hipSuccess)                                                  \
+      return fail(e_ == hipErrorOutOfMemory ? HOLO_ENOMEM : HOLO_EDEVICE,  \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+#define HOLO_TRY(expr)                 \
+  do {                                 \
+    holo_status s_ = (expr);           \
+    if (s_ != HOLO_OK) return s_;      \
+  } while (0)
+
+constexpr size_t kMaxCount = (size_t(1) << 31) - 1;
+
+// Growable device buffer.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int device = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(device);
+      (void)hipFree(ptr);
+      (void)hipSetDevice(cur);
+    }
+    ptr = nullptr;
+    bytes = 0;
+  }
+};
+
+holo_status ensure(DevBuf& b, size_t bytes, int device) {
+  if (bytes <= b.bytes && b.ptr) return HOLO_OK;
+  b.release();
+  b.device = device;
+  const size_t want = std::max<size_t>(bytes, 256);
+  HOLO_HIP(hipMalloc(&b.ptr, want));
+  b.bytes = want;
+  return HOLO_OK;
+}
+
+}  // namespace
+
+struct holo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // staging buffers for host-pointer calls and scratch
+  DevBuf amp, phase, u, v, y, w, partial, small, b, pos, ph;
+};
+
+struct holo_plan {
+  int device = 0;
+  size_t N = 0;
+  DevBuf u;  // AoS xyz, 3N doubles
+  // FP32-split tiling
+  unsigned tile = 16;
+  size_t n_tiles = 0;
+  double max_offset = 0.0;
+  DevBuf centre, geo, perm, wc;
+};
+
+namespace {
+
+holo_status use_device(holo_ctx* ctx) {
+  if (!ctx) return fail(HOLO_EINVAL, "null context");
+  HOLO_HIP(hipSetDevice(ctx->device));
+  return HOLO_OK;
+}
+
+holo_status check_common(holo_ctx* ctx, int precision, int sign, double lambda,
+                         size_t N, size_t M) {
+  if (!ctx) return fail(HOLO_EINVAL, "null context");
+  if (precision != HOLO_FP64 && precision != HOLO_FP32_SPLIT)
+    return fail(HOLO_EINVAL, "precision must be HOLO_FP64 or HOLO_FP32_SPLIT");
+  if (sign != 1 && sign != -1) return fail(HOLO_EINVAL, "sign must be +1 or -1");
+  if (!(lambda > 0.0) || !std::isfinite(lambda))
+    return fail(HOLO_EINVAL, "lambda must be positive and finite");
+  if (N > kMaxCount || M > kMaxCount)
+    return fail(HOLO_EINVAL, "N and M must be below 2^31");
+  return HOLO_OK;
+}
+
+// Device view of an input array: the caller's pointer in device mode, a
+// staged copy otherwise.
+template <typename T>
+holo_status stage_in(holo_ctx* ctx, DevBuf& buf, const T* src, size_t count,
+                     bool dev, const T** out) {
+  if (count == 0) {
+    *out = nullptr;
+    return HOLO_OK;
+  }
+  if (!src) return fail(HOLO_EINVAL, "null input array");
+  if (dev) {
+    *out = src;
+    return HOLO_OK;
+  }
+  HOLO_TRY(ensure(buf, count * sizeof(T), ctx->device));
+  HOLO_HIP(hipMemcpyAsync(buf.ptr, src, count * sizeof(T), hipMemcpyHostToDevice,
+                          ctx->stream));
+  *out = static_cast<const T*>(buf.ptr);
+  return HOLO_OK;
+}
+
+template <typename T>
+holo_status out_buf(holo_ctx* ctx, DevBuf& buf, T* dst, size_t count, bool dev,
+                    T** out) {
+  if (count && !dst) return fail(HOLO_EINVAL, "null output array");
+  if (dev) {
+    *out = dst;
+    return HOLO_OK;
+  }
+  HOLO_TRY(ensure(buf, std::max<size_t>(count, 1) * sizeof(T), ctx->device));
+  *out = static_cast<T*>(buf.ptr);
+  return HOLO_OK;
+}
+
+template <typename T>
+holo_status finish_out(holo_ctx* ctx, T* host, const T* devp, size_t count,
+                       bool dev) {
+  if (!dev && count)
+    HOLO_HIP(hipMemcpyAsync(host, devp, count * sizeof(T), hipMemcpyDeviceToHost,
+                            ctx->stream));
+  if (!dev) HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  return HOLO_OK;
+}
+
+size_t split_elems_f64(size_t M, size_t N) {
+  const int R = holo::tuning().r_f64;
+  const int S = N ? holo::choose_splits(M, R == 1 || R == 4 ? R : 2, N, 1024) : 1;
+  return S > 1 ? (size_t)S * M : 0;
+}
+
+size_t split_elems_f32(size_t M, size_t n_tiles) {
+  const int R = holo::tuning().r_f32;
+  const int S = n_tiles ? holo::choose_splits(M, R == 1 || R == 4 ? R : 2, n_tiles, 64) : 1;
+  return S > 1 ? (size_t)S * M : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Spatial tiling of source positions for the FP32-split kernels.
+//
+// Sources are binned into cells of a regular grid over their bounding box
+// (only axes with extent), sized so a cell holds about `tile` sources; each
+// cell's sources, in input order, fill tiles of `tile` slots (the last one
+// padded with weight-0 slots).  The tile centre is the midpoint of its
+// sources' bounding box.  For a regular n x n lattice with n divisible by
+// sqrt(tile) every tile is an exact sqrt(tile) x sqrt(tile) block.
+struct HostTiles {
+  std::vector<double4> centre;
+  std::vector<float4> geo;
+  std::vector<int32_t> perm;
+  double max_offset = 0.0;
+};
+
+void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = INFINITY;
+    hi[d] = -INFINITY;
+  }
+  for (size_t n = 0; n < N; ++n)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], u[3 * n + d]);
+      hi[d] = std::max(hi[d], u[3 * n + d]);
+    }
+  double ext[3], emax = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    ext[d] = N ? hi[d] - lo[d] : 0.0;
+    emax = std::max(emax, ext[d]);
+  }
+  int active[3], na = 0;
+  for (int d = 0; d < 3; ++d)
+    if (ext[d] > 1e-9 * emax && ext[d] > 0.0) active[na++] = d;
+  // cells per axis
+  size_t cells[3] = {1, 1, 1};
+  if (na > 0 && N > T) {
+    const double target_cells = (double)N / (double)T;
+    double vol = 1.0;
+    for (int i = 0; i < na; ++i) vol *= ext[active[i]];
+    const double h = std::pow(vol / target_cells, 1.0 / na);
+    for (int i = 0; i < na; ++i) {
+      const double c = std::round(ext[active[i]] / h);
+      cells[active[i]] = (size_t)std::max(1.0, std::min(c, 1e6));
+    }
+  }
+  std::vector<uint64_t> key(N);
+  for (size_t n = 0; n < N; ++n) {
+    uint64_t k = 0;
+    for (int d = 2; d >= 0; --d) {
+      size_t c = 0;
+      if (cells[d] > 1) {
+        const double f = (u[3 * n + d] - lo[d]) / ext[d] * (double)cells[d];
+        c = (size_t)std::max(0.0, std::floor(f));
+        if (c >= cells[d]) c = cells[d] - 1;
+      }
+      k = k * cells[d] + c;
+    }
+    key[n] = k;
+  }
+  std::vector<int32_t> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return key[a] < key[b]; });
+  out.centre.clear();
+  out.geo.clear();
+  out.perm.clear();
+  out.max_offset = 0.0;
+  size_t i = 0;
+  while (i < N) {
+    size_t j = i;
+    while (j < N && key[order[j]] == key[order[i]]) ++j;  // one cell: [i, j)
+    for (size_t t0 = i; t0 < j; t0 += T) {
+      const size_t t1 = std::min(j, t0 + T);
+      double blo[3] = {INFINITY, INFINITY, INFINITY};
+      double bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (size_t s = t0; s < t1; ++s)
+        for (int d = 0; d < 3; ++d) {
+          blo[d] = std::min(blo[d], u[3 * (size_t)order[s] + d]);
+          bhi[d] = std::max(bhi[d], u[3 * (size_t)order[s] + d]);
+        }
+      double c[3];
+      for (int d = 0; d < 3; ++d) c[d] = 0.5 * (blo[d] + bhi[d]);
+      out.centre.push_back(double4{c[0], c[1], c[2], 0.0});
+      for (size_t s = t0; s < t0 + T; ++s) {
+        if (s < t1) {
+          const int32_t n = order[s];
+          const float fx = (float)(u[3 * (size_t)n] - c[0]);
+          const float fy = (float)(u[3 * (size_t)n + 1] - c[1]);
+          const float fz = (float)(u[3 * (size_t)n + 2] - c[2]);
+          const double dd = (double)fx * fx + (double)fy * fy + (double)fz * fz;
+          out.geo.push_back(float4{fx, fy, fz, (float)dd});
+          out.perm.push_back(n);
+          out.max_offset = std::max(out.max_offset, std::sqrt(dd));
+        } else {
+          out.geo.push_back(float4{0.f, 0.f, 0.f, 0.f});
+          out.perm.push_back(-1);
+        }
+      }
+    }
+    i = j;
+  }
+}
+
+holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
+  HostTiles t;
+  build_tiles(u_host, p->N, p->tile, t);
+  p->n_tiles = t.centre.size();
+  p->max_offset = t.max_offset;
+  if (p->n_tiles * (size_t)p->tile > kMaxCount)
+    return fail(HOLO_EINVAL, "too many tile slots");
+  const size_t slots = p->n_tiles * p->tile;
+  HOLO_TRY(ensure(p->centre, std::max<size_t>(p->n_tiles, 1) * sizeof(double4), ctx->device));
+  HOLO_TRY(ensure(p->geo, std::max<size_t>(slots, 1) * sizeof(float4), ctx->device));
+  HOLO_TRY(ensure(p->perm, std::max<size_t>(slots, 1) * sizeof(int32_t), ctx->device));
+  HOLO_TRY(ensure(p->wc, std::max<size_t>(slots, 1) * sizeof(float2), ctx->device));
+  if (p->n_tiles) {
+    HOLO_HIP(hipMemcpyAsync(p->centre.ptr, t.centre.data(), p->n_tiles * sizeof(double4),
+                            hipMemcpyHostToDevice, ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(p->geo.ptr, t.geo.data(), slots * sizeof(float4),
+                            hipMemcpyHostToDevice, ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(p->perm.ptr, t.perm.data(), slots * sizeof(int32_t),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
+  // host vectors are freed on return: wait for the copies
+  HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  return HOLO_OK;
+}
+
+holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
+                       unsigned tile, bool dev) {
+  p->device = ctx->device;
+  p->N = N;
+  p->tile = tile ? tile : 16;
+  if (p->tile != 8 && p->tile != 16 && p->tile != 32)
+    return fail(HOLO_EINVAL, "tile must be 8, 16 or 32");
+  if (N && !u) return fail(HOLO_EINVAL, "null source positions");
+  std::vector<double> host;
+  const double* uh = u;
+  HOLO_TRY(ensure(p->u, std::max<size_t>(3 * N, 1) * sizeof(double), ctx->device));
+  if (N) {
+    if (dev) {
+      host.resize(3 * N);
+      HOLO_HIP(hipMemcpyAsync(host.data(), u, 3 * N * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HOLO_HIP(hipStreamSynchronize(ctx->stream));
+      uh = host.data();
+      HOLO_HIP(hipMemcpyAsync(p->u.ptr, u, 3 * N * sizeof(double), hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    } else {
+      HOLO_HIP(hipMemcpyAsync(p->u.ptr, u, 3 * N * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+    }
+  }
+  return plan_upload_tiles(ctx, p, uh);
+}
+
+// y (device) = sum over the plan's sources; weights already device-resident.
+holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
+                     double lambda, holo::WeightForm wf, const double* w0,
+                     const double* w1, const double* v, size_t M, double* y) {
+  if (M == 0) return HOLO_OK;
+  if (precision == HOLO_FP64) {
+    const size_t pe = split_elems_f64(M, p->N);
+    if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
+    const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
+    HOLO_HIP(holo::launch_superpose_f64(sign, wf, w0, w1, static_cast<const double*>(p->u.ptr),
+                                        p->N, v, M, K, reinterpret_cast<double2*>(y),
+                                        static_cast<double2*>(ctx->partial.ptr), pe,
+                                        ctx->stream));
+  } else {
+    const size_t slots = p->n_tiles * p->tile;
+    HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->perm.ptr),
+                                       slots, static_cast<float2*>(p->wc.ptr), ctx->stream));
+    holo::TileSet ts;
+    ts.centre = static_cast<const double4*>(p->centre.ptr);
+    ts.geo = static_cast<const float4*>(p->geo.ptr);
+    ts.perm = static_cast<const int32_t*>(p->perm.ptr);
+    ts.n_tiles = (uint32_t)p->n_tiles;
+    ts.tile = p->tile;
+    const size_t pe = split_elems_f32(M, p->n_tiles);
+    if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
+    HOLO_HIP(holo::launch_superpose_f32s(sign, ts, static_cast<const float2*>(p->wc.ptr), v, M,
+                                         lambda, reinterpret_cast<double2*>(y),
+                                         static_cast<double2*>(ctx->partial.ptr), pe,
+                                         ctx->stream));
+  }
+  return HOLO_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+
+extern "C" {
+
+int holo_abi_version(void) { return HOLO_ABI_VERSION; }
+
+const char* holo_status_string(holo_status s) {
+  switch (s) {
+    case HOLO_OK: return "ok";
+    case HOLO_EINVAL: return "invalid argument";
+    case HOLO_ENOMEM: return "out of memory";
+    case HOLO_EDEVICE: return "device error";
+    case HOLO_EIO: return "i/o error";
+  }
+  return "unknown status";
+}
+
+const char* holo_last_error(void) { return g_error.c_str(); }
+
+holo_status holo_create(holo_ctx** out, int device, void* stream) {
+  if (!out) return fail(HOLO_EINVAL, "null context pointer");
+  *out = nullptr;
+  int count = 0;
+  HOLO_HIP(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(HOLO_EDEVICE, "no HIP device " + std::to_string(device));
+  hipDeviceProp_t prop;
+  HOLO_HIP(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(HOLO_EDEVICE, std::string("libholo is built for gfx950, device is ") +
+                                  prop.gcnArchName);
+  HOLO_HIP(hipSetDevice(device));
+  holo_ctx* c = new holo_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = static_cast<hipStream_t>(stream);
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      return fail(HOLO_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return HOLO_OK;
+}
+
+void holo_destroy(holo_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+holo_status holo_set_stream(holo_ctx* ctx, void* stream) {
+  HOLO_TRY(use_device(ctx));
+  if (!stream) return fail(HOLO_EINVAL, "null stream");
+  if (ctx->own_stream) {
+    HOLO_HIP(hipStreamSynchronize(ctx->stream));
+    HOLO_HIP(hipStreamDestroy(ctx->stream));
+    ctx->own_stream = false;
+  }
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return HOLO_OK;
+}
+
+holo_status holo_synchronize(holo_ctx* ctx) {
+  HOLO_TRY(use_device(ctx));
+  HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  return HOLO_OK;
+}
+
+holo_status holo_set_option(const char* name, long value) {
+  if (!name) return fail(HOLO_EINVAL, "null option name");
+  holo::Tuning& t = holo::tuning();
+  const std::string n(name);
+  if (n == "r_f64" || n == "r_f32") {
+    if (value != 1 && value != 2 && value != 4)
+      return fail(HOLO_EINVAL, n + " must be 1, 2 or 4");
+    (n == "r_f64" ? t.r_f64 : t.r_f32) = (int)value;
+  } else if (n == "splits") {
+    if (value < 0 || value > 64) return fail(HOLO_EINVAL, "splits must be in [0, 64]");
+    t.splits = (int)value;
+  } else if (n == "target_waves") {
+    if (value < 1) return fail(HOLO_EINVAL, "target_waves must be positive");
+    t.target_waves = (int)value;
+  } else if (n == "f64_libm") {
+    t.f64_libm = value != 0;
+  } else {
+    return fail(HOLO_EINVAL, "unknown option " + n);
+  }
+  return HOLO_OK;
+}
+
+long holo_get_option(const char* name) {
+  if (!name) return -1;
+  const holo::Tuning& t = holo::tuning();
+  const std::string n(name);
+  if (n == "r_f64") return t.r_f64;
+  if (n == "r_f32") return t.r_f32;
+  if (n == "splits") return t.splits;
+  if (n == "target_waves") return t.target_waves;
+  if (n == "f64_libm") return t.f64_libm ? 1 : 0;
+  return -1;
+}
+
+holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sign,
+                                double lambda, const double* amp, const double* phase,
+                                const double* u, size_t N, const double* v, size_t M,
+                                double* y, unsigned flags) {
+  HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
+  HOLO_TRY(use_device(ctx));
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  if (M == 0) return HOLO_OK;
+  double* dy;
+  HOLO_TRY(out_buf(ctx, ctx->y, y, 2 * M, dev, &dy));
+  const double *da, *dp, *dv;
+  HOLO_TRY(stage_in(ctx, ctx->amp, amp, N, dev, &da));
+  HOLO_TRY(stage_in(ctx, ctx->phase, phase, N, dev, &dp));
+  HOLO_TRY(stage_in(ctx, ctx->v, v, 3 * M, dev, &dv));
+  if (N == 0) {
+    HOLO_HIP(holo::launch_zero(dy, 2 * M, ctx->stream));
+    return finish_out(ctx, y, dy, 2 * M, dev);
+  }
+  if (precision == HOLO_FP64) {
+    const double* du;
+    HOLO_TRY(stage_in(ctx, ctx->u, u, 3 * N, dev, &du));
+    const size_t pe = split_elems_f64(M, N);
+    if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
+    const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
+    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, da, dp, du, N, dv, M, K,
+                                        reinterpret_cast<double2*>(dy),
+                                        static_cast<double2*>(ctx->partial.ptr), pe,
+                                        ctx->stream));
+  } else {
+    holo_plan plan;
+    HOLO_TRY(plan_build(ctx, &plan, u, N, 16, dev));
+    HOLO_TRY(plan_run(ctx, &plan, precision, sign, lambda, holo::kPolar, da, dp, dv, M, dy));
+    HOLO_HIP(hipStreamSynchronize(ctx->stream));  // plan buffers die here
+  }
+  return finish_out(ctx, y, dy, 2 * M, dev);
+}
+
+holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
+                              double lambda, const double* amp, const double* phase,
+                              const double* u, size_t N, const double* v, size_t M,
+                              uint32_t K, uint64_t seed, double* y, unsigned flags) {
+  HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
+  HOLO_TRY(use_device(ctx));
+  if (K == 0) return fail(HOLO_EINVAL, "K must be positive");
+  if (N == 0) return fail(HOLO_EINVAL, "Monte Carlo needs at least one source");
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  if (M == 0) return HOLO_OK;
+  double* dy;
+  HOLO_TRY(out_buf(ctx, ctx->y, y, 2 * M, dev, &dy));
+  const double *da, *dp, *du, *dv;
+  HOLO_TRY(stage_in(ctx, ctx->amp, amp, N, dev, &da));
+  HOLO_TRY(stage_in(ctx, ctx->phase, phase, N, dev, &dp));
+  HOLO_TRY(stage_in(ctx, ctx->u, u, 3 * N, dev, &du));
+  HOLO_TRY(stage_in(ctx, ctx->v, v, 3 * M, dev, &dv));
+  const bool fp64 = precision == HOLO_FP64;
+  HOLO_TRY(ensure(ctx->pos, N * sizeof(double4), ctx->device));
+  HOLO_TRY(ensure(ctx->ph, N * sizeof(double), ctx->device));
+  HOLO_HIP(holo::launch_mc_pack(fp64, da, dp, du, N, static_cast<double4*>(ctx->pos.ptr),
+                                static_cast<double*>(ctx->ph.ptr), ctx->stream));
+  const double Kw = (2 * M_PI) / lambda;
+  HOLO_HIP(holo::launch_mc(sign, fp64, static_cast<const double4*>(ctx->pos.ptr),
+                           static_cast<const double*>(ctx->ph.ptr), N, dv, M, K, seed, Kw,
+                           lambda, reinterpret_cast<double2*>(dy), ctx->stream));
+  return finish_out(ctx, y, dy, 2 * M, dev);
+}
+
+holo_status holo_mc_indices(holo_ctx* ctx, uint64_t seed, uint64_t N, size_t m0,
+                            size_t count, uint32_t K, int64_t* idx, unsigned flags) {
+  HOLO_TRY(use_device(ctx));
+  if (N == 0 || N > kMaxCount) return fail(HOLO_EINVAL, "N must be in [1, 2^31)");
+  if (m0 + count > kMaxCount) return fail(HOLO_EINVAL, "target range beyond 2^31");
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  const size_t total = count * (size_t)K;
+  if (!total) return HOLO_OK;
+  int64_t* di;
+  HOLO_TRY(out_buf(ctx, ctx->w, idx, total, dev, &di));
+  HOLO_HIP(holo::launch_mc_indices(seed, N, m0, count, K, di, ctx->stream));
+  return finish_out(ctx, idx, di, total, dev);
+}
+
+holo_status holo_normalize_polar(holo_ctx* ctx, const double* y, size_t M, double* amp,
+                                 double* phase, double* max_amp, unsigned flags) {
+  HOLO_TRY(use_device(ctx));
+  if (M > kMaxCount) return fail(HOLO_EINVAL, "M must be below 2^31");
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  if (M == 0) {
+    if (max_amp && !dev) *max_amp = 0.0;
+    return HOLO_OK;
+  }
+  const double* dy;
+  HOLO_TRY(stage_in(ctx, ctx->y, y, 2 * M, dev, &dy));
+  double *da, *dp;
+  HOLO_TRY(out_buf(ctx, ctx->amp, amp, M, dev, &da));
+  HOLO_TRY(out_buf(ctx, ctx->phase, phase, M, dev, &dp));
+  HOLO_TRY(ensure(ctx->small, 2048 * sizeof(double), ctx->device));
+  double* scratch = static_cast<double*>(ctx->small.ptr);
+  double* dmax = (dev && max_amp) ? max_amp : scratch + 1024;
+  HOLO_HIP(holo::launch_normalize_polar(reinterpret_cast<const double2*>(dy), M, da, dp, dmax,
+                                        scratch, ctx->stream));
+  if (!dev) {
+    HOLO_HIP(hipMemcpyAsync(amp, da, M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(phase, dp, M * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    double mx = 0.0;
+    HOLO_HIP(hipMemcpyAsync(&mx, dmax, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HOLO_HIP(hipStreamSynchronize(ctx->stream));
+    if (max_amp) *max_amp = mx;
+  }
+  return HOLO_OK;
+}
+
+holo_status holo_plan_create(holo_ctx* ctx, holo_plan** plan, const double* u, size_t N,
+                             unsigned tile, unsigned flags) {
+  HOLO_TRY(use_device(ctx));
+  if (!plan) return fail(HOLO_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  if (N > kMaxCount) return fail(HOLO_EINVAL, "N must be below 2^31");
+  holo_plan* p = new holo_plan();
+  holo_status s = plan_build(ctx, p, u, N, tile, flags & HOLO_DEVICE_POINTERS);
+  if (s != HOLO_OK) {
+    delete p;
+    return s;
+  }
+  *plan = p;
+  return HOLO_OK;
+}
+
+void holo_plan_destroy(holo_plan* plan) { delete plan; }
+
+holo_status holo_plan_info(const holo_plan* plan, size_t* n_sources, size_t* n_tiles,
+                           unsigned* tile, double* max_offset) {
+  if (!plan) return fail(HOLO_EINVAL, "null plan");
+  if (n_sources) *n_sources = plan->N;
+  if (n_tiles) *n_tiles = plan->n_tiles;
+  if (tile) *tile = plan->tile;
+  if (max_offset) *max_offset = plan->max_offset;
+  return HOLO_OK;
+}
+
+static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int precision,
+                                      int sign, double lambda, holo::WeightForm wf,
+                                      const double* w0, const double* w1, const double* v,
+                                      size_t M, double* y, unsigned flags) {
+  if (!plan) return fail(HOLO_EINVAL, "null plan");
+  HOLO_TRY(check_common(ctx, precision, sign, lambda, plan->N, M));
+  HOLO_TRY(use_device(ctx));
+  if (plan->device != ctx->device) return fail(HOLO_EINVAL, "plan belongs to another device");
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  if (M == 0) return HOLO_OK;
+  double* dy;
+  HOLO_TRY(out_buf(ctx, ctx->y, y, 2 * M, dev, &dy));
+  const double* dv;
+  HOLO_TRY(stage_in(ctx, ctx->v, v, 3 * M, dev, &dv));
+  const size_t N = plan->N;
+  if (N == 0) {
+    HOLO_HIP(holo::launch_zero(dy, 2 * M, ctx->stream));
+    return finish_out(ctx, y, dy, 2 * M, dev);
+  }
+  const double *d0, *d1 = nullptr;
+  if (wf == holo::kPolar) {
+    HOLO_TRY(stage_in(ctx, ctx->amp, w0, N, dev, &d0));
+    HOLO_TRY(stage_in(ctx, ctx->phase, w1, N, dev, &d1));
+  } else {
+    HOLO_TRY(stage_in(ctx, ctx->amp, w0, 2 * N, dev, &d0));
+  }
+  HOLO_TRY(plan_run(ctx, plan, precision, sign, lambda, wf, d0, d1, dv, M, dy));
+  return finish_out(ctx, y, dy, 2 * M, dev);
+}
+
+holo_status holo_plan_transform(holo_ctx* ctx, const holo_plan* plan, holo_precision precision,
+                                int sign, double lambda, const double* amp,
+                                const double* phase, const double* v, size_t M, double* y,
+                                unsigned flags) {
+  return plan_transform_any(ctx, plan, precision, sign, lambda, holo::kPolar, amp, phase, v, M,
+                            y, flags);
+}
+
+holo_status holo_plan_transform_complex(holo_ctx* ctx, const holo_plan* plan,
+                                        holo_precision precision, int sign, double lambda,
+                                        const double* w, const double* v, size_t M, double* y,
+                                        unsigned flags) {
+  return plan_transform_any(ctx, plan, precision, sign, lambda, holo::kCartesian, w, nullptr,
+                            v, M, y, flags);
+}
+
+holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
+                              holo_precision precision, int sign, double lambda,
+                              const double* g, const double* amp, const double* theta,
+                              const double* u, size_t N, double* grad_amp, double* grad_theta,
+                              unsigned flags) {
+  if (!plan_targets) return fail(HOLO_EINVAL, "null plan");
+  HOLO_TRY(check_common(ctx, precision, sign, lambda, N, plan_targets->N));
+  HOLO_TRY(use_device(ctx));
+  if (plan_targets->device != ctx->device)
+    return fail(HOLO_EINVAL, "plan belongs to another device");
+  const bool dev = flags & HOLO_DEVICE_POINTERS;
+  if (N == 0) return HOLO_OK;
+  const size_t M = plan_targets->N;
+  const double *dg = nullptr, *da, *dt, *du;
+  HOLO_TRY(stage_in(ctx, ctx->w, g, 2 * M, dev, &dg));
+  HOLO_TRY(stage_in(ctx, ctx->amp, amp, N, dev, &da));
+  HOLO_TRY(stage_in(ctx, ctx->phase, theta, N, dev, &dt));
+  HOLO_TRY(stage_in(ctx, ctx->u, u, 3 * N, dev, &du));
+  double *dga, *dgt;
+  HOLO_TRY(out_buf(ctx, ctx->pos, grad_amp, N, dev, &dga));
+  HOLO_TRY(out_buf(ctx, ctx->ph, grad_theta, N, dev, &dgt));
+  HOLO_TRY(ensure(ctx->b, 2 * N * sizeof(double), ctx->device));
+  double* db = static_cast<double*>(ctx->b.ptr);
+  if (M == 0) {
+    HOLO_HIP(holo::launch_zero(db, 2 * N, ctx->stream));
+  } else {
+    // b_n: the forward targets carry g and radiate back with the sign flipped
+    HOLO_TRY(plan_run(ctx, plan_targets, precision, -sign, lambda, holo::kCartesian, dg,
+                      nullptr, du, N, db));
+  }
+  HOLO_HIP(holo::launch_adjoint_epilogue(reinterpret_cast<const double2*>(db), da, dt, N, dga,
+                                         dgt, ctx->stream));
+  if (!dev) {
+    HOLO_HIP(hipMemcpyAsync(grad_amp, dga, N * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(grad_theta, dgt, N * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return HOLO_OK;
+}
+
+holo_status holo_read_dat(const char* path, double* buf, size_t capacity, size_t* count) {
+  if (!path || !count) return fail(HOLO_EINVAL, "null path or count");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(HOLO_EIO, std::string("cannot open ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (size <= 0 || size % (long)sizeof(double)) {
+    std::fclose(f);
+    return fail(HOLO_EIO, std::string("empty or ragged .dat file ") + path);
+  }
+  const size_t n = (size_t)size / sizeof(double);
+  *count = n;
+  if (buf) {
+    if (capacity < n) {
+      std::fclose(f);
+      return fail(HOLO_EINVAL, "buffer too small for " + std::string(path));
+    }
+    if (std::fread(buf, sizeof(double), n, f) != n) {
+      std::fclose(f);
+      return fail(HOLO_EIO, std::string("short read of ") + path);
+    }
+  }
+  std::fclose(f);
+  return HOLO_OK;
+}
+
+holo_status holo_write_dat(const char* path, const double* buf, size_t count) {
+  if (!path || (count && !buf)) return fail(HOLO_EINVAL, "null path or buffer");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(HOLO_EIO, std::string("cannot create ") + path);
+  const size_t w = count ? std::fwrite(buf, sizeof(double), count, f) : 0;
+  const int rc = std::fclose(f);
+  if (w != count || rc != 0) return fail(HOLO_EIO, std::string("write failed: ") + path);
+  return HOLO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Device math for the Huygens superposition on gfx950 (CDNA4, wave64).
+//
+// pair_f64: the FP64 pair of cuda/superposition.cu:35-51 with the operation
+// order pinned (each step one IEEE rounding):
+//   dx = v0-u0, dy = v1-u1, dz = v2-u2
+//   s   = fma(dz, dz, fma(dy, dy, dx*dx))
+//   d   = sqrt(s)                         (correctly rounded)
+//   arg = fma(+/-d, K, phi)               (phase +/- distance*K, one rounding)
+//   r   = a / d
+// The phase argument reaches ~3.5e6 rad at 0.35 m and 632.8 nm, where one ulp
+// of d moves the phase by ~5e-10 rad, so d and arg must be bit-identical to
+// the CPU restatement for FP64 results to agree to 1e-12 relative L2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "holo_math.hpp"
+
+namespace holo {
+
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+
+struct PairF64 {
+  double r;    // amplitude a/d
+  double arg;  // phase argument
+};
+
+// MATH = 0: hm_sqrt_inv (d correctly rounded, 1/d to ~1 ulp); MATH = 1: the
+// compiler's sqrt and IEEE division.  Both give the same d and arg bits.
+template <int SIGN, int MATH = 0>
+__device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
+                                            double uy, double uz, double vx,
+                                            double vy, double vz, double K) {
+#pragma clang fp contract(off)
+  const double dx = vx - ux;
+  const double dy = vy - uy;
+  const double dz = vz - uz;
+  const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  PairF64 p;
+  if constexpr (MATH == 0) {
+    double d, inv;
+    hm_sqrt_inv(s, d, inv);
+    p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);
+    p.r = a * inv;
+  } else {
+    const double d = __builtin_sqrt(s);
+    p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);
+    p.r = a / d;
+  }
+  return p;
+}
+
+// sin/cos of the pair argument: hm_sincos (MATH = 0) or the device libm.
+template <int MATH = 0>
+__device__ __forceinline__ void sincos_f64(double x, double& s, double& c) {
+  if constexpr (MATH == 0) {
+    hm_sincos(x, s, c);
+  } else {
+    sincos(x, &s, &c);
+  }
+}
+
+// Distance only (same pinned order as pair_f64).
+__device__ __forceinline__ double distance_f64(double ux, double uy, double uz,
+                                               double vx, double vy,
+                                               double vz) {
+#pragma clang fp contract(off)
+  const double dx = vx - ux;
+  const double dy = vy - uy;
+  const double dz = vz - uz;
+  return __builtin_sqrt(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)));
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11), counter-based: no state arrays.
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    U4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+  }
+  return c;
+}
+
+constexpr uint32_t kMcStream = 0x4d43u;  // third counter word of MC draws
+
+// Lemire multiply-shift of a 64-bit uniform word into [0, N).
+__device__ __forceinline__ uint32_t lemire_index(uint32_t lo, uint32_t hi,
+                                                 uint64_t N) {
+  const uint64_t r = ((uint64_t)hi << 32) | lo;
+  return (uint32_t)__umul64hi(r, N);
+}
+
+// Fractional cycle in [-0.5, 0.5]: x - rint(x), exact in FP64 for |x| < 2^52.
+__device__ __forceinline__ double frac_cycle(double x) {
+  return x - __builtin_rint(x);
+}
+
+// sin/cos of 2*pi*t for t in revolutions (v_sin_f32 / v_cos_f32 take
+// revolutions).
+__device__ __forceinline__ void sincos_rev_f32(float t, float& s, float& c) {
+  s = __builtin_amdgcn_sinf(t);
+  c = __builtin_amdgcn_cosf(t);
+}
+
+}  // namespace holo

@@ -1,0 +1,89 @@
+// Internal launch interface between the C-ABI (holo_capi.hip) and the
+// kernels (holo_kernels.hip).  Every pointer here is a device pointer and
+// every launch is stream-ordered.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace holo {
+
+enum WeightForm { kPolar = 0, kCartesian = 1 };
+
+// Source set for the FP32-split kernels: sources grouped into spatial tiles
+// of `tile` slots.  Slot s of tile t is geo[t*tile + s] = (dx, dy, dz, |d|^2)
+// relative to the tile centre centre[t] = (cx, cy, cz, 0); perm[slot] is the
+// caller's source index or -1 for a padding slot (weight 0).
+struct TileSet {
+  const double4* centre = nullptr;
+  const float4* geo = nullptr;
+  const int32_t* perm = nullptr;
+  uint32_t n_tiles = 0;
+  uint32_t tile = 0;
+};
+
+struct Tuning {
+  int r_f64 = 2;          // targets per lane, FP64 kernel (1, 2, 4)
+  int r_f32 = 2;          // targets per lane, FP32-split kernel (1, 2, 4)
+  int splits = 0;         // source splits (0 = automatic)
+  int target_waves = 16384;
+  bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
+};
+
+Tuning& tuning();
+
+// Number of source splits S chosen for a launch over M targets with R
+// targets per lane and `units` source units (sources or tiles).
+int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
+
+// y (interleaved complex, M) = sum_n pair(n, m).  Polar weights: w0 = amp,
+// w1 = phase.  Cartesian: w0 = interleaved complex weights, w1 unused.
+// `partial` must hold S*M double2 when S > 1 (see partial_elems()).
+hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
+                                const double* w1, const double* u, size_t N,
+                                const double* v, size_t M, double K,
+                                double2* y, double2* partial,
+                                size_t partial_cap, hipStream_t stream);
+
+// FP32-split: weights wc (float2 per slot), tiles as above.
+hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
+                                 const float2* wc, const double* v, size_t M,
+                                 double lambda, double2* y, double2* partial,
+                                 size_t partial_cap, hipStream_t stream);
+
+// wc[slot] = a e^{i phi} (polar) or w (cartesian) of source perm[slot].
+hipError_t launch_tile_weights(WeightForm wf, const double* w0,
+                               const double* w1, const int32_t* perm,
+                               size_t n_slots, float2* wc, hipStream_t stream);
+
+// Monte Carlo: y[m] = (N/K) sum_k pair(n_{m,k}, m).
+// FP64: pos = (x,y,z,amp) per source, ph = phase per source.
+// FP32-split: pos = (x,y,z, bitcast float2 weight).
+hipError_t launch_mc(int sign, bool fp64, const double4* pos,
+                     const double* ph, size_t N, const double* v, size_t M,
+                     uint32_t K, uint64_t seed, double K_wave, double lambda,
+                     double2* y, hipStream_t stream);
+
+hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
+                          const double* u, size_t N, double4* pos, double* ph,
+                          hipStream_t stream);
+
+hipError_t launch_mc_indices(uint64_t seed, uint64_t N, size_t m0,
+                             size_t count, uint32_t K, int64_t* idx,
+                             hipStream_t stream);
+
+// max |y| into *max_out (device), then amp/phase = polar(y / max) (or y when
+// max < 1e-6).  scratch: >= 1024 doubles.
+hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
+                                  double* phase, double* max_out,
+                                  double* scratch, hipStream_t stream);
+
+hipError_t launch_zero(double* x, size_t n, hipStream_t stream);
+
+// Adjoint epilogue: grad_amp = Re(e^{-i theta} b), grad_theta = A Im(...).
+hipError_t launch_adjoint_epilogue(const double2* b, const double* amp,
+                                   const double* theta, size_t N,
+                                   double* grad_amp, double* grad_theta,
+                                   hipStream_t stream);
+
+}  // namespace holo

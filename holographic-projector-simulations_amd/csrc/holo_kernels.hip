@@ -1,0 +1,609 @@
+// Hand-written gfx950 kernels for the Huygens point-source superposition
+//   y_m = sum_n a_n / d_nm * exp(i (phi_n +/- 2 pi d_nm / lambda))
+// (cuda/superposition.cu:35-154 computes the same sum with per_block and a
+// CUB/cuBLAS reduction; these kernels are a new design for CDNA4):
+//
+// * One lane owns R targets; their positions and FP64 accumulators stay in
+//   VGPRs for the whole launch.
+// * The source index is wave-uniform, so source data is fetched with scalar
+//   loads into SGPRs and broadcast to all 64 lanes for free -- no LDS and no
+//   per-lane source traffic.
+// * Sources are split into S contiguous chunks (blockIdx.y) only when the
+//   target count alone cannot fill the 256 CUs; the S partial fields are
+//   summed afterwards in a fixed order, so results are deterministic.
+// * No MFMA: the sum is not a contraction; it is bound by VALU and
+//   transcendental issue.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "holo_device.hpp"
+#include "holo_internal.hpp"
+
+namespace holo {
+
+Tuning& tuning() {
+  static Tuning t;
+  return t;
+}
+
+int choose_splits(size_t M, int R, size_t units, size_t min_units) {
+  const Tuning& tu = tuning();
+  if (tu.splits > 0) return (int)std::min<size_t>(tu.splits, std::max<size_t>(units, 1));
+  const size_t waves = (M + 64 * R - 1) / (64 * R);
+  size_t S = (tu.target_waves + waves - 1) / std::max<size_t>(waves, 1);
+  const size_t max_by_units = std::max<size_t>(units / std::max<size_t>(min_units, 1), 1);
+  S = std::min(S, max_by_units);
+  S = std::min<size_t>(S, 64);
+  return (int)std::max<size_t>(S, 1);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 brute force.
+
+template <int SIGN, int R, int WF, int MATH>
+__global__ __launch_bounds__(kBlock) void k_superpose_f64(
+    const double* __restrict__ w0, const double* __restrict__ w1,
+    const double* __restrict__ u, uint32_t N, uint32_t chunk,
+    const double* __restrict__ v, uint32_t M, double K,
+    double2* __restrict__ out) {
+  const uint32_t n0 = blockIdx.y * chunk;
+  const uint32_t n1 = min(N, n0 + chunk);
+  const uint32_t base = blockIdx.x * (kBlock * R) + threadIdx.x;
+  double vx[R], vy[R], vz[R], ar[R], ai[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = base + r * kBlock;
+    const size_t mm = m < M ? m : M - 1;
+    vx[r] = v[3 * mm];
+    vy[r] = v[3 * mm + 1];
+    vz[r] = v[3 * mm + 2];
+    ar[r] = 0.0;
+    ai[r] = 0.0;
+  }
+  for (uint32_t n = n0; n < n1; ++n) {
+    const double ux = u[3 * (size_t)n];
+    const double uy = u[3 * (size_t)n + 1];
+    const double uz = u[3 * (size_t)n + 2];
+    if constexpr (WF == kPolar) {
+      const double a = w0[n];
+      const double phi = w1[n];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const PairF64 p = pair_f64<SIGN, MATH>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
+        double s, c;
+        sincos_f64<MATH>(p.arg, s, c);
+        ar[r] = __builtin_fma(p.r, c, ar[r]);
+        ai[r] = __builtin_fma(p.r, s, ai[r]);
+      }
+    } else {
+      const double wr = w0[2 * (size_t)n];
+      const double wi = w0[2 * (size_t)n + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double dx = vx[r] - ux, dy = vy[r] - uy, dz = vz[r] - uz;
+        double d, id;
+        hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+        const double arg = (SIGN > 0 ? d : -d) * K;
+        double s, c;
+        sincos_f64<MATH>(arg, s, c);
+        c *= id;
+        s *= id;
+        ar[r] += wr * c - wi * s;
+        ai[r] += wr * s + wi * c;
+      }
+    }
+  }
+  double2* dst = out + (size_t)blockIdx.y * M;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = base + r * kBlock;
+    if (m < M) dst[m] = make_double2(ar[r], ai[r]);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(
+    const double2* __restrict__ partial, uint32_t S, uint32_t M,
+    double2* __restrict__ y) {
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= M) return;
+  double re = 0.0, im = 0.0;
+  for (uint32_t s = 0; s < S; ++s) {  // fixed order: deterministic
+    const double2 p = partial[(size_t)s * M + m];
+    re += p.x;
+    im += p.y;
+  }
+  y[m] = make_double2(re, im);
+}
+
+static hipError_t reduce_if_split(int S, size_t M, const double2* partial,
+                                  double2* y, hipStream_t stream) {
+  if (S <= 1) return hipSuccess;
+  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
+  k_reduce_partials<<<blocks, kBlock, 0, stream>>>(partial, (uint32_t)S,
+                                                   (uint32_t)M, y);
+  return hipGetLastError();
+}
+
+template <int SIGN, int R, int WF>
+static void launch_f64_t(dim3 grid, const double* w0, const double* w1,
+                         const double* u, size_t N, uint32_t chunk,
+                         const double* v, size_t M, double K, double2* out,
+                         hipStream_t stream) {
+  if (tuning().f64_libm)
+    k_superpose_f64<SIGN, R, WF, 1><<<grid, kBlock, 0, stream>>>(
+        w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
+  else
+    k_superpose_f64<SIGN, R, WF, 0><<<grid, kBlock, 0, stream>>>(
+        w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
+}
+
+template <int SIGN, int WF>
+static void launch_f64_r(int R, dim3 grid, const double* w0, const double* w1,
+                         const double* u, size_t N, uint32_t chunk,
+                         const double* v, size_t M, double K, double2* out,
+                         hipStream_t stream) {
+  switch (R) {
+    case 1: launch_f64_t<SIGN, 1, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
+    case 4: launch_f64_t<SIGN, 4, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
+    default: launch_f64_t<SIGN, 2, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
+  }
+}
+
+hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
+                                const double* w1, const double* u, size_t N,
+                                const double* v, size_t M, double K,
+                                double2* y, double2* partial,
+                                size_t partial_cap, hipStream_t stream) {
+  if (M == 0) return hipSuccess;
+  int R = tuning().r_f64;
+  if (R != 1 && R != 2 && R != 4) R = 2;
+  int S = N ? choose_splits(M, R, N, 1024) : 1;
+  while (S > 1 && (size_t)S * M > partial_cap) --S;
+  const uint32_t chunk = N ? (uint32_t)((N + S - 1) / S) : 0;
+  dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
+            (unsigned)S);
+  double2* out = S > 1 ? partial : y;
+  if (sign > 0) {
+    if (wf == kPolar) launch_f64_r<+1, kPolar>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
+    else launch_f64_r<+1, kCartesian>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
+  } else {
+    if (wf == kPolar) launch_f64_r<-1, kPolar>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
+    else launch_f64_r<-1, kCartesian>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return reduce_if_split(S, M, partial, y, stream);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-split brute force over tiles of T sources.
+//
+// Per (target, tile), in FP64: r = v - c, d_ref = |r|, the fractional cycle
+// f = frac(d_ref / lambda), and the tile rotation e^{i sign 2 pi f}.
+// Per pair, in FP32 and without cancellation:
+//   q = |delta|^2 - 2 r.delta        (= d^2 - d_ref^2, delta = u - c)
+//   W = q / (2 d_ref lambda)         (first-order offset, cycles)
+//   t = W - (lambda / (2 d_ref)) W^2 (= (d - d_ref)/lambda to O(q^3))
+//   1/d = 1/d_ref - t lambda / d_ref^2
+//   term = w * e^{i sign 2 pi t} / d
+// The T terms of a tile are summed in FP32, rotated by the tile's FP64-derived
+// phase and added into FP64 accumulators.
+
+template <int SIGN, int R, int T>
+__global__ __launch_bounds__(kBlock) void k_superpose_f32s(
+    const double4* __restrict__ centre, const float4* __restrict__ geo,
+    const float2* __restrict__ wc, uint32_t n_tiles, uint32_t chunk_tiles,
+    const double* __restrict__ v, uint32_t M, double lambda,
+    double inv_lambda, double2* __restrict__ out) {
+  const uint32_t t0 = blockIdx.y * chunk_tiles;
+  const uint32_t t1 = min(n_tiles, t0 + chunk_tiles);
+  const uint32_t base = blockIdx.x * (kBlock * R) + threadIdx.x;
+  double vx[R], vy[R], vz[R], ar[R], ai[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = base + r * kBlock;
+    const size_t mm = m < M ? m : M - 1;
+    vx[r] = v[3 * mm];
+    vy[r] = v[3 * mm + 1];
+    vz[r] = v[3 * mm + 2];
+    ar[r] = 0.0;
+    ai[r] = 0.0;
+  }
+  for (uint32_t t = t0; t < t1; ++t) {
+    const double4 c = centre[t];
+    float m2rx[R], m2ry[R], m2rz[R], alpha[R], beta[R], gam[R], invd[R];
+    float ct[R], st[R], sr[R], si[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double rx = vx[r] - c.x, ry = vy[r] - c.y, rz = vz[r] - c.z;
+      double dref, id;
+      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      const float f = (float)frac_cycle(dref * inv_lambda);
+      sincos_rev_f32(f, st[r], ct[r]);
+      m2rx[r] = (float)(-2.0 * rx);
+      m2ry[r] = (float)(-2.0 * ry);
+      m2rz[r] = (float)(-2.0 * rz);
+      alpha[r] = (float)(0.5 * id * inv_lambda);
+      beta[r] = (float)(0.5 * lambda * id);
+      gam[r] = (float)(lambda * id * id);
+      invd[r] = (float)id;
+      sr[r] = 0.f;
+      si[r] = 0.f;
+    }
+    const float4* g = geo + (size_t)t * T;
+    const float2* w = wc + (size_t)t * T;
+#pragma unroll 4
+    for (int j = 0; j < T; ++j) {
+      const float4 gj = g[j];
+      const float2 wj = w[j];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float q = __builtin_fmaf(m2rx[r], gj.x,
+                        __builtin_fmaf(m2ry[r], gj.y,
+                        __builtin_fmaf(m2rz[r], gj.z, gj.w)));
+        const float W = q * alpha[r];
+        const float tt = __builtin_fmaf(-W * beta[r], W, W);
+        float sn, cs;
+        sincos_rev_f32(tt, sn, cs);
+        const float idd = __builtin_fmaf(-tt, gam[r], invd[r]);
+        const float pr = idd * cs;
+        const float pi = idd * sn;
+        if constexpr (SIGN > 0) {
+          sr[r] = __builtin_fmaf(wj.x, pr, __builtin_fmaf(-wj.y, pi, sr[r]));
+          si[r] = __builtin_fmaf(wj.x, pi, __builtin_fmaf(wj.y, pr, si[r]));
+        } else {
+          sr[r] = __builtin_fmaf(wj.x, pr, __builtin_fmaf(wj.y, pi, sr[r]));
+          si[r] = __builtin_fmaf(-wj.x, pi, __builtin_fmaf(wj.y, pr, si[r]));
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double xr = sr[r], xi = si[r];
+      const double cr = ct[r], cs = SIGN > 0 ? (double)st[r] : -(double)st[r];
+      ar[r] += xr * cr - xi * cs;
+      ai[r] += xr * cs + xi * cr;
+    }
+  }
+  double2* dst = out + (size_t)blockIdx.y * M;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = base + r * kBlock;
+    if (m < M) dst[m] = make_double2(ar[r], ai[r]);
+  }
+}
+
+template <int SIGN, int T>
+static void launch_f32s_r(int R, dim3 grid, const TileSet& ts,
+                          const float2* wc, uint32_t chunk, const double* v,
+                          size_t M, double lambda, double2* out,
+                          hipStream_t stream) {
+  const double il = 1.0 / lambda;
+  switch (R) {
+    case 1:
+      k_superpose_f32s<SIGN, 1, T><<<grid, kBlock, 0, stream>>>(
+          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+      break;
+    case 4:
+      k_superpose_f32s<SIGN, 4, T><<<grid, kBlock, 0, stream>>>(
+          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+      break;
+    default:
+      k_superpose_f32s<SIGN, 2, T><<<grid, kBlock, 0, stream>>>(
+          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+      break;
+  }
+}
+
+hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
+                                 const float2* wc, const double* v, size_t M,
+                                 double lambda, double2* y, double2* partial,
+                                 size_t partial_cap, hipStream_t stream) {
+  if (M == 0) return hipSuccess;
+  int R = tuning().r_f32;
+  if (R != 1 && R != 2 && R != 4) R = 2;
+  int S = ts.n_tiles ? choose_splits(M, R, ts.n_tiles, 64) : 1;
+  while (S > 1 && (size_t)S * M > partial_cap) --S;
+  const uint32_t chunk = ts.n_tiles ? (ts.n_tiles + S - 1) / S : 0;
+  dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
+            (unsigned)S);
+  double2* out = S > 1 ? partial : y;
+  if (ts.tile == 16) {
+    if (sign > 0) launch_f32s_r<+1, 16>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+    else launch_f32s_r<-1, 16>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+  } else if (ts.tile == 8) {
+    if (sign > 0) launch_f32s_r<+1, 8>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+    else launch_f32s_r<-1, 8>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+  } else if (ts.tile == 32) {
+    if (sign > 0) launch_f32s_r<+1, 32>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+    else launch_f32s_r<-1, 32>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return reduce_if_split(S, M, partial, y, stream);
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_weights(
+    int wf, const double* __restrict__ w0, const double* __restrict__ w1,
+    const int32_t* __restrict__ perm, uint32_t n_slots,
+    float2* __restrict__ wc) {
+  const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n_slots) return;
+  const int32_t n = perm[s];
+  float2 r = make_float2(0.f, 0.f);
+  if (n >= 0) {
+    if (wf == kPolar) {
+      double sn, cs;
+      sincos(w1[n], &sn, &cs);
+      r = make_float2((float)(w0[n] * cs), (float)(w0[n] * sn));
+    } else {
+      r = make_float2((float)w0[2 * (size_t)n], (float)w0[2 * (size_t)n + 1]);
+    }
+  }
+  wc[s] = r;
+}
+
+hipError_t launch_tile_weights(WeightForm wf, const double* w0,
+                               const double* w1, const int32_t* perm,
+                               size_t n_slots, float2* wc,
+                               hipStream_t stream) {
+  if (!n_slots) return hipSuccess;
+  const unsigned blocks = (unsigned)((n_slots + kBlock - 1) / kBlock);
+  k_tile_weights<<<blocks, kBlock, 0, stream>>>((int)wf, w0, w1, perm,
+                                               (uint32_t)n_slots, wc);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Monte Carlo: target m draws K indices n_{m,k} from Philox4x32-10 keyed by
+// the seed with counter (k/2, m, 0, kMcStream); words (x,y) serve even k,
+// (z,w) odd k; Lemire multiply-shift maps 64 bits into [0, N).
+
+template <int SIGN, bool FP64>
+__global__ __launch_bounds__(kBlock) void k_mc(
+    const double4* __restrict__ pos, const double* __restrict__ ph,
+    uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
+    uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
+    double scale, double2* __restrict__ y) {
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= M) return;
+  const double vx = v[3 * (size_t)m], vy = v[3 * (size_t)m + 1],
+               vz = v[3 * (size_t)m + 2];
+  double ar = 0.0, ai = 0.0;
+  float fr = 0.f, fi = 0.f;
+  for (uint32_t k = 0; k < K; k += 2) {
+    const U4 o = philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, seed_lo, seed_hi);
+    const uint32_t idx[2] = {lemire_index(o.x, o.y, N), lemire_index(o.z, o.w, N)};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && k + 1 >= K) break;
+      const double4 p = pos[idx[h]];
+      if constexpr (FP64) {
+        const PairF64 q = pair_f64<SIGN>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
+        double s, c;
+        hm_sincos(q.arg, s, c);
+        ar = __builtin_fma(q.r, c, ar);
+        ai = __builtin_fma(q.r, s, ai);
+      } else {
+        const double ddx = vx - p.x, ddy = vy - p.y, ddz = vz - p.z;
+        double d, idd;
+        hm_sqrt_inv(__builtin_fma(ddz, ddz, __builtin_fma(ddy, ddy, ddx * ddx)), d, idd);
+        const float f = (float)frac_cycle(d * inv_lambda);
+        float sn, cs;
+        sincos_rev_f32(f, sn, cs);
+        if (SIGN < 0) sn = -sn;
+        const float id = (float)idd;
+        const float2 w = __builtin_bit_cast(float2, p.w);
+        const float pr = id * cs, pi = id * sn;
+        fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
+        fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+      }
+    }
+    if constexpr (!FP64) {
+      if ((k & 31) == 30 || k + 2 >= K) {  // flush FP32 partials every 32 draws
+        ar += fr;
+        ai += fi;
+        fr = 0.f;
+        fi = 0.f;
+      }
+    }
+  }
+  y[m] = make_double2(ar * scale, ai * scale);
+}
+
+hipError_t launch_mc(int sign, bool fp64, const double4* pos,
+                     const double* ph, size_t N, const double* v, size_t M,
+                     uint32_t K, uint64_t seed, double K_wave, double lambda,
+                     double2* y, hipStream_t stream) {
+  if (!M) return hipSuccess;
+  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
+  const double scale = (double)N / (double)K;
+  const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
+  const double il = 1.0 / lambda;
+  if (sign > 0) {
+    if (fp64) k_mc<+1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+    else k_mc<+1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+  } else {
+    if (fp64) k_mc<-1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+    else k_mc<-1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+  }
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_mc_pack(
+    int fp64, const double* __restrict__ amp, const double* __restrict__ phase,
+    const double* __restrict__ u, uint32_t N, double4* __restrict__ pos,
+    double* __restrict__ ph) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double4 p;
+  p.x = u[3 * (size_t)n];
+  p.y = u[3 * (size_t)n + 1];
+  p.z = u[3 * (size_t)n + 2];
+  if (fp64) {
+    p.w = amp[n];
+    ph[n] = phase[n];
+  } else {
+    double sn, cs;
+    sincos(phase[n], &sn, &cs);
+    const float2 w = make_float2((float)(amp[n] * cs), (float)(amp[n] * sn));
+    p.w = __builtin_bit_cast(double, w);
+  }
+  pos[n] = p;
+}
+
+hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
+                          const double* u, size_t N, double4* pos, double* ph,
+                          hipStream_t stream) {
+  if (!N) return hipSuccess;
+  const unsigned blocks = (unsigned)((N + kBlock - 1) / kBlock);
+  k_mc_pack<<<blocks, kBlock, 0, stream>>>(fp64 ? 1 : 0, amp, phase, u,
+                                          (uint32_t)N, pos, ph);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_mc_indices(
+    uint32_t seed_lo, uint32_t seed_hi, uint64_t N, uint32_t m0,
+    uint32_t count, uint32_t K, int64_t* __restrict__ idx) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= count) return;
+  const uint32_t m = m0 + j;
+  for (uint32_t k = 0; k < K; k += 2) {
+    const U4 o = philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, seed_lo, seed_hi);
+    idx[(size_t)j * K + k] = lemire_index(o.x, o.y, N);
+    if (k + 1 < K) idx[(size_t)j * K + k + 1] = lemire_index(o.z, o.w, N);
+  }
+}
+
+hipError_t launch_mc_indices(uint64_t seed, uint64_t N, size_t m0,
+                             size_t count, uint32_t K, int64_t* idx,
+                             hipStream_t stream) {
+  if (!count || !K) return hipSuccess;
+  const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+  k_mc_indices<<<blocks, kBlock, 0, stream>>>((uint32_t)seed,
+                                             (uint32_t)(seed >> 32), N,
+                                             (uint32_t)m0, (uint32_t)count, K,
+                                             idx);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// time_transform post-processing (cuda/transform.cu:106-139, 400-405):
+// max |y| (order-free, so deterministic), then polar(y / max).
+
+constexpr unsigned kMaxBlocks = 1024;
+
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off));
+  return x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_abs(
+    const double2* __restrict__ y, uint32_t M, double* __restrict__ block_max) {
+  __shared__ double red[kBlock / 64];
+  double mx = 0.0;
+  for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < M;
+       m += gridDim.x * kBlock) {
+    const double2 z = y[m];
+    mx = fmax(mx, hypot(z.x, z.y));
+  }
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) b = fmax(b, red[i]);
+    block_max[blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_final_max(
+    const double* __restrict__ block_max, uint32_t nb,
+    double* __restrict__ max_out) {
+  __shared__ double red[kBlock / 64];
+  double mx = 0.0;
+  for (uint32_t i = threadIdx.x; i < nb; i += kBlock) mx = fmax(mx, block_max[i]);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) b = fmax(b, red[i]);
+    max_out[0] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_polar(
+    const double2* __restrict__ y, uint32_t M,
+    const double* __restrict__ max_in, double* __restrict__ amp,
+    double* __restrict__ phase) {
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= M) return;
+  const double mx = max_in[0];
+  const double div = mx < 1e-6 ? 1.0 : mx;
+  const double2 z = y[m];
+  const double re = z.x / div, im = z.y / div;
+  amp[m] = hypot(re, im);
+  phase[m] = atan2(im, re);
+}
+
+hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
+                                  double* phase, double* max_out,
+                                  double* scratch, hipStream_t stream) {
+  if (!M) return hipSuccess;
+  unsigned nb = (unsigned)std::min<size_t>((M + kBlock - 1) / kBlock, kMaxBlocks);
+  k_max_abs<<<nb, kBlock, 0, stream>>>(y, (uint32_t)M, scratch);
+  k_final_max<<<1, kBlock, 0, stream>>>(scratch, nb, max_out);
+  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
+  k_to_polar<<<blocks, kBlock, 0, stream>>>(y, (uint32_t)M, max_out, amp, phase);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+
+// Zero fill with a kernel rather than hipMemsetAsync, so that device-pointer
+// calls stay safe to record into a HIP graph.
+__global__ __launch_bounds__(kBlock) void k_zero(double* __restrict__ x,
+                                                 size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kBlock)
+    x[i] = 0.0;
+}
+
+hipError_t launch_zero(double* x, size_t n, hipStream_t stream) {
+  if (!n) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+  k_zero<<<blocks, kBlock, 0, stream>>>(x, n);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_adjoint_epilogue(
+    const double2* __restrict__ b, const double* __restrict__ amp,
+    const double* __restrict__ theta, uint32_t N,
+    double* __restrict__ grad_amp, double* __restrict__ grad_theta) {
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double s, c;
+  sincos(theta[n], &s, &c);
+  const double2 z = b[n];
+  const double pr = c * z.x + s * z.y;
+  const double pi = c * z.y - s * z.x;
+  grad_amp[n] = pr;
+  grad_theta[n] = amp[n] * pi;
+}
+
+hipError_t launch_adjoint_epilogue(const double2* b, const double* amp,
+                                   const double* theta, size_t N,
+                                   double* grad_amp, double* grad_theta,
+                                   hipStream_t stream) {
+  if (!N) return hipSuccess;
+  const unsigned blocks = (unsigned)((N + kBlock - 1) / kBlock);
+  k_adjoint_epilogue<<<blocks, kBlock, 0, stream>>>(b, amp, theta, (uint32_t)N,
+                                                   grad_amp, grad_theta);
+  return hipGetLastError();
+}
+
+}  // namespace holo

@@ -1,0 +1,180 @@
+/*
+ * holo.h -- C ABI of the MI355X (gfx950) Huygens superposition library
+ * (libholo.so).
+ *
+ * The library computes the point-source superposition of
+ * voschezang/Holographic-Projector-Simulations:
+ *
+ *     y_m = sum_n  a_n / d_nm * exp(i (phi_n + sign * 2 pi d_nm / lambda)),
+ *     d_nm = |v_m - u_n|_2
+ *
+ * (cuda/superposition.cu:35-51), with sign = +1 for Direction::Forwards and
+ * -1 for Direction::Backwards (cuda/superposition.cu:19).
+ *
+ * Conventions
+ *  - Positions are AoS xyz doubles, element i at [3i, 3i+1, 3i+2]
+ *    (cuda/macros.h:48-56 Ix(i, dim)).
+ *  - Fields are interleaved complex doubles (re, im), like
+ *    std::vector<cuDoubleComplex> (cuda/macros.h:40 WAVE).
+ *  - Without HOLO_DEVICE_POINTERS every array is host memory and the call
+ *    returns after the result is in host memory.  With it every array is
+ *    device memory on the context's device and the call is ordered on the
+ *    context's stream without synchronising.
+ *  - Errors are returned as holo_status; the thread's last message is
+ *    available from holo_last_error().  Nothing calls exit() or abort().
+ *  - N and M must be below 2^31.  A source and a target at the same point
+ *    (d = 0) yield a non-finite term, as in the reference.
+ *  - One context per host thread; calls on one context are not reentrant.
+ */
+#ifndef HOLO_H
+#define HOLO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HOLO_ABI_VERSION 1
+
+typedef enum holo_status {
+  HOLO_OK = 0,
+  HOLO_EINVAL = 1,  /* invalid argument: size, sign, lambda, null pointer, precision */
+  HOLO_ENOMEM = 2,  /* device or host allocation failed */
+  HOLO_EDEVICE = 3, /* HIP runtime error or no usable gfx950 device */
+  HOLO_EIO = 4      /* file read/write error (holo_read_dat / holo_write_dat) */
+} holo_status;
+
+typedef enum holo_precision {
+  HOLO_FP64 = 0,      /* plain FP64: the drop-in; matches the FP64 CPU formulation */
+  HOLO_FP32_SPLIT = 1 /* FP32 pair math on FP64 per-tile references (~1e-7 rel. L2) */
+} holo_precision;
+
+#define HOLO_FORWARDS (+1)  /* phase + 2 pi d / lambda (Direction::Forwards) */
+#define HOLO_BACKWARDS (-1) /* phase - 2 pi d / lambda (Direction::Backwards) */
+
+#define HOLO_DEVICE_POINTERS 0x1u /* all array arguments are device pointers */
+
+typedef struct holo_ctx holo_ctx;
+typedef struct holo_plan holo_plan;
+
+/* ---- library / context ------------------------------------------------- */
+
+int holo_abi_version(void);
+const char *holo_status_string(holo_status status);
+/* Message of the last failing call on this thread ("" if none). */
+const char *holo_last_error(void);
+
+/* Create a context on HIP device `device` using `stream` (a hipStream_t, or
+ * NULL for a stream owned by the context).  Replaces the per-call stream and
+ * cuBLAS-handle creation of cuda/transform.cu:233-241. */
+holo_status holo_create(holo_ctx **ctx, int device, void *stream);
+void holo_destroy(holo_ctx *ctx);
+holo_status holo_set_stream(holo_ctx *ctx, void *stream);
+holo_status holo_synchronize(holo_ctx *ctx);
+
+/* Tuning knobs (process-wide): "r_f64", "r_f32" targets per lane (1, 2, 4),
+ * "splits" source splits (0 = automatic), "target_waves", "f64_libm" (0/1:
+ * use the device libm sqrt/div/sincos in the FP64 kernel).  Results do not
+ * depend on them beyond summation order.  Replaces the blockDim/gridDim/
+ * thread_size/n_streams hints of cuda/input.h:116-122. */
+holo_status holo_set_option(const char *name, long value);
+long holo_get_option(const char *name);
+
+/* ---- brute force: transform_full ----------------------------------------
+ * Replaces transform_full<Direction, Algorithm, shared_memory>(x, u, v,
+ * Geometry) (cuda/transform.cu:147-340): y[m] for m < M, unnormalised.
+ * amp, phase: N doubles; u: 3N; v: 3M; y: 2M (interleaved complex). */
+holo_status holo_transform_full(holo_ctx *ctx, holo_precision precision,
+                                int sign, double lambda, const double *amp,
+                                const double *phase, const double *u,
+                                size_t N, const double *v, size_t M,
+                                double *y, unsigned flags);
+
+/* ---- Monte Carlo: transform ---------------------------------------------
+ * The reference's transform() (named in README.md:127, absent from the
+ * snapshot; remnants cuda/kernel.cu:101-106,197-212): each target m draws K
+ * source indices n_{m,k} uniformly with replacement and
+ *     y[m] = (N / K) sum_k pair(n_{m,k}, m).
+ * Indices come from Philox4x32-10 keyed by `seed`, counter
+ * (k/2, m_lo, m_hi, 0x4d43); words (0,1) serve even k, (2,3) odd k; the
+ * 64-bit word r maps to floor(r N / 2^64).  They depend only on
+ * (seed, m, k, N), not on the launch. */
+holo_status holo_transform_mc(holo_ctx *ctx, holo_precision precision,
+                              int sign, double lambda, const double *amp,
+                              const double *phase, const double *u, size_t N,
+                              const double *v, size_t M, uint32_t K,
+                              uint64_t seed, double *y, unsigned flags);
+
+/* Sample indices n_{m,k} for targets m0 .. m0+count-1 (row-major count x K,
+ * int64), as drawn on the device by holo_transform_mc. */
+holo_status holo_mc_indices(holo_ctx *ctx, uint64_t seed, uint64_t N,
+                            size_t m0, size_t count, uint32_t K,
+                            int64_t *idx, unsigned flags);
+
+/* ---- time_transform post-processing --------------------------------------
+ * normalize_amp + polar conversion of cuda/transform.cu:106-139, 376,
+ * 400-405: max_amp = max |y|; if max_amp >= 1e-6, y /= max_amp; then
+ * amp = |y|, phase = atan2(im, re).  *max_amp receives the maximum (host
+ * pointer, or device pointer with HOLO_DEVICE_POINTERS; may be NULL). */
+holo_status holo_normalize_polar(holo_ctx *ctx, const double *y, size_t M,
+                                 double *amp, double *phase, double *max_amp,
+                                 unsigned flags);
+
+/* ---- plans: a fixed source geometry reused across calls -----------------
+ * A plan keeps the N source positions resident on the device and, for
+ * HOLO_FP32_SPLIT, groups them into spatial tiles of `tile` sources
+ * (8, 16 or 32; 0 = 16).  No reference counterpart: it replaces the
+ * per-call host-to-device copies of cuda/transform.cu:209-213. */
+holo_status holo_plan_create(holo_ctx *ctx, holo_plan **plan, const double *u,
+                             size_t N, unsigned tile, unsigned flags);
+void holo_plan_destroy(holo_plan *plan);
+/* n_sources, n_tiles, tile, and the largest source offset from its tile
+ * centre (metres); any pointer may be NULL. */
+holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
+                           size_t *n_tiles, unsigned *tile,
+                           double *max_offset);
+
+/* Polar weights (amp, phase: N each), as holo_transform_full. */
+holo_status holo_plan_transform(holo_ctx *ctx, const holo_plan *plan,
+                                holo_precision precision, int sign,
+                                double lambda, const double *amp,
+                                const double *phase, const double *v, size_t M,
+                                double *y, unsigned flags);
+
+/* Complex weights w (interleaved, N): y[m] = sum_n w_n exp(i sign d K)/d. */
+holo_status holo_plan_transform_complex(holo_ctx *ctx, const holo_plan *plan,
+                                        holo_precision precision, int sign,
+                                        double lambda, const double *w,
+                                        const double *v, size_t M, double *y,
+                                        unsigned flags);
+
+/* Gradient of a real loss L through z = transform(amp, theta) with sources
+ * u (N) and targets the plan's points (M): given
+ * g_m = dL/dRe z_m + i dL/dIm z_m (interleaved, M),
+ *     b_n            = sum_m g_m conj(G_mn),  G_mn = exp(i sign d K)/d
+ *     grad_amp[n]    = Re(exp(-i theta_n) b_n)
+ *     grad_theta[n]  = amp_n Im(exp(-i theta_n) b_n)
+ * `plan_targets` tiles the forward targets (they are the adjoint's sources);
+ * `sign` is the forward sign.  No reference counterpart. */
+holo_status holo_plan_adjoint(holo_ctx *ctx, const holo_plan *plan_targets,
+                              holo_precision precision, int sign,
+                              double lambda, const double *g,
+                              const double *amp, const double *theta,
+                              const double *u, size_t N, double *grad_amp,
+                              double *grad_theta, unsigned flags);
+
+/* ---- .dat files -----------------------------------------------------------
+ * Headerless little-endian float64 arrays (cuda/util.h:248-283 read_bytes,
+ * 295-337 write_bytes).  holo_read_dat: *count receives the number of
+ * doubles; with buf == NULL only the count is returned. */
+holo_status holo_read_dat(const char *path, double *buf, size_t capacity,
+                          size_t *count);
+holo_status holo_write_dat(const char *path, const double *buf, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HOLO_H */

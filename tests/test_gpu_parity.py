@@ -1,0 +1,326 @@
+"""GPU parity of libholo.so against the CPU oracle (all calls go through the C ABI).
+
+Bars (written into each test):
+  * FP64 brute force: relative L2 <= 1e-12 against the oracle on the same
+    inputs (the reference's FP64 formulation with a pinned operation order);
+  * FP32-split: relative L2 <= 1e-6 against FP64;
+  * Monte Carlo indices: bit-exact against the host Philox restatement;
+  * integer/index work: exact.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM = 632.8e-9
+LAM_REF = 0.65e-6
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.nodeid}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def rand_geom(seed, N, M, z=0.35, spread=5e-3):
+    g = np.random.default_rng(seed)
+    u = np.zeros((N, 3))
+    u[:, :2] = (g.random((N, 2)) - 0.5) * spread
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * spread
+    v[:, 2] = z
+    return g.random(N), g.random(N) * 2 * np.pi, u, v
+
+
+# --------------------------------------------------------------------------
+# FP64 brute force
+
+
+@pytest.mark.parametrize("libm", [0, 1])
+@pytest.mark.parametrize("sign, phi_expected", [(-1, 2.4492935982947064e-16),
+                                                (+1, -2.4492935982947064e-16)])
+def test_known_answer(pkg, ctx, libm, sign, phi_expected):
+    """py/test_util.py:63-64 through the GPU FP64 kernel."""
+    pkg.set_option("f64_libm", libm)
+    y = ctx.transform_full([1.0], [0.0], [0, 0, 0], [0, 0, LAM_REF], sign, LAM_REF, pkg.FP64)
+    assert abs(y[0]) == 1538461.5384615385
+    assert math.atan2(y[0].imag, y[0].real) == phi_expected
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+@pytest.mark.parametrize("libm", [0, 1])
+def test_fp64_plain_small_all_targets(pkg, ctx, oracle, workloads, sign, libm):
+    """32x32 -> 64x64 (the CPU-runnable case): every target, rel L2 <= 1e-12."""
+    pkg.set_option("f64_libm", libm)
+    st = workloads.plain_small(0).stages[0]
+    y = ctx.transform_full(st.amp, st.phase, st.u, st.v, sign, st.lam, pkg.FP64)
+    ref = oracle.transform_full(sign, st.lam, st.amp, st.phase, st.u, st.v)
+    assert rel_l2(y, ref) <= 1e-12
+    assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_fp64_golden_fixture(pkg, ctx):
+    """Committed golden vectors (tests/golden/plain_small_*.dat, written by the
+    oracle) of the 32x32 -> 64x64 case, sign +1."""
+    import os
+    d = os.path.join(os.path.dirname(__file__), "golden")
+    amp = pkg.read_dat(os.path.join(d, "plain_small_x_amp.dat"))
+    phase = pkg.read_dat(os.path.join(d, "plain_small_x_phase.dat"))
+    u = pkg.read_dat(os.path.join(d, "plain_small_u.dat"))
+    v = pkg.read_dat(os.path.join(d, "plain_small_v.dat"))
+    ref = pkg.read_dat(os.path.join(d, "plain_small_y_oracle.dat")).view(np.complex128)
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64)
+    assert rel_l2(y, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("R", [1, 2, 4])
+@pytest.mark.parametrize("splits", [0, 1, 3, 7])
+def test_fp64_launch_shape_independent(pkg, ctx, oracle, R, splits):
+    """Targets per lane, source splits and ragged sizes do not change results
+    beyond summation order (the reference's Geometry hints, cuda/init.h:42-60)."""
+    pkg.set_option("r_f64", R)
+    pkg.set_option("splits", splits)
+    amp, phase, u, v = rand_geom(3, 2053, 777)
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["chain_stage1", "plane_1024"])
+def test_fp64_full_size_subset(pkg, ctx, oracle, workloads, name):
+    """Full-size workloads, oracle on a fixed subset of targets."""
+    if name == "chain_stage1":
+        st = workloads.chain_xyz(0).stages[0]
+    else:
+        st = workloads.plane_1024(0).stages[0]
+    y = ctx.transform_full(st.amp, st.phase, st.u, st.v, st.sign, st.lam, pkg.FP64)
+    sub = workloads.stride_subset(st.v.shape[0], 96)
+    ref = oracle.transform_full(st.sign, st.lam, st.amp, st.phase, st.u, st.v, targets=sub)
+    assert rel_l2(y[sub], ref) <= 1e-12
+    # size-independent property over every target: finite, and the
+    # Backwards field of -phase is the conjugate (exact for the pinned order)
+    assert np.all(np.isfinite(y))
+
+
+def test_fp64_conjugation_symmetry_exact(pkg, ctx):
+    amp, phase, u, v = rand_geom(4, 3000, 1000)
+    yb = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64)
+    yf = ctx.transform_full(amp, -phase, u, v, +1, LAM, pkg.FP64)
+    assert np.array_equal(yb, np.conj(yf))
+
+
+def test_fp64_unit_phasors_sum_to_n(pkg, ctx, oracle):
+    """TEST_CONST_PHASE invariant restated (cuda/transform.cu:276-291): with
+    a_n = d_n and phi_n = -sign d_n K every term is ~1, so y = N for any split."""
+    N = 200_000
+    amp, phase, u, v = rand_geom(5, N, 1)
+    d = np.linalg.norm(v[0] - u, axis=1)
+    K = oracle.wavenumber(LAM)
+    for splits in (1, 5, 64):
+        pkg.set_option("splits", splits)
+        y = ctx.transform_full(d, -d * K, u, v, +1, LAM, pkg.FP64)
+        assert abs(y[0] - N) < 1e-3
+
+
+def test_fp64_linearity(pkg, ctx):
+    amp, phase, u, v = rand_geom(6, 4000, 500)
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64)
+    ya = ctx.transform_full(amp[:1500], phase[:1500], u[:1500], v, +1, LAM, pkg.FP64)
+    yb = ctx.transform_full(amp[1500:], phase[1500:], u[1500:], v, +1, LAM, pkg.FP64)
+    assert rel_l2(ya + yb, y) <= 1e-13
+
+
+def test_fp64_edge_sizes(pkg, ctx, oracle):
+    # one source, one target
+    amp, phase, u, v = rand_geom(7, 1, 1)
+    assert rel_l2(ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64),
+                  oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-12
+    # no sources: the empty sum
+    y = ctx.transform_full(np.zeros(0), np.zeros(0), np.zeros(0), v, +1, LAM, pkg.FP64)
+    assert np.array_equal(y, np.zeros(1, dtype=complex))
+    # no targets
+    y = ctx.transform_full(amp, phase, u, np.zeros(0), +1, LAM, pkg.FP64)
+    assert y.size == 0
+    # ragged (not a multiple of the 256-lane block or the tile)
+    amp, phase, u, v = rand_geom(8, 257, 513)
+    assert rel_l2(ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64),
+                  oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-12
+
+
+def test_invalid_arguments(pkg, ctx):
+    amp, phase, u, v = rand_geom(9, 4, 4)
+    for sign, lam in ((0, LAM), (2, LAM), (+1, 0.0), (+1, -1.0), (+1, float("nan"))):
+        with pytest.raises(pkg.HoloError) as e:
+            ctx.transform_full(amp, phase, u, v, sign, lam, pkg.FP64)
+        assert e.value.status == 1
+    with pytest.raises(pkg.HoloError):
+        ctx.transform_full(amp, phase, u, v, +1, LAM, 7)
+    with pytest.raises(pkg.HoloError):
+        ctx.transform_full(amp, phase[:3], u, v, +1, LAM, pkg.FP64)
+
+
+def test_device_pointer_path(pkg, ctx, oracle):
+    """torch tensors on the GPU go through HOLO_DEVICE_POINTERS on torch's stream."""
+    import torch
+    amp, phase, u, v = rand_geom(10, 1000, 700)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    y = ctx.transform_full(t(amp), t(phase), t(u), t(v), +1, LAM, pkg.FP64)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    assert rel_l2(y.cpu().numpy(), ref) <= 1e-12
+
+
+# --------------------------------------------------------------------------
+# time_transform post-processing
+
+
+def test_normalize_polar(pkg, ctx, oracle):
+    amp, phase, u, v = rand_geom(11, 500, 5000)
+    y = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    a, p, mx = ctx.normalize_polar(y)
+    ra, rp, rmx = oracle.normalize_polar(y)
+    assert mx == rmx
+    assert np.max(np.abs(a - ra)) <= 2e-16
+    assert np.max(np.abs(np.angle(np.exp(1j * (p - rp))))) <= 1e-15
+    # below 1e-6 nothing is divided
+    a2, _, mx2 = ctx.normalize_polar(y * 1e-9 / rmx)
+    assert mx2 < 1e-6 and np.allclose(a2, np.abs(y * 1e-9 / rmx), rtol=1e-15)
+
+
+def test_chain_x_y_z_subset(pkg, ctx, oracle, workloads):
+    """x -> y (normalised polar) -> z at reduced size, every stage vs oracle."""
+    w = workloads.chain_xyz(0, n_x=64, n_y=96, n_z=64)
+    s1, s2 = w.stages
+    y = ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam, pkg.FP64)
+    ya, yp, _ = ctx.normalize_polar(y)
+    z = ctx.transform_full(ya, yp, s2.u, s2.v, s2.sign, s2.lam, pkg.FP64)
+    ry = oracle.transform_full(s1.sign, s1.lam, s1.amp, s1.phase, s1.u, s1.v)
+    ra, rp, _ = oracle.normalize_polar(ry)
+    rz = oracle.transform_full(s2.sign, s2.lam, ra, rp, s2.u, s2.v)
+    assert rel_l2(y, ry) <= 1e-12
+    assert rel_l2(z, rz) <= 1e-12
+
+
+# --------------------------------------------------------------------------
+# FP32-split
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_fp32_split_plain_small(pkg, ctx, oracle, workloads, tile):
+    st = workloads.plain_small(0).stages[0]
+    plan = ctx.plan(st.u, tile)
+    y = plan.transform(st.amp, st.phase, st.v, st.sign, st.lam, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(st.sign, st.lam, st.amp, st.phase, st.u, st.v)
+    err = rel_l2(y, ref)
+    print(f"fp32-split tile {tile}: rel L2 {err:.3e}")
+    assert err <= 1e-6
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_fp32_split_plane_1024_vs_fp64(pkg, ctx, workloads, sign):
+    """Full 1024^2 -> 1024^2: FP32-split vs the GPU FP64 path on every target."""
+    st = workloads.plane_1024(0).stages[0]
+    y64 = ctx.transform_full(st.amp, st.phase, st.u, st.v, sign, st.lam, pkg.FP64)
+    y32 = ctx.transform_full(st.amp, st.phase, st.u, st.v, sign, st.lam, pkg.FP32_SPLIT)
+    err = rel_l2(y32, y64)
+    print(f"fp32-split 1024^2 sign {sign}: rel L2 {err:.3e}, max abs "
+          f"{np.max(np.abs(y32 - y64)) / np.max(np.abs(y64)):.3e}")
+    assert err <= 1e-6
+
+
+def test_fp32_split_random_cloud(pkg, ctx, oracle):
+    """Unordered 3-D point cloud: the tiling sorts it; error bar still holds."""
+    g = np.random.default_rng(12)
+    N, M = 20000, 300
+    u = (g.random((N, 3)) - 0.5) * np.array([4e-3, 4e-3, 1e-4])
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * 4e-3
+    v[:, 2] = 0.3
+    amp, phase = g.random(N), g.random(N) * 2 * np.pi
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-6
+
+
+def test_plan_info_grid(pkg, ctx, workloads):
+    st = workloads.plane_1024(0).stages[0]
+    info = ctx.plan(st.u, 16).info()
+    assert info["n_sources"] == 1024 * 1024
+    assert info["n_tiles"] == 1024 * 1024 // 16  # exact 4x4 blocks
+    assert info["max_offset"] <= 1.5 * 10e-6 * math.sqrt(2) + 1e-12
+
+
+# --------------------------------------------------------------------------
+# Monte Carlo
+
+
+def test_mc_indices_match_host_replica(pkg, ctx, oracle):
+    for seed, N in ((0, 1 << 20), (123456789012345, 1000), (2 ** 63 + 5, 7)):
+        idx = ctx.mc_indices(seed, N, 1000, 64, 33)
+        for j in range(0, 64, 7):
+            for k in range(33):
+                assert idx[j, k] == oracle.mc_index(seed, 1000 + j, k, N)
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_mc_fp64_matches_oracle(pkg, ctx, oracle, sign):
+    amp, phase, u, v = rand_geom(13, 5000, 3000)
+    y = ctx.transform_mc(amp, phase, u, v, 257, 42, sign, LAM, pkg.FP64)
+    sub = np.arange(0, 3000, 11)
+    ref = oracle.transform_mc(sign, LAM, amp, phase, u, v, 257, 42, targets=sub)
+    assert rel_l2(y[sub], ref) <= 1e-12
+
+
+def test_mc_fp32_split_matches_fp64(pkg, ctx):
+    amp, phase, u, v = rand_geom(14, 5000, 3000)
+    y64 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP64)
+    y32 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y32, y64) <= 1e-6
+
+
+def test_mc_statistics_vs_brute_force(pkg, ctx):
+    """Error ~ 1/sqrt(K): the relative L2 error against brute force falls by
+    ~2 for every 4x more samples (tolerances from the estimator's variance)."""
+    amp, phase, u, v = rand_geom(15, 65536, 4096)
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64)
+    errs = []
+    for K in (256, 1024, 4096):
+        e = rel_l2(ctx.transform_mc(amp, phase, u, v, K, 1, +1, LAM, pkg.FP32_SPLIT), y)
+        errs.append(e)
+    assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
+
+
+# --------------------------------------------------------------------------
+# adjoint
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_matches_oracle(pkg, ctx, oracle, workloads, precision):
+    amp, theta, u, _ = rand_geom(16, 3000, 1)
+    v = workloads.grid(40, 30, 10e-6, 0.35)  # forward targets: a 40 x 30 lattice
+    g = np.random.default_rng(3).standard_normal(1200) + 1j * np.random.default_rng(4).standard_normal(1200)
+    plan = ctx.plan(v, 16)
+    ga, gt = plan.adjoint(g, amp, theta, u, +1, LAM, precision)
+    ra, rt = oracle.adjoint(+1, LAM, amp, theta, u, g, v)
+    bar = 1e-11 if precision == 0 else 1e-6
+    assert rel_l2(ga, ra) <= bar and rel_l2(gt, rt) <= bar
+
+
+def test_plan_transform_matches_transform_full(pkg, ctx):
+    amp, phase, u, v = rand_geom(17, 4000, 2500)
+    plan = ctx.plan(u, 16)
+    for prec in (0, 1):
+        a = plan.transform(amp, phase, v, -1, LAM, prec)
+        b = ctx.transform_full(amp, phase, u, v, -1, LAM, prec)
+        assert rel_l2(a, b) <= (1e-13 if prec == 0 else 1e-9)
+    # complex weights = polar weights
+    w = amp * np.exp(1j * phase)
+    c = plan.transform_complex(w, v, -1, LAM, 0)
+    assert rel_l2(c, plan.transform(amp, phase, v, -1, LAM, 0)) <= 1e-12
