@@ -1,0 +1,242 @@
+"""Benchmark: Huygens superposition on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-extras] [--skip-cpu]
+
+Timed step: the FP64 brute-force chain x -> y -> z at full size
+(x: 512^2 sources at z=0, 10 um, amplitude = 64 Gaussian spots, phase U[0,2pi);
+ y: 1024^2 projector at z=0.35 m, 7.56 um, sign +1;
+ z: 512^2 plane at z=0, 10 um, sign -1; lambda 632.8 nm), 2 x 2.75e11 pairs,
+with the time_transform normalisation + polar conversion between the stages.
+Inputs are resident in HBM when the timed region starts.
+
+Side measurements printed in the same JSON line: FP32-split vs FP64 on
+1024^2 -> 1024^2 (pairs/s and relative L2 / max-abs error), the Monte Carlo
+estimator for K in {256, 1024, 4096} (pairs/s and error vs brute force), one
+hologram forward + adjoint gradient step 1024^2 -> 512^2 (ms), and the NumPy
+restatement of the reference's CPU path timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "phasor pairs/s (FP64 vs FP32-split @rel err≤1e-6) + ms per adjoint grad step"
+FLOP_PER_PAIR = 29            # cuda/macros.h:71-80
+PEAK_FP64 = 78.6e12           # MI355X FP64 vector (and matrix) peak, FLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def read_traffic(kernel: str):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            try:
+                return float(line.split("=", 1)[1].split()[0])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def cpu_baseline(stage, n_targets: int):
+    """The NumPy restatement of py/util.py:92-147 (map_sum_kernel over a Pool)
+    on a fixed stride subset of stage-1 targets.  Runs before the GPU is
+    initialised, so the worker processes fork from a process without it."""
+    from oracle import numpy_baseline as nb
+    cores = min(16, nb.cpu_count())
+    M = stage.v.shape[0]
+    sub = (np.arange(n_targets, dtype=np.int64) * M) // n_targets
+    x = np.stack([stage.amp, stage.phase], axis=1)
+    y = np.empty((n_targets, 2))
+    t0 = time.perf_counter()
+    # CUDA sign s corresponds to the Python direction -s (py/util.py:106)
+    nb.map_sum_kernel(x, y, stage.u, stage.v[sub], direction=-stage.sign, lam=stage.lam,
+                      parallel=cores)
+    dt = time.perf_counter() - t0
+    N = stage.u.shape[0]
+    res = {"value": N * n_targets / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+           "sample": f"x->y stage, {N} sources x {n_targets} stride-sampled targets "
+                     f"({N * n_targets:.3g} pairs, {dt:.1f} s wall on {cores} processes)"}
+    return res, sub, y[:, 0] * np.exp(1j * y[:, 1])
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--skip-extras", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-targets", type=int, default=1024)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the "
+              "superposition shards by targets with no exchange, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+    holo = __graft_entry__.load_package()
+    import importlib
+    wl = importlib.import_module("holo_amd.workloads")
+    w = wl.chain_xyz(0)
+    s1, s2 = w.stages
+
+    cpu = None
+    if not a.skip_cpu:
+        log(f"[bench] cpu baseline: {a.cpu_targets} targets")
+        cpu = cpu_baseline(s1, a.cpu_targets)
+        log(f"[bench] cpu: {cpu[0]}")
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ctx = holo.Context(0)
+    T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+    # ---------------- main workload: FP64 chain x -> y -> z -----------------
+    amp_x, ph_x, u_x, v_y, w_z = T(s1.amp), T(s1.phase), T(s1.u), T(s1.v), T(s2.v)
+    plan_x = holo.Plan(ctx, u_x)
+    plan_y = holo.Plan(ctx, v_y)
+    M1, M2 = v_y.shape[0], w_z.shape[0]
+    y = torch.empty((M1, 2), dtype=torch.float64, device=dev)
+    z = torch.empty((M2, 2), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    launch_ms = []
+
+    def chain_step(record):
+        if record:
+            ev[0].record(stream)
+        plan_x.transform(amp_x, ph_x, v_y, s1.sign, s1.lam, holo.FP64, out=y)
+        if record:
+            ev[1].record(stream)
+        amp_y, ph_y, _ = ctx.normalize_polar(y)
+        if record:
+            ev[2].record(stream)
+        plan_y.transform(amp_y, ph_y, w_z, s2.sign, s2.lam, holo.FP64, out=z)
+        if record:
+            ev[3].record(stream)
+
+    log(f"[bench] {w.name}: warmup {a.warmup}")
+    for _ in range(a.warmup):
+        chain_step(False)
+    torch.cuda.synchronize(dev)
+    log(f"[bench] timing {a.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        chain_step(True)
+        torch.cuda.synchronize(dev)
+        launch_ms += [ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])]
+    torch.cuda.synchronize(dev)
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / a.steps
+    pairs_step = s1.pairs + s2.pairs
+    value = pairs_step / (dt / a.steps)
+    log(f"[bench] chain: {ms_per_step:.1f} ms/step, {value:.4g} pairs/s")
+
+    pairs_launch = s1.pairs  # both stages: 512^2 x 1024^2 pairs
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    achieved = FLOP_PER_PAIR * pairs_launch / avg_launch_s
+    roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                "traffic": read_traffic("k_superpose_f64"),
+                "kernel": "k_superpose_f64 (+ partial-sum reduce)",
+                "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_launch,
+                "avg_launch_ms": 1e3 * avg_launch_s}
+
+    out = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "FP64 brute-force chain x->y->z: 512^2 -> 1024^2 -> 512^2, "
+                                  "lambda 632.8 nm, z 0 / 0.35 m, pitch 10 / 7.56 um",
+                      "pairs_per_step": pairs_step, "precision": "FP64"},
+           "gflops_29": value * FLOP_PER_PAIR / 1e9,
+           "roofline": roofline}
+
+    # ---------------- FP32-split vs FP64, Monte Carlo, adjoint ---------------
+    if not a.skip_extras:
+        p3 = wl.plane_1024(0).stages[0]
+        amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
+        plan3 = holo.Plan(ctx, u3)
+        y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
+        y32 = torch.empty_like(y64)
+        log("[bench] plane 1024^2: FP64")
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
+        torch.cuda.synchronize(dev)
+        t64 = time.perf_counter() - t0
+        log("[bench] plane 1024^2: FP32-split")
+        plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
+        torch.cuda.synchronize(dev)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
+        torch.cuda.synchronize(dev)
+        t32 = (time.perf_counter() - t0) / reps
+        f64 = torch.view_as_complex(y64).cpu().numpy()
+        f32 = torch.view_as_complex(y32).cpu().numpy()
+        out["fp32_split"] = {
+            "workload": "1024^2 -> 1024^2 (z 0 -> 0.35 m, 10 / 7.56 um, amp U[0,1), phase U[0,2pi))",
+            "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
+            "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
+            "rel_l2_vs_fp64": rel_l2(f32, f64),
+            "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)))}
+        log(f"[bench] fp32-split: {out['fp32_split']}")
+
+        mc = {}
+        for K in (256, 1024, 4096):
+            ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
+            torch.cuda.synchronize(dev)
+            tk = time.perf_counter() - t0
+            e = rel_l2(ymc.cpu().numpy(), f64)
+            mc[str(K)] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk, "rel_l2_vs_brute_force": e}
+        out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0", "K": mc}
+        log(f"[bench] monte carlo: {mc}")
+
+        hol = importlib.import_module("holo_amd.hologram")
+        r = hol.optimise(steps=3, warmup=1)
+        out["adjoint"] = {"workload": "phase-only hologram 1024^2 (7.56 um, z=0) -> 512^2 "
+                                      "(10 um, z=0.35 m), FP32-split forward + adjoint, Adam",
+                          "ms_per_grad_step": r.ms_per_step, "target_ms": 50.0,
+                          "pairs_per_step": 2 * 1024 * 1024 * 512 * 512,
+                          "loss_after_4_steps": r.losses[-1]}
+        log(f"[bench] adjoint: {out['adjoint']}")
+
+    # ---------------- CPU baseline (timed before the GPU work) ----------------
+    if cpu is not None:
+        res, sub, ycpu = cpu
+        # y holds the last chain step's stage-1 field (same inputs every step)
+        yh = torch.view_as_complex(y).cpu().numpy()
+        res["agreement_rel_l2_vs_gpu"] = rel_l2(ycpu, yh[sub])
+        out["cpu_baseline"] = res
+
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

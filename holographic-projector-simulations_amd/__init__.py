@@ -65,6 +65,13 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise HoloError(3, f"{LIB_PATH} is missing; run build.py (or __graft_entry__.build())")
+    # torch ships its own HIP runtime.  Loading it first makes libholo.so's
+    # libamdhip64.so.7 dependency resolve to that same runtime, so one process
+    # never holds two HIP runtimes (the second would find no GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, sz, i, u32, u64, d = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
                               ctypes.c_uint32, ctypes.c_uint64, ctypes.c_double)
@@ -206,7 +213,8 @@ class Context:
         return self._h
 
     def set_stream(self, stream) -> None:
-        check(lib().holo_set_stream(self._h, int(stream)))
+        """Order work on `stream` (a hipStream_t handle; 0 = default stream)."""
+        check(lib().holo_set_stream(self._h, int(stream) or None))
 
     def synchronize(self) -> None:
         check(lib().holo_synchronize(self._h))

@@ -420,7 +420,7 @@ void holo_destroy(holo_ctx* ctx) {
 
 holo_status holo_set_stream(holo_ctx* ctx, void* stream) {
   HOLO_TRY(use_device(ctx));
-  if (!stream) return fail(HOLO_EINVAL, "null stream");
+  if (ctx->own_stream && ctx->stream == static_cast<hipStream_t>(stream)) return HOLO_OK;
   if (ctx->own_stream) {
     HOLO_HIP(hipStreamSynchronize(ctx->stream));
     HOLO_HIP(hipStreamDestroy(ctx->stream));

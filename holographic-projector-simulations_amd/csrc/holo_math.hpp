@@ -55,10 +55,12 @@ HM_HD void hm_sincos(double x, double& s, double& c) {
   c = ((q + 1u) & 2u) ? -co : co;
 }
 
-// d = sqrt(s), correctly rounded, and inv = 1/d to ~1 ulp, for normal s.
-// Goldschmidt iteration from the hardware reciprocal square root, then two
-// Newton corrections of d using h ~ 1/(2 sqrt s) (Markstein's final step) and
-// one more refinement of h.  On the host the seed comes from 1/sqrt(s).
+// d = sqrt(s), correctly rounded, and inv = 1/d to ~2e-15 relative, for
+// normal s.  Goldschmidt iteration from the hardware reciprocal square root,
+// then two Newton corrections of d using h ~ 1/(2 sqrt s) (Markstein's final
+// step) and one more refinement of h.  On gfx950, d matched the host's IEEE
+// sqrt on 1.7e7 random s in [1e-8, 20]; inv is used only for amplitudes.
+// On the host the seed comes from 1/sqrt(s).
 HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
 #pragma clang fp contract(off)
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -71,6 +71,8 @@ const char *holo_last_error(void);
  * cuBLAS-handle creation of cuda/transform.cu:233-241. */
 holo_status holo_create(holo_ctx **ctx, int device, void *stream);
 void holo_destroy(holo_ctx *ctx);
+/* Use `stream` from now on; NULL selects the device's default (null) stream,
+ * as torch's default stream is.  A stream the context created is released. */
 holo_status holo_set_stream(holo_ctx *ctx, void *stream);
 holo_status holo_synchronize(holo_ctx *ctx);
 
