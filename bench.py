@@ -85,7 +85,12 @@ def main():
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-targets", type=int, default=1024)
+    ap.add_argument("--sections", default="chain,fp32,mc,adjoint",
+                    help="comma list of chain, fp32, mc, adjoint (profiling runs pick one)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy 3 times (HBM counter calibration)")
     a = ap.parse_args()
+    sections = set(a.sections.split(","))
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the "
               "superposition shards by targets with no exchange, see DESIGN.md)",
@@ -100,18 +105,104 @@ def main():
     w = wl.chain_xyz(0)
     s1, s2 = w.stages
 
+    if a.skip_extras:
+        sections &= {"chain"}
+    if "chain" not in sections:
+        a.skip_cpu = True
     cpu = None
-    if not a.skip_cpu:
+    if not a.skip_cpu and not a.calibrate_copy:
         log(f"[bench] cpu baseline: {a.cpu_targets} targets")
         cpu = cpu_baseline(s1, a.cpu_targets)
         log(f"[bench] cpu: {cpu[0]}")
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if a.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"calibrate_copy_bytes_read": 1 << 30, "bytes_written": 1 << 30,
+                          "launches": 3}))
+        return
     ctx = holo.Context(0)
     T = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    out = {"metric": METRIC, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic"}
 
     # ---------------- main workload: FP64 chain x -> y -> z -----------------
+    if "chain" in sections:
+        out.update(run_chain(a, holo, ctx, s1, s2, T, dev, cpu, w.name))
+
+    # ---------------- FP32-split vs FP64, Monte Carlo, adjoint ---------------
+    f64 = None
+    if sections & {"fp32", "mc"}:
+        p3 = wl.plane_1024(0).stages[0]
+        amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
+        plan3 = holo.Plan(ctx, u3)
+        y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
+        if "fp32" in sections:
+            y32 = torch.empty_like(y64)
+            log("[bench] plane 1024^2: FP64")
+            torch.cuda.synchronize(dev)
+            t0 = time.perf_counter()
+            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
+            torch.cuda.synchronize(dev)
+            t64 = time.perf_counter() - t0
+            f64 = torch.view_as_complex(y64).cpu().numpy()
+            log("[bench] plane 1024^2: FP32-split")
+            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
+            torch.cuda.synchronize(dev)
+            reps = 3
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
+            torch.cuda.synchronize(dev)
+            t32 = (time.perf_counter() - t0) / reps
+            f32 = torch.view_as_complex(y32).cpu().numpy()
+            out["fp32_split"] = {
+                "workload": "1024^2 -> 1024^2 (z 0 -> 0.35 m, 10 / 7.56 um, amp U[0,1), "
+                            "phase U[0,2pi)), tiles of 16",
+                "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
+                "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
+                "rel_l2_vs_fp64": rel_l2(f32, f64),
+                "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)))}
+            log(f"[bench] fp32-split: {out['fp32_split']}")
+        if "mc" in sections:
+            mc = {}
+            for K in (256, 1024, 4096):
+                ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
+                torch.cuda.synchronize(dev)
+                t0 = time.perf_counter()
+                ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
+                torch.cuda.synchronize(dev)
+                tk = time.perf_counter() - t0
+                mc[str(K)] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk}
+                if f64 is not None:
+                    mc[str(K)]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
+            out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0; "
+                                              "expected rel. error sqrt(N/K) for a random-phase field",
+                                  "K": mc}
+            log(f"[bench] monte carlo: {mc}")
+
+    if "adjoint" in sections:
+        hol = importlib.import_module("holo_amd.hologram")
+        r = hol.optimise(steps=3, warmup=1)
+        out["adjoint"] = {"workload": "phase-only hologram 1024^2 (7.56 um, z=0) -> 512^2 "
+                                      "(10 um, z=0.35 m), FP32-split forward + adjoint, Adam",
+                          "ms_per_grad_step": r.ms_per_step, "target_ms": 50.0,
+                          "pairs_per_step": 2 * 1024 * 1024 * 512 * 512,
+                          "loss_after_4_steps": r.losses[-1]}
+        log(f"[bench] adjoint: {out['adjoint']}")
+
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
+    import torch
     amp_x, ph_x, u_x, v_y, w_z = T(s1.amp), T(s1.phase), T(s1.u), T(s1.v), T(s2.v)
     plan_x = holo.Plan(ctx, u_x)
     plan_y = holo.Plan(ctx, v_y)
@@ -135,7 +226,7 @@ def main():
         if record:
             ev[3].record(stream)
 
-    log(f"[bench] {w.name}: warmup {a.warmup}")
+    log(f"[bench] {name}: warmup {a.warmup}")
     for _ in range(a.warmup):
         chain_step(False)
     torch.cuda.synchronize(dev)
@@ -161,81 +252,19 @@ def main():
                 "kernel": "k_superpose_f64 (+ partial-sum reduce)",
                 "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_launch,
                 "avg_launch_ms": 1e3 * avg_launch_s}
-
-    out = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1,
-           "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_per_step,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f64", "data": "synthetic",
+    res = {"value": value, "unit": "pairs/s", "ms_per_step": ms_per_step,
            "config": {"workload": "FP64 brute-force chain x->y->z: 512^2 -> 1024^2 -> 512^2, "
                                   "lambda 632.8 nm, z 0 / 0.35 m, pitch 10 / 7.56 um",
                       "pairs_per_step": pairs_step, "precision": "FP64"},
            "gflops_29": value * FLOP_PER_PAIR / 1e9,
            "roofline": roofline}
-
-    # ---------------- FP32-split vs FP64, Monte Carlo, adjoint ---------------
-    if not a.skip_extras:
-        p3 = wl.plane_1024(0).stages[0]
-        amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
-        plan3 = holo.Plan(ctx, u3)
-        y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
-        y32 = torch.empty_like(y64)
-        log("[bench] plane 1024^2: FP64")
-        torch.cuda.synchronize(dev)
-        t0 = time.perf_counter()
-        plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
-        torch.cuda.synchronize(dev)
-        t64 = time.perf_counter() - t0
-        log("[bench] plane 1024^2: FP32-split")
-        plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
-        torch.cuda.synchronize(dev)
-        reps = 3
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
-        torch.cuda.synchronize(dev)
-        t32 = (time.perf_counter() - t0) / reps
-        f64 = torch.view_as_complex(y64).cpu().numpy()
-        f32 = torch.view_as_complex(y32).cpu().numpy()
-        out["fp32_split"] = {
-            "workload": "1024^2 -> 1024^2 (z 0 -> 0.35 m, 10 / 7.56 um, amp U[0,1), phase U[0,2pi))",
-            "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
-            "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
-            "rel_l2_vs_fp64": rel_l2(f32, f64),
-            "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)))}
-        log(f"[bench] fp32-split: {out['fp32_split']}")
-
-        mc = {}
-        for K in (256, 1024, 4096):
-            ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
-            torch.cuda.synchronize(dev)
-            t0 = time.perf_counter()
-            ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
-            torch.cuda.synchronize(dev)
-            tk = time.perf_counter() - t0
-            e = rel_l2(ymc.cpu().numpy(), f64)
-            mc[str(K)] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk, "rel_l2_vs_brute_force": e}
-        out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0", "K": mc}
-        log(f"[bench] monte carlo: {mc}")
-
-        hol = importlib.import_module("holo_amd.hologram")
-        r = hol.optimise(steps=3, warmup=1)
-        out["adjoint"] = {"workload": "phase-only hologram 1024^2 (7.56 um, z=0) -> 512^2 "
-                                      "(10 um, z=0.35 m), FP32-split forward + adjoint, Adam",
-                          "ms_per_grad_step": r.ms_per_step, "target_ms": 50.0,
-                          "pairs_per_step": 2 * 1024 * 1024 * 512 * 512,
-                          "loss_after_4_steps": r.losses[-1]}
-        log(f"[bench] adjoint: {out['adjoint']}")
-
-    # ---------------- CPU baseline (timed before the GPU work) ----------------
     if cpu is not None:
-        res, sub, ycpu = cpu
+        cres, sub, ycpu = cpu
         # y holds the last chain step's stage-1 field (same inputs every step)
         yh = torch.view_as_complex(y).cpu().numpy()
-        res["agreement_rel_l2_vs_gpu"] = rel_l2(ycpu, yh[sub])
-        out["cpu_baseline"] = res
-
-    ctx.close()
-    print(json.dumps(out), flush=True)
+        cres["agreement_rel_l2_vs_gpu"] = rel_l2(ycpu, yh[sub])
+        res["cpu_baseline"] = cres
+    return res
 
 
 if __name__ == "__main__":

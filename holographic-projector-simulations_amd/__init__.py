@@ -102,6 +102,11 @@ def lib() -> ctypes.CDLL:
     L.holo_plan_adjoint.argtypes = [vp, vp, i, i, d, vp, vp, vp, vp, sz, vp, vp, ctypes.c_uint]
     L.holo_read_dat.argtypes = [ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)]
     L.holo_write_dat.argtypes = [ctypes.c_char_p, vp, sz]
+    dp = ctypes.POINTER(d)
+    L.holo_host_sincos.argtypes = [d, dp, dp]
+    L.holo_host_sincos.restype = None
+    L.holo_host_sqrt_inv.argtypes = [d, dp, dp]
+    L.holo_host_sqrt_inv.restype = None
     for name in ("holo_create", "holo_set_stream", "holo_synchronize", "holo_set_option",
                  "holo_transform_full", "holo_transform_mc", "holo_mc_indices",
                  "holo_normalize_polar", "holo_plan_create", "holo_plan_info",
@@ -389,6 +394,13 @@ class Plan:
                                       _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt),
                                       DEVICE_POINTERS if device else 0))
         return ga, gt
+
+
+def host_sincos(x: float):
+    """Host build of the FP64 kernel's sin/cos: (sin x, cos x)."""
+    s, c = ctypes.c_double(), ctypes.c_double()
+    lib().holo_host_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
 
 
 # ---------------------------------------------------------------------------

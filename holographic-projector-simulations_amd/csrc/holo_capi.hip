@@ -16,6 +16,7 @@
 
 #include "../../include/holo.h"
 #include "holo_internal.hpp"
+#include "holo_math.hpp"
 
 namespace {
 
@@ -89,6 +90,7 @@ struct holo_plan {
   unsigned tile = 16;
   size_t n_tiles = 0;
   double max_offset = 0.0;
+  bool planar = true;
   DevBuf centre, geo, perm, wc;
 };
 
@@ -183,6 +185,7 @@ struct HostTiles {
   std::vector<float4> geo;
   std::vector<int32_t> perm;
   double max_offset = 0.0;
+  bool planar = true;
 };
 
 void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
@@ -238,6 +241,7 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
   out.geo.clear();
   out.perm.clear();
   out.max_offset = 0.0;
+  out.planar = true;
   size_t i = 0;
   while (i < N) {
     size_t j = i;
@@ -263,6 +267,7 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
           const double dd = (double)fx * fx + (double)fy * fy + (double)fz * fz;
           out.geo.push_back(float4{fx, fy, fz, (float)dd});
           out.perm.push_back(n);
+          if (fz != 0.f) out.planar = false;
           out.max_offset = std::max(out.max_offset, std::sqrt(dd));
         } else {
           out.geo.push_back(float4{0.f, 0.f, 0.f, 0.f});
@@ -279,6 +284,7 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
   build_tiles(u_host, p->N, p->tile, t);
   p->n_tiles = t.centre.size();
   p->max_offset = t.max_offset;
+  p->planar = t.planar;
   if (p->n_tiles * (size_t)p->tile > kMaxCount)
     return fail(HOLO_EINVAL, "too many tile slots");
   const size_t slots = p->n_tiles * p->tile;
@@ -304,8 +310,8 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
   p->device = ctx->device;
   p->N = N;
   p->tile = tile ? tile : 16;
-  if (p->tile != 8 && p->tile != 16 && p->tile != 32)
-    return fail(HOLO_EINVAL, "tile must be 8, 16 or 32");
+  if (p->tile != 8 && p->tile != 16 && p->tile != 32 && p->tile != 64)
+    return fail(HOLO_EINVAL, "tile must be 8, 16, 32 or 64");
   if (N && !u) return fail(HOLO_EINVAL, "null source positions");
   std::vector<double> host;
   const double* uh = u;
@@ -350,6 +356,7 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     ts.perm = static_cast<const int32_t*>(p->perm.ptr);
     ts.n_tiles = (uint32_t)p->n_tiles;
     ts.tile = p->tile;
+    ts.planar = p->planar;
     const size_t pe = split_elems_f32(M, p->n_tiles);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     HOLO_HIP(holo::launch_superpose_f32s(sign, ts, static_cast<const float2*>(p->wc.ptr), v, M,
@@ -694,6 +701,20 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
     HOLO_HIP(hipStreamSynchronize(ctx->stream));
   }
   return HOLO_OK;
+}
+
+void holo_host_sincos(double x, double* s, double* c) {
+  double a, b;
+  holo::hm_sincos(x, a, b);
+  if (s) *s = a;
+  if (c) *c = b;
+}
+
+void holo_host_sqrt_inv(double s, double* d, double* inv) {
+  double a, b;
+  holo::hm_sqrt_inv(s, a, b);
+  if (d) *d = a;
+  if (inv) *inv = b;
 }
 
 holo_status holo_read_dat(const char* path, double* buf, size_t capacity, size_t* count) {

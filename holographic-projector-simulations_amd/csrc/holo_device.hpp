@@ -50,14 +50,21 @@ __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
   return p;
 }
 
-// sin/cos of the pair argument: hm_sincos (MATH = 0) or the device libm.
+// r * (cos x, sin x) accumulated into (ar, ai): hm_sincos_unsigned with the
+// sign folded into r (MATH = 0) or the device libm sincos (MATH = 1).
 template <int MATH = 0>
-__device__ __forceinline__ void sincos_f64(double x, double& s, double& c) {
+__device__ __forceinline__ void accumulate_phasor(double r, double x,
+                                                  double& ar, double& ai) {
+  double s, c;
   if constexpr (MATH == 0) {
-    hm_sincos(x, s, c);
+    unsigned long long m;
+    hm_sincos_unsigned(x, s, c, m);
+    r = hm_apply_sign(r, m);
   } else {
     sincos(x, &s, &c);
   }
+  ar = __builtin_fma(r, c, ar);
+  ai = __builtin_fma(r, s, ai);
 }
 
 // Distance only (same pinned order as pair_f64).

@@ -20,6 +20,7 @@ struct TileSet {
   const int32_t* perm = nullptr;
   uint32_t n_tiles = 0;
   uint32_t tile = 0;
+  bool planar = false;  // every slot has delta_z == 0
 };
 
 struct Tuning {

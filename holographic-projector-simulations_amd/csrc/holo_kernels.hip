@@ -70,10 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const PairF64 p = pair_f64<SIGN, MATH>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
-        double s, c;
-        sincos_f64<MATH>(p.arg, s, c);
-        ar[r] = __builtin_fma(p.r, c, ar[r]);
-        ai[r] = __builtin_fma(p.r, s, ai[r]);
+        accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r]);
       }
     } else {
       const double wr = w0[2 * (size_t)n];
@@ -85,7 +82,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
         hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
         const double arg = (SIGN > 0 ? d : -d) * K;
         double s, c;
-        sincos_f64<MATH>(arg, s, c);
+        if constexpr (MATH == 0) {
+          hm_sincos(arg, s, c);
+        } else {
+          sincos(arg, &s, &c);
+        }
         c *= id;
         s *= id;
         ar[r] += wr * c - wi * s;
@@ -181,15 +182,19 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
 // Per (target, tile), in FP64: r = v - c, d_ref = |r|, the fractional cycle
 // f = frac(d_ref / lambda), and the tile rotation e^{i sign 2 pi f}.
 // Per pair, in FP32 and without cancellation:
-//   q = |delta|^2 - 2 r.delta        (= d^2 - d_ref^2, delta = u - c)
-//   W = q / (2 d_ref lambda)         (first-order offset, cycles)
-//   t = W - (lambda / (2 d_ref)) W^2 (= (d - d_ref)/lambda to O(q^3))
-//   1/d = 1/d_ref - t lambda / d_ref^2
+//   q = |delta|^2 - 2 r.delta          (= d^2 - d_ref^2, delta = u - c)
+//   t = (d - d_ref) / lambda = q (alpha + c2 q)        to O(q^3), with
+//       alpha = 1 / (2 d_ref lambda),  c2 = -1 / (8 d_ref^3 lambda)
+//   1/d = 1/d_ref + q g,  g = -1 / (2 d_ref^3)          to O(q^2)
 //   term = w * e^{i sign 2 pi t} / d
-// The T terms of a tile are summed in FP32, rotated by the tile's FP64-derived
-// phase and added into FP64 accumulators.
+// (d = sqrt(d_ref^2 + q) = d_ref + q/(2 d_ref) - q^2/(8 d_ref^3) + ...).  The
+// T terms of a tile are summed in FP32, rotated by the tile's FP64-derived
+// phase and added into FP64 accumulators.  PLANAR: every slot of every tile
+// has delta_z = 0 (sources on planes z = const), so the z term of q drops.
+// Valid in the far field of each tile: |q| / d_ref^2 << 1 (tile radius much
+// smaller than the distance) and |t| of order one cycle.
 
-template <int SIGN, int R, int T>
+template <int SIGN, int R, int T, bool PLANAR>
 __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     const double4* __restrict__ centre, const float4* __restrict__ geo,
     const float2* __restrict__ wc, uint32_t n_tiles, uint32_t chunk_tiles,
@@ -211,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
   }
   for (uint32_t t = t0; t < t1; ++t) {
     const double4 c = centre[t];
-    float m2rx[R], m2ry[R], m2rz[R], alpha[R], beta[R], gam[R], invd[R];
+    float m2rx[R], m2ry[R], m2rz[R], alpha[R], c2[R], gq[R], invd[R];
     float ct[R], st[R], sr[R], si[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -223,9 +228,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
       m2rx[r] = (float)(-2.0 * rx);
       m2ry[r] = (float)(-2.0 * ry);
       m2rz[r] = (float)(-2.0 * rz);
-      alpha[r] = (float)(0.5 * id * inv_lambda);
-      beta[r] = (float)(0.5 * lambda * id);
-      gam[r] = (float)(lambda * id * id);
+      const double id2 = id * id;
+      const double a = 0.5 * id * inv_lambda;
+      alpha[r] = (float)a;
+      c2[r] = (float)(-0.25 * a * id2);
+      gq[r] = (float)(-0.5 * id2 * id);
       invd[r] = (float)id;
       sr[r] = 0.f;
       si[r] = 0.f;
@@ -238,14 +245,13 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
       const float2 wj = w[j];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const float q = __builtin_fmaf(m2rx[r], gj.x,
-                        __builtin_fmaf(m2ry[r], gj.y,
-                        __builtin_fmaf(m2rz[r], gj.z, gj.w)));
-        const float W = q * alpha[r];
-        const float tt = __builtin_fmaf(-W * beta[r], W, W);
+        float q = __builtin_fmaf(m2ry[r], gj.y, gj.w);
+        if constexpr (!PLANAR) q = __builtin_fmaf(m2rz[r], gj.z, q);
+        q = __builtin_fmaf(m2rx[r], gj.x, q);
+        const float tt = q * __builtin_fmaf(c2[r], q, alpha[r]);
         float sn, cs;
         sincos_rev_f32(tt, sn, cs);
-        const float idd = __builtin_fmaf(-tt, gam[r], invd[r]);
+        const float idd = __builtin_fmaf(q, gq[r], invd[r]);
         const float pr = idd * cs;
         const float pi = idd * sn;
         if constexpr (SIGN > 0) {
@@ -273,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
   }
 }
 
-template <int SIGN, int T>
+template <int SIGN, int T, bool PLANAR>
 static void launch_f32s_r(int R, dim3 grid, const TileSet& ts,
                           const float2* wc, uint32_t chunk, const double* v,
                           size_t M, double lambda, double2* out,
@@ -281,17 +287,31 @@ static void launch_f32s_r(int R, dim3 grid, const TileSet& ts,
   const double il = 1.0 / lambda;
   switch (R) {
     case 1:
-      k_superpose_f32s<SIGN, 1, T><<<grid, kBlock, 0, stream>>>(
+      k_superpose_f32s<SIGN, 1, T, PLANAR><<<grid, kBlock, 0, stream>>>(
           ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
     case 4:
-      k_superpose_f32s<SIGN, 4, T><<<grid, kBlock, 0, stream>>>(
+      k_superpose_f32s<SIGN, 4, T, PLANAR><<<grid, kBlock, 0, stream>>>(
           ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
     default:
-      k_superpose_f32s<SIGN, 2, T><<<grid, kBlock, 0, stream>>>(
+      k_superpose_f32s<SIGN, 2, T, PLANAR><<<grid, kBlock, 0, stream>>>(
           ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
+  }
+}
+
+template <int T>
+static void launch_f32s_t(int sign, bool planar, int R, dim3 grid,
+                          const TileSet& ts, const float2* wc, uint32_t chunk,
+                          const double* v, size_t M, double lambda,
+                          double2* out, hipStream_t stream) {
+  if (sign > 0) {
+    if (planar) launch_f32s_r<+1, T, true>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+    else launch_f32s_r<+1, T, false>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+  } else {
+    if (planar) launch_f32s_r<-1, T, true>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
+    else launch_f32s_r<-1, T, false>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
   }
 }
 
@@ -308,17 +328,12 @@ hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
   dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
             (unsigned)S);
   double2* out = S > 1 ? partial : y;
-  if (ts.tile == 16) {
-    if (sign > 0) launch_f32s_r<+1, 16>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-    else launch_f32s_r<-1, 16>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-  } else if (ts.tile == 8) {
-    if (sign > 0) launch_f32s_r<+1, 8>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-    else launch_f32s_r<-1, 8>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-  } else if (ts.tile == 32) {
-    if (sign > 0) launch_f32s_r<+1, 32>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-    else launch_f32s_r<-1, 32>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-  } else {
-    return hipErrorInvalidValue;
+  switch (ts.tile) {
+    case 8: launch_f32s_t<8>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
+    case 16: launch_f32s_t<16>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
+    case 32: launch_f32s_t<32>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
+    case 64: launch_f32s_t<64>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
+    default: return hipErrorInvalidValue;
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -382,10 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
       const double4 p = pos[idx[h]];
       if constexpr (FP64) {
         const PairF64 q = pair_f64<SIGN>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
-        double s, c;
-        hm_sincos(q.arg, s, c);
-        ar = __builtin_fma(q.r, c, ar);
-        ai = __builtin_fma(q.r, s, ai);
+        accumulate_phasor<0>(q.r, q.arg, ar, ai);
       } else {
         const double ddx = vx - p.x, ddy = vy - p.y, ddz = vz - p.z;
         double d, idd;

@@ -1,18 +1,21 @@
 // FP64 sin/cos and sqrt/reciprocal-sqrt for the superposition inner loop.
 //
 // hm_sincos: Cody & Waite argument reduction (W. J. Cody, W. Waite,
-// "Software Manual for the Elementary Functions", 1980) with fused
-// multiply-adds and a two-term pi/2, followed by Taylor polynomials on
-// [-pi/4, pi/4]:
-//   k = rint(x * 2/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r)
-//   sin r = r + r^3 (S1 + r^2 (S2 + ... S7)),  S_j = (-1)^j / (2j+1)!
-//   cos r = 1 + r^2 (C1 + r^2 (C2 + ... C8)),  C_j = (-1)^j / (2j)!
-// P1 = RN(pi/2), P2 = RN(pi/2 - P1); the neglected third term is < 2^-109, so
-// for |x| < 2^50 the reduced argument is within ~1.2e-16 of x - k pi/2 and
-// the result within ~3e-16 of sin x, cos x.  The polynomial truncation is
-// below 7e-17 on [-pi/4, pi/4].  Only multiplies, FMAs and rint are used, so
-// the host and device builds produce identical bits.  The superposition's
-// arguments stay below 2^27 rad (d < 13 m at 632.8 nm), far inside the domain.
+// "Software Manual for the Elementary Functions", 1980) modulo pi with fused
+// multiply-adds and a two-term pi, followed by near-minimax polynomials on
+// [-pi/2, pi/2]:
+//   k = rint(x / pi);  r = fma(-k, PI1, x);  r = fma(-k, PI2, r)
+//   sin r = r + r^3 P(r^2),  cos r = 1 + r^2 Q(r^2)
+//   sin x = (-1)^k sin r,    cos x = (-1)^k cos r
+// PI1 = RN(pi), PI2 = RN(pi - PI1); the neglected third term is 3e-33, so for
+// |x| < 2^50 the reduced argument is within ~1.2e-16 of x - k pi.  P (degree
+// 6) and Q (degree 7) are least-squares fits on Chebyshev nodes computed by
+// gen_sincos_coeffs.py; their absolute error is 1.2e-16 (sin) and 3e-17
+// (cos) on [-pi/2, pi/2].  Reducing by pi instead of pi/2 leaves a common
+// sign (-1)^k and no sin/cos swap: hm_sincos_unsigned returns that sign as a
+// bit mask for the caller to fold into an amplitude with one xor.  Only
+// multiplies, FMAs, rint and integer ops are used, so host and device builds
+// give identical bits.  Superposition arguments stay below 2^27 rad.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -20,39 +23,48 @@ namespace holo {
 
 #define HM_HD __host__ __device__ __forceinline__
 
-HM_HD void hm_sincos(double x, double& s, double& c) {
+HM_HD void hm_sincos_unsigned(double x, double& s, double& c,
+                              unsigned long long& sign_mask) {
 #pragma clang fp contract(off)
-  constexpr double kTwoOverPi = 0x1.45f306dc9c883p-1;
-  constexpr double kP1 = 0x1.921fb54442d18p+0;
-  constexpr double kP2 = 0x1.1a62633145c07p-54;
+  constexpr double kOneOverPi = 0x1.45f306dc9c883p-2;
+  constexpr double kPi1 = 0x1.921fb54442d18p+1;
+  constexpr double kPi2 = 0x1.1a62633145c07p-53;
   constexpr double kMagic = 0x1.8p52;  // 1.5 * 2^52: low word = integer
-  const double k = __builtin_rint(x * kTwoOverPi);
-  double r = __builtin_fma(-k, kP1, x);
-  r = __builtin_fma(-k, kP2, r);
-  const double kb = k + kMagic;
-  const unsigned q = (unsigned)__builtin_bit_cast(unsigned long long, kb) & 3u;
-  const double r2 = r * r;
-  double ps = -0x1.ae7f3e733b81fp-41;               // S7
-  ps = __builtin_fma(ps, r2, 0x1.6124613a86d09p-33);  // S6
-  ps = __builtin_fma(ps, r2, -0x1.ae64567f544e4p-26); // S5
-  ps = __builtin_fma(ps, r2, 0x1.71de3a556c734p-19);  // S4
-  ps = __builtin_fma(ps, r2, -0x1.a01a01a01a01ap-13); // S3
-  ps = __builtin_fma(ps, r2, 0x1.1111111111111p-7);   // S2
-  ps = __builtin_fma(ps, r2, -0x1.5555555555555p-3);  // S1
-  const double sr = __builtin_fma(r * r2, ps, r);
-  double pc = 0x1.ae7f3e733b81fp-45;                  // C8
-  pc = __builtin_fma(pc, r2, -0x1.93974a8c07c9dp-37); // C7
-  pc = __builtin_fma(pc, r2, 0x1.1eed8eff8d898p-29);  // C6
-  pc = __builtin_fma(pc, r2, -0x1.27e4fb7789f5cp-22); // C5
-  pc = __builtin_fma(pc, r2, 0x1.a01a01a01a01ap-16);  // C4
-  pc = __builtin_fma(pc, r2, -0x1.6c16c16c16c17p-10); // C3
-  pc = __builtin_fma(pc, r2, 0x1.5555555555555p-5);   // C2
-  pc = __builtin_fma(pc, r2, -0.5);                   // C1
-  const double cr = __builtin_fma(pc, r2, 1.0);
-  double so = (q & 1u) ? cr : sr;
-  double co = (q & 1u) ? sr : cr;
-  s = (q & 2u) ? -so : so;
-  c = ((q + 1u) & 2u) ? -co : co;
+  const double k = __builtin_rint(x * kOneOverPi);
+  double r = __builtin_fma(-k, kPi1, x);
+  r = __builtin_fma(-k, kPi2, r);
+  const unsigned lo = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic);
+  sign_mask = (unsigned long long)(lo << 31) << 32;  // bit 63 = parity of k
+  const double u = r * r;
+  double p = -0x1.9e8d5c0fb24cdp-41;
+  p = __builtin_fma(p, u, 0x1.60e1e537cfcadp-33);
+  p = __builtin_fma(p, u, -0x1.ae63326f52860p-26);
+  p = __builtin_fma(p, u, 0x1.71de378613a35p-19);
+  p = __builtin_fma(p, u, -0x1.a01a01987bec7p-13);
+  p = __builtin_fma(p, u, 0x1.1111111106d2cp-7);
+  p = __builtin_fma(p, u, -0x1.5555555555412p-3);
+  s = __builtin_fma(r * u, p, r);
+  double q = 0x1.9f3d757541f50p-45;
+  q = __builtin_fma(q, u, -0x1.9351b15a30d5cp-37);
+  q = __builtin_fma(q, u, 0x1.1eece3c299f48p-29);
+  q = __builtin_fma(q, u, -0x1.27e4f98df5d2cp-22);
+  q = __builtin_fma(q, u, 0x1.a01a0199b870dp-16);
+  q = __builtin_fma(q, u, -0x1.6c16c16c0af4ep-10);
+  q = __builtin_fma(q, u, 0x1.5555555555403p-5);
+  q = __builtin_fma(q, u, -0x1.ffffffffffffdp-2);
+  c = __builtin_fma(q, u, 1.0);
+}
+
+// Flip the sign of x where bit 63 of mask is set.
+HM_HD double hm_apply_sign(double x, unsigned long long mask) {
+  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ mask);
+}
+
+HM_HD void hm_sincos(double x, double& s, double& c) {
+  unsigned long long m;
+  hm_sincos_unsigned(x, s, c, m);
+  s = hm_apply_sign(s, m);
+  c = hm_apply_sign(c, m);
 }
 
 // d = sqrt(s), correctly rounded, and inv = 1/d to ~2e-15 relative, for

@@ -127,7 +127,7 @@ holo_status holo_normalize_polar(holo_ctx *ctx, const double *y, size_t M,
 /* ---- plans: a fixed source geometry reused across calls -----------------
  * A plan keeps the N source positions resident on the device and, for
  * HOLO_FP32_SPLIT, groups them into spatial tiles of `tile` sources
- * (8, 16 or 32; 0 = 16).  No reference counterpart: it replaces the
+ * (8, 16, 32 or 64; 0 = 16).  No reference counterpart: it replaces the
  * per-call host-to-device copies of cuda/transform.cu:209-213. */
 holo_status holo_plan_create(holo_ctx *ctx, holo_plan **plan, const double *u,
                              size_t N, unsigned tile, unsigned flags);
@@ -174,6 +174,13 @@ holo_status holo_plan_adjoint(holo_ctx *ctx, const holo_plan *plan_targets,
 holo_status holo_read_dat(const char *path, double *buf, size_t capacity,
                           size_t *count);
 holo_status holo_write_dat(const char *path, const double *buf, size_t count);
+
+/* ---- host builds of the device math (verification only) -----------------
+ * The FP64 kernel's sin/cos (Cody-Waite reduction modulo pi + polynomials)
+ * and sqrt / reciprocal, compiled for the host from the same source; the
+ * sin/cos bits match the device's.  No reference counterpart. */
+void holo_host_sincos(double x, double *s, double *c);
+void holo_host_sqrt_inv(double s, double *d, double *inv);
 
 #ifdef __cplusplus
 }

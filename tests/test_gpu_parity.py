@@ -212,7 +212,7 @@ def test_chain_x_y_z_subset(pkg, ctx, oracle, workloads):
 # FP32-split
 
 
-@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("tile", [8, 16, 32, 64])
 def test_fp32_split_plain_small(pkg, ctx, oracle, workloads, tile):
     st = workloads.plain_small(0).stages[0]
     plan = ctx.plan(st.u, tile)
@@ -223,16 +223,39 @@ def test_fp32_split_plain_small(pkg, ctx, oracle, workloads, tile):
     assert err <= 1e-6
 
 
-@pytest.mark.parametrize("sign", [+1, -1])
-def test_fp32_split_plane_1024_vs_fp64(pkg, ctx, workloads, sign):
-    """Full 1024^2 -> 1024^2: FP32-split vs the GPU FP64 path on every target."""
+@pytest.fixture(scope="module")
+def plane_1024_fp64(ctx, pkg, workloads):
     st = workloads.plane_1024(0).stages[0]
-    y64 = ctx.transform_full(st.amp, st.phase, st.u, st.v, sign, st.lam, pkg.FP64)
-    y32 = ctx.transform_full(st.amp, st.phase, st.u, st.v, sign, st.lam, pkg.FP32_SPLIT)
+    return st, {s: ctx.transform_full(st.amp, st.phase, st.u, st.v, s, st.lam, pkg.FP64)
+                for s in (+1, -1)}
+
+
+@pytest.mark.parametrize("tile", [16, 64])
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_fp32_split_plane_1024_vs_fp64(pkg, ctx, plane_1024_fp64, sign, tile):
+    """Full 1024^2 -> 1024^2: FP32-split vs the GPU FP64 path on every target."""
+    st, y64 = plane_1024_fp64
+    y64 = y64[sign]
+    plan = ctx.plan(st.u, tile)
+    y32 = plan.transform(st.amp, st.phase, st.v, sign, st.lam, pkg.FP32_SPLIT)
     err = rel_l2(y32, y64)
-    print(f"fp32-split 1024^2 sign {sign}: rel L2 {err:.3e}, max abs "
+    print(f"fp32-split 1024^2 sign {sign} tile {tile}: rel L2 {err:.3e}, max abs "
           f"{np.max(np.abs(y32 - y64)) / np.max(np.abs(y64)):.3e}")
     assert err <= 1e-6
+
+
+def test_fp32_split_thin_slab_not_planar(pkg, ctx, oracle):
+    """Sources in a 1 um thick slab: tiles are not planar (delta_z != 0)."""
+    g = np.random.default_rng(21)
+    N, M = 30000, 400
+    u = (g.random((N, 3)) - 0.5) * np.array([3e-3, 3e-3, 1e-6])
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * 3e-3
+    v[:, 2] = 0.25
+    amp, phase = g.random(N), g.random(N) * 2 * np.pi
+    y = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(-1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-6
 
 
 def test_fp32_split_random_cloud(pkg, ctx, oracle):
