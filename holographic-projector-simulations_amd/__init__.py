@@ -103,6 +103,9 @@ def lib() -> ctypes.CDLL:
     L.holo_read_dat.argtypes = [ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)]
     L.holo_write_dat.argtypes = [ctypes.c_char_p, vp, sz]
     dp = ctypes.POINTER(d)
+    L.holo_plan_layout.argtypes = [vp, sz, ctypes.c_uint, vp, sz, ctypes.POINTER(sz), dp,
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.holo_plan_layout.restype = i
     L.holo_host_sincos.argtypes = [d, dp, dp]
     L.holo_host_sincos.restype = None
     L.holo_host_sqrt_inv.argtypes = [d, dp, dp]
@@ -297,14 +300,14 @@ class Context:
                                          ctypes.addressof(mx), 0))
         return amp, ph, mx.value
 
-    def plan(self, u, tile: int = 16) -> "Plan":
+    def plan(self, u, tile: int = 64) -> "Plan":
         return Plan(self, u, tile)
 
 
 class Plan:
     """Source geometry resident on the device (and tiled for FP32-split)."""
 
-    def __init__(self, ctx: Context, u, tile: int = 16):
+    def __init__(self, ctx: Context, u, tile: int = 64):
         self.ctx = ctx
         device = _is_torch(u) and u.is_cuda
         (uu,) = _prep([u], device)
@@ -394,6 +397,19 @@ class Plan:
                                       _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt),
                                       DEVICE_POINTERS if device else 0))
         return ga, gt
+
+
+def plan_layout(u, tile: int = 64):
+    """Host-side tiling of positions u (no device): (perm, n_tiles, max_offset, planar)."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    N = u.size // 3
+    nt, off, pl = ctypes.c_size_t(), ctypes.c_double(), ctypes.c_int()
+    check(lib().holo_plan_layout(_ptr(u), N, int(tile), None, 0, ctypes.byref(nt),
+                                 ctypes.byref(off), ctypes.byref(pl)))
+    perm = np.empty(max(nt.value * (tile or 64), 1), dtype=np.int32)
+    check(lib().holo_plan_layout(_ptr(u), N, int(tile), _ptr(perm), perm.size, ctypes.byref(nt),
+                                 ctypes.byref(off), ctypes.byref(pl)))
+    return perm[:nt.value * (tile or 64)], nt.value, off.value, bool(pl.value)
 
 
 def host_sincos(x: float):

@@ -174,12 +174,15 @@ size_t split_elems_f32(size_t M, size_t n_tiles) {
 // ---------------------------------------------------------------------------
 // Spatial tiling of source positions for the FP32-split kernels.
 //
-// Sources are binned into cells of a regular grid over their bounding box
-// (only axes with extent), sized so a cell holds about `tile` sources; each
-// cell's sources, in input order, fill tiles of `tile` slots (the last one
-// padded with weight-0 slots).  The tile centre is the midpoint of its
-// sources' bounding box.  For a regular n x n lattice with n divisible by
-// sqrt(tile) every tile is an exact sqrt(tile) x sqrt(tile) block.
+// Sources are binned into cells of a regular grid over their bounding box,
+// sized so that a cell holds about `tile` sources (axes whose extent would
+// get fewer than ~1.5 cells are dropped, so slabs and lines are binned in 2-D
+// or 1-D).  Cells are visited in Morton order; each cell's sources, in input
+// order, fill whole tiles, and the partial remainders of consecutive cells
+// are merged while the merged tile stays within two cell widths.  The last
+// tile is padded with weight-0 slots.  The tile centre is the midpoint of
+// its sources' bounding box.  A regular n x n lattice with n divisible by
+// sqrt(tile) gets exact sqrt(tile) x sqrt(tile) blocks and no padding.
 struct HostTiles {
   std::vector<double4> centre;
   std::vector<float4> geo;
@@ -188,12 +191,50 @@ struct HostTiles {
   bool planar = true;
 };
 
-void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
-  double lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) {
-    lo[d] = INFINITY;
-    hi[d] = -INFINITY;
+uint64_t spread_bits21(uint64_t x) {
+  x &= 0x1fffff;
+  x = (x | x << 32) & 0x1f00000000ffffULL;
+  x = (x | x << 16) & 0x1f0000ff0000ffULL;
+  x = (x | x << 8) & 0x100f00f00f00f00fULL;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ULL;
+  x = (x | x << 2) & 0x1249249249249249ULL;
+  return x;
+}
+
+void emit_tile(const double* u, const int32_t* idx, size_t cnt, unsigned T,
+               HostTiles& out) {
+  double blo[3] = {INFINITY, INFINITY, INFINITY};
+  double bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t s = 0; s < cnt; ++s)
+    for (int d = 0; d < 3; ++d) {
+      blo[d] = std::min(blo[d], u[3 * (size_t)idx[s] + d]);
+      bhi[d] = std::max(bhi[d], u[3 * (size_t)idx[s] + d]);
+    }
+  double c[3];
+  for (int d = 0; d < 3; ++d) c[d] = 0.5 * (blo[d] + bhi[d]);
+  out.centre.push_back(double4{c[0], c[1], c[2], 0.0});
+  for (size_t s = 0; s < T; ++s) {
+    if (s < cnt) {
+      const int32_t n = idx[s];
+      const float fx = (float)(u[3 * (size_t)n] - c[0]);
+      const float fy = (float)(u[3 * (size_t)n + 1] - c[1]);
+      const float fz = (float)(u[3 * (size_t)n + 2] - c[2]);
+      const double dd = (double)fx * fx + (double)fy * fy + (double)fz * fz;
+      out.geo.push_back(float4{fx, fy, fz, (float)dd});
+      out.perm.push_back(n);
+      if (fz != 0.f) out.planar = false;
+      out.max_offset = std::max(out.max_offset, std::sqrt(dd));
+    } else {
+      out.geo.push_back(float4{0.f, 0.f, 0.f, 0.f});
+      out.perm.push_back(-1);
+    }
   }
+}
+
+void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
+  out = HostTiles();
+  if (N == 0) return;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (size_t n = 0; n < N; ++n)
     for (int d = 0; d < 3; ++d) {
       lo[d] = std::min(lo[d], u[3 * n + d]);
@@ -201,35 +242,46 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
     }
   double ext[3], emax = 0.0;
   for (int d = 0; d < 3; ++d) {
-    ext[d] = N ? hi[d] - lo[d] : 0.0;
+    ext[d] = hi[d] - lo[d];
     emax = std::max(emax, ext[d]);
   }
-  int active[3], na = 0;
-  for (int d = 0; d < 3; ++d)
-    if (ext[d] > 1e-9 * emax && ext[d] > 0.0) active[na++] = d;
-  // cells per axis
+  bool act[3];
+  for (int d = 0; d < 3; ++d) act[d] = ext[d] > 0.0 && ext[d] > 1e-9 * emax;
   size_t cells[3] = {1, 1, 1};
-  if (na > 0 && N > T) {
-    const double target_cells = (double)N / (double)T;
-    double vol = 1.0;
-    for (int i = 0; i < na; ++i) vol *= ext[active[i]];
-    const double h = std::pow(vol / target_cells, 1.0 / na);
-    for (int i = 0; i < na; ++i) {
-      const double c = std::round(ext[active[i]] / h);
-      cells[active[i]] = (size_t)std::max(1.0, std::min(c, 1e6));
+  double h = 0.0;
+  if (N > T) {
+    for (int iter = 0; iter < 3; ++iter) {
+      int na = 0;
+      double vol = 1.0;
+      for (int d = 0; d < 3; ++d)
+        if (act[d]) {
+          ++na;
+          vol *= ext[d];
+        }
+      if (!na) break;
+      h = std::pow(vol * (double)T / (double)N, 1.0 / na);
+      bool changed = false;
+      for (int d = 0; d < 3; ++d)
+        if (act[d] && ext[d] / h < 1.5) {
+          act[d] = false;
+          changed = true;
+        }
+      if (!changed) break;
     }
+    for (int d = 0; d < 3; ++d)
+      if (act[d]) cells[d] = (size_t)std::max(1.0, std::min(std::round(ext[d] / h), 2097151.0));
   }
   std::vector<uint64_t> key(N);
   for (size_t n = 0; n < N; ++n) {
     uint64_t k = 0;
-    for (int d = 2; d >= 0; --d) {
-      size_t c = 0;
+    for (int d = 0; d < 3; ++d) {
+      uint64_t c = 0;
       if (cells[d] > 1) {
         const double f = (u[3 * n + d] - lo[d]) / ext[d] * (double)cells[d];
-        c = (size_t)std::max(0.0, std::floor(f));
+        c = (uint64_t)std::max(0.0, std::floor(f));
         if (c >= cells[d]) c = cells[d] - 1;
       }
-      k = k * cells[d] + c;
+      k |= spread_bits21(c) << d;
     }
     key[n] = k;
   }
@@ -237,46 +289,43 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(),
                    [&](int32_t a, int32_t b) { return key[a] < key[b]; });
-  out.centre.clear();
-  out.geo.clear();
-  out.perm.clear();
-  out.max_offset = 0.0;
-  out.planar = true;
+  const double merge_limit = 2.0 * h;
+  std::vector<int32_t> pending;
+  double plo[3], phi[3];
   size_t i = 0;
   while (i < N) {
     size_t j = i;
     while (j < N && key[order[j]] == key[order[i]]) ++j;  // one cell: [i, j)
-    for (size_t t0 = i; t0 < j; t0 += T) {
-      const size_t t1 = std::min(j, t0 + T);
-      double blo[3] = {INFINITY, INFINITY, INFINITY};
-      double bhi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      for (size_t s = t0; s < t1; ++s)
+    const size_t full = (j - i) / T * T;
+    for (size_t t0 = i; t0 < i + full; t0 += T) emit_tile(u, &order[t0], T, T, out);
+    if (full < j - i) {
+      double llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (size_t s = i + full; s < j; ++s)
         for (int d = 0; d < 3; ++d) {
-          blo[d] = std::min(blo[d], u[3 * (size_t)order[s] + d]);
-          bhi[d] = std::max(bhi[d], u[3 * (size_t)order[s] + d]);
+          llo[d] = std::min(llo[d], u[3 * (size_t)order[s] + d]);
+          lhi[d] = std::max(lhi[d], u[3 * (size_t)order[s] + d]);
         }
-      double c[3];
-      for (int d = 0; d < 3; ++d) c[d] = 0.5 * (blo[d] + bhi[d]);
-      out.centre.push_back(double4{c[0], c[1], c[2], 0.0});
-      for (size_t s = t0; s < t0 + T; ++s) {
-        if (s < t1) {
-          const int32_t n = order[s];
-          const float fx = (float)(u[3 * (size_t)n] - c[0]);
-          const float fy = (float)(u[3 * (size_t)n + 1] - c[1]);
-          const float fz = (float)(u[3 * (size_t)n + 2] - c[2]);
-          const double dd = (double)fx * fx + (double)fy * fy + (double)fz * fz;
-          out.geo.push_back(float4{fx, fy, fz, (float)dd});
-          out.perm.push_back(n);
-          if (fz != 0.f) out.planar = false;
-          out.max_offset = std::max(out.max_offset, std::sqrt(dd));
-        } else {
-          out.geo.push_back(float4{0.f, 0.f, 0.f, 0.f});
-          out.perm.push_back(-1);
+      bool merge = !pending.empty() && pending.size() + (j - i - full) <= T;
+      for (int d = 0; d < 3 && merge; ++d)
+        merge = std::max(phi[d], lhi[d]) - std::min(plo[d], llo[d]) <= merge_limit;
+      if (!merge) {
+        if (!pending.empty()) emit_tile(u, pending.data(), pending.size(), T, out);
+        pending.clear();
+        for (int d = 0; d < 3; ++d) {
+          plo[d] = llo[d];
+          phi[d] = lhi[d];
+        }
+      } else {
+        for (int d = 0; d < 3; ++d) {
+          plo[d] = std::min(plo[d], llo[d]);
+          phi[d] = std::max(phi[d], lhi[d]);
         }
       }
+      pending.insert(pending.end(), order.begin() + (i + full), order.begin() + j);
     }
     i = j;
   }
+  if (!pending.empty()) emit_tile(u, pending.data(), pending.size(), T, out);
 }
 
 holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
@@ -309,7 +358,7 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
                        unsigned tile, bool dev) {
   p->device = ctx->device;
   p->N = N;
-  p->tile = tile ? tile : 16;
+  p->tile = tile ? tile : 64;
   if (p->tile != 8 && p->tile != 16 && p->tile != 32 && p->tile != 64)
     return fail(HOLO_EINVAL, "tile must be 8, 16, 32 or 64");
   if (N && !u) return fail(HOLO_EINVAL, "null source positions");
@@ -507,7 +556,7 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
                                         ctx->stream));
   } else {
     holo_plan plan;
-    HOLO_TRY(plan_build(ctx, &plan, u, N, 16, dev));
+    HOLO_TRY(plan_build(ctx, &plan, u, N, 0, dev));
     HOLO_TRY(plan_run(ctx, &plan, precision, sign, lambda, holo::kPolar, da, dp, dv, M, dy));
     HOLO_HIP(hipStreamSynchronize(ctx->stream));  // plan buffers die here
   }
@@ -699,6 +748,26 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
     HOLO_HIP(hipMemcpyAsync(grad_theta, dgt, N * sizeof(double), hipMemcpyDeviceToHost,
                             ctx->stream));
     HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  return HOLO_OK;
+}
+
+holo_status holo_plan_layout(const double* u, size_t N, unsigned tile, int32_t* perm,
+                             size_t perm_capacity, size_t* n_tiles, double* max_offset,
+                             int* planar) {
+  if (N > kMaxCount) return fail(HOLO_EINVAL, "N must be below 2^31");
+  if (N && !u) return fail(HOLO_EINVAL, "null source positions");
+  const unsigned T = tile ? tile : 64;
+  if (T != 8 && T != 16 && T != 32 && T != 64)
+    return fail(HOLO_EINVAL, "tile must be 8, 16, 32 or 64");
+  HostTiles t;
+  build_tiles(u, N, T, t);
+  if (n_tiles) *n_tiles = t.centre.size();
+  if (max_offset) *max_offset = t.max_offset;
+  if (planar) *planar = t.planar ? 1 : 0;
+  if (perm) {
+    if (perm_capacity < t.perm.size()) return fail(HOLO_EINVAL, "perm buffer too small");
+    std::copy(t.perm.begin(), t.perm.end(), perm);
   }
   return HOLO_OK;
 }

@@ -24,7 +24,7 @@ struct TileSet {
 };
 
 struct Tuning {
-  int r_f64 = 2;          // targets per lane, FP64 kernel (1, 2, 4)
+  int r_f64 = 4;          // targets per lane, FP64 kernel (1, 2, 4)
   int r_f32 = 2;          // targets per lane, FP32-split kernel (1, 2, 4)
   int splits = 0;         // source splits (0 = automatic)
   int target_waves = 16384;

@@ -39,7 +39,7 @@ class Propagator:
     """Differentiable z = superposition(A, theta) from points u to points v."""
 
     def __init__(self, u: torch.Tensor, v: torch.Tensor, sign: int = +1,
-                 lam: float = 632.8e-9, precision: int = FP32_SPLIT, tile: int = 16,
+                 lam: float = 632.8e-9, precision: int = FP32_SPLIT, tile: int = 64,
                  ctx: Context = None):
         if not (u.is_cuda and v.is_cuda):
             raise ValueError("Propagator needs GPU tensors (no CPU fallback)")

@@ -48,7 +48,7 @@ typedef enum holo_status {
 
 typedef enum holo_precision {
   HOLO_FP64 = 0,      /* plain FP64: the drop-in; matches the FP64 CPU formulation */
-  HOLO_FP32_SPLIT = 1 /* FP32 pair math on FP64 per-tile references (~1e-7 rel. L2) */
+  HOLO_FP32_SPLIT = 1 /* FP32 pair math on FP64 per-tile references (~3e-7 rel. L2) */
 } holo_precision;
 
 #define HOLO_FORWARDS (+1)  /* phase + 2 pi d / lambda (Direction::Forwards) */
@@ -127,7 +127,7 @@ holo_status holo_normalize_polar(holo_ctx *ctx, const double *y, size_t M,
 /* ---- plans: a fixed source geometry reused across calls -----------------
  * A plan keeps the N source positions resident on the device and, for
  * HOLO_FP32_SPLIT, groups them into spatial tiles of `tile` sources
- * (8, 16, 32 or 64; 0 = 16).  No reference counterpart: it replaces the
+ * (8, 16, 32 or 64; 0 = 64).  No reference counterpart: it replaces the
  * per-call host-to-device copies of cuda/transform.cu:209-213. */
 holo_status holo_plan_create(holo_ctx *ctx, holo_plan **plan, const double *u,
                              size_t N, unsigned tile, unsigned flags);
@@ -137,6 +137,14 @@ void holo_plan_destroy(holo_plan *plan);
 holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
                            size_t *n_tiles, unsigned *tile,
                            double *max_offset);
+
+/* Host-only preview of the tiling a plan builds (no device needed): tile
+ * count, slot -> source index map (n_tiles * tile entries, -1 = padding;
+ * perm may be NULL), largest offset from a tile centre, and whether every
+ * tile lies in one z plane. */
+holo_status holo_plan_layout(const double *u, size_t N, unsigned tile,
+                             int32_t *perm, size_t perm_capacity,
+                             size_t *n_tiles, double *max_offset, int *planar);
 
 /* Polar weights (amp, phase: N each), as holo_transform_full. */
 holo_status holo_plan_transform(holo_ctx *ctx, const holo_plan *plan,

@@ -46,6 +46,6 @@ def _reset_options():
     """Tuning knobs are process-wide: restore defaults around every test."""
     yield
     if os.path.exists(holo.LIB_PATH):
-        for k, v in (("r_f64", 2), ("r_f32", 2), ("splits", 0), ("target_waves", 16384),
+        for k, v in (("r_f64", 4), ("r_f32", 2), ("splits", 0), ("target_waves", 16384),
                      ("f64_libm", 0)):
             holo.set_option(k, v)

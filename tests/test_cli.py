@@ -1,0 +1,137 @@
+"""The `holo` command line (flags and files of cuda/main.cu + cuda/input.h)."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "holographic-projector-simulations_amd", "bin", "holo")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+LAM_REF = 0.65e-6  # the CLI's default wavelength (cuda/macros.h:23-25)
+
+
+def run(args, cwd=None, env=None):
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, env=e, timeout=120)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    if not os.path.exists(CLI):
+        pytest.skip("holo CLI not built")
+
+
+def test_help_exits_zero():
+    r = run(["-h"])
+    assert r.returncode == 0 and "Usage" in r.stdout and "-f {dir}" in r.stdout
+
+
+@pytest.mark.parametrize("args, msg", [
+    (["-p", "4"], "algorithm"),
+    (["-p", "0"], "algorithm"),
+    (["-y", "0"], "-y"),
+    (["-b", "64", "-B", "32"], "blockDim"),
+    (["-w", "0"], "z offset"),
+    (["-w", "0.1", "-m", "-0.1"], "projection plane"),
+])
+def test_invalid_arguments_exit_nonzero(args, msg):
+    """cuda/input.h:129-149 asserts become messages and exit status 1."""
+    r = run(args)
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_missing_and_ragged_inputs(tmp_path):
+    r = run(["-f", str(tmp_path)])
+    assert r.returncode == 1 and "x0_amp.dat" in r.stderr
+    for name in ("x0_amp.dat", "x0_phase.dat", "u0.dat", "v0.dat"):
+        shutil.copy(os.path.join(GOLDEN, "plain_small_" + {"x0_amp.dat": "x_amp.dat",
+                                                         "x0_phase.dat": "x_phase.dat",
+                                                         "u0.dat": "u.dat",
+                                                         "v0.dat": "v.dat"}[name]),
+                    tmp_path / name)
+    (tmp_path / "u0.dat").write_bytes((tmp_path / "u0.dat").read_bytes()[:-24])
+    r = run(["-f", str(tmp_path)])
+    assert r.returncode == 1 and "source counts" in r.stderr
+
+
+def stage_golden(d):
+    for src, dst in (("x_amp", "x0_amp"), ("x_phase", "x0_phase"), ("u", "u0"), ("v", "v0")):
+        shutil.copy(os.path.join(GOLDEN, f"plain_small_{src}.dat"), os.path.join(d, f"{dst}.dat"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("projection", [False, True])
+def test_file_mode_matches_oracle(tmp_path, oracle, projection):
+    """holo -f DIR: y = time_transform<Backwards>(x, u, v) written as y0_*.dat
+    with the reference's normalisation and polar conversion; -F also writes the
+    forwards transform as z0_0 (cuda/main.cu:203-222)."""
+    stage_golden(tmp_path)
+    amp = np.fromfile(tmp_path / "x0_amp.dat", "<f8")
+    phase = np.fromfile(tmp_path / "x0_phase.dat", "<f8")
+    u = np.fromfile(tmp_path / "u0.dat", "<f8")
+    v = np.fromfile(tmp_path / "v0.dat", "<f8")
+    M = v.size // 3
+    if projection:
+        shutil.copy(tmp_path / "v0.dat", tmp_path / "w0.dat")
+    args = ["-f", str(tmp_path), "-y", str(M)] + (["-F"] if projection else [])
+    r = run(args)
+    assert r.returncode == 0, r.stderr
+    ya = np.fromfile(tmp_path / "y0_amp.dat", "<f8")
+    yp = np.fromfile(tmp_path / "y0_phase.dat", "<f8")
+    ref = oracle.transform_full(-1, LAM_REF, amp, phase, u, v)
+    ra, rp, _ = oracle.normalize_polar(ref)
+    assert ya.size == M and np.max(np.abs(ya - ra)) < 1e-12
+    z = ya * np.exp(1j * yp)
+    assert np.linalg.norm(z - ra * np.exp(1j * rp)) / np.linalg.norm(ra) < 1e-12
+    assert np.array_equal(np.fromfile(tmp_path / "v0.dat", "<f8"), v)
+    lines = [json.loads(l) for l in open(tmp_path / "out.json")]
+    assert lines[0]["phasor"] == "y0" and lines[0]["pos"] == "v0" and lines[0]["len"] == M
+    assert lines[0]["precision"] == 8 and lines[0]["dims"] == 3
+    if projection:
+        za = np.fromfile(tmp_path / "z0_0_amp.dat", "<f8")
+        rf = oracle.transform_full(+1, LAM_REF, amp, phase, u, v)
+        fa, _, _ = oracle.normalize_polar(rf)
+        assert np.max(np.abs(za - fa)) < 1e-12
+        assert lines[1]["phasor"] == "z0_0" and lines[1]["pos"] == "w0_0"
+
+
+@pytest.mark.gpu
+def test_target_count_comes_from_y_flag(tmp_path, oracle):
+    """M is -y, not the size of v0.dat: extra targets are the origin."""
+    stage_golden(tmp_path)
+    v = np.fromfile(tmp_path / "v0.dat", "<f8")
+    M = v.size // 3
+    r = run(["-f", str(tmp_path), "-y", str(M + 10)])
+    assert r.returncode == 0, r.stderr
+    ya = np.fromfile(tmp_path / "y0_amp.dat", "<f8")
+    assert ya.size == M + 10
+    amp = np.fromfile(tmp_path / "x0_amp.dat", "<f8")
+    phase = np.fromfile(tmp_path / "x0_phase.dat", "<f8")
+    u = np.fromfile(tmp_path / "u0.dat", "<f8")
+    vv = np.concatenate([v, np.zeros(30)])
+    ra, _, _ = oracle.normalize_polar(oracle.transform_full(-1, LAM_REF, amp, phase, u, vv))
+    assert np.max(np.abs(ya - ra)) < 1e-12
+
+
+@pytest.mark.gpu
+def test_generated_mode_writes_reference_files(tmp_path):
+    """No -f: object circle -> projector lattice -> projection lattice in ../tmp/."""
+    work = tmp_path / "run"
+    (tmp_path / "tmp").mkdir()
+    work.mkdir()
+    r = run(["-x", "16", "-y", "4096", "-z", "1024", "-Z", "2"], cwd=str(work))
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "tmp"
+    lines = [json.loads(l) for l in open(out / "out.json")]
+    assert [l["phasor"] for l in lines] == ["x0", "y0", "z0_0", "z0_1"]
+    y = np.fromfile(out / "y0_amp.dat", "<f8")
+    assert y.size == 4096 and abs(y.max() - 1.0) < 1e-12
+    w = np.fromfile(out / "w0_1.dat", "<f8").reshape(-1, 3)
+    assert w.shape == (1024, 3) and np.allclose(w[:, 2], 0.35)
+    u = np.fromfile(out / "u0.dat", "<f8").reshape(-1, 3)
+    assert np.allclose(np.hypot(u[:, 0], u[:, 1]), 0.5e-3)  # circle of width 1 mm
