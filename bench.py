@@ -164,7 +164,7 @@ def main():
             f32 = torch.view_as_complex(y32).cpu().numpy()
             out["fp32_split"] = {
                 "workload": "1024^2 -> 1024^2 (z 0 -> 0.35 m, 10 / 7.56 um, amp U[0,1), "
-                            "phase U[0,2pi)), tiles of 16",
+                            "phase U[0,2pi)), tiles of 64",
                 "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
                 "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
                 "rel_l2_vs_fp64": rel_l2(f32, f64),

@@ -383,6 +383,18 @@ int body(Params& params) {
 
 int main(int argc, char** argv) {
   Params params = read_args(argc, argv);
+  if (const char* l = getenv("HOLO_LAMBDA")) {
+    char* end = nullptr;
+    const double lam = strtod(l, &end);
+    if (!(lam > 0.0) || end == l) die("HOLO_LAMBDA must be a positive wavelength in metres");
+    holo::settings().lambda = lam;
+  }
+  if (const char* pr = getenv("HOLO_PRECISION")) {
+    const std::string v(pr);
+    if (v == "fp64") holo::settings().precision = HOLO_FP64;
+    else if (v == "fp32split") holo::settings().precision = HOLO_FP32_SPLIT;
+    else die("HOLO_PRECISION must be fp64 or fp32split");
+  }
   const char* pp = getenv("HOLO_PROJECT_PHASE");
   const bool project_phase = !(pp && std::string(pp) == "0");
   return project_phase ? body<false>(params) : body<true>(params);

@@ -120,12 +120,23 @@ inline Geometry geometry(size_t n_x, size_t n_y, int algorithm = 2,
 
 inline int sign_of(Direction d) { return d == Direction::Forwards ? HOLO_FORWARDS : HOLO_BACKWARDS; }
 
+// Wavelength and precision used by transform_full / time_transform (the
+// reference fixes LAMBDA at compile time, cuda/macros.h:23-25).
+struct Settings {
+  double lambda = HOLO_LAMBDA;
+  holo_precision precision = HOLO_FP64;
+};
+inline Settings& settings() {
+  static Settings s;
+  return s;
+}
+
 // Field at `M_out` targets: v is used up to its length and the rest are the
 // origin, as the reference's zero-filled last batch.
 inline std::vector<Wave> superpose(Direction dir, const std::vector<Polar>& x,
                                    const std::vector<double>& u, const std::vector<double>& v,
-                                   size_t M_out, double lambda = HOLO_LAMBDA,
-                                   holo_precision precision = HOLO_FP64) {
+                                   size_t M_out, double lambda = settings().lambda,
+                                   holo_precision precision = settings().precision) {
   const size_t N = x.size();
   if (u.size() < 3 * N) throw Error(HOLO_EINVAL, "u holds fewer than 3N coordinates");
   std::vector<double> amp(N), phase(N);

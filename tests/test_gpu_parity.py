@@ -338,7 +338,7 @@ def test_adjoint_matches_oracle(pkg, ctx, oracle, workloads, precision):
 
 def test_plan_transform_matches_transform_full(pkg, ctx):
     amp, phase, u, v = rand_geom(17, 4000, 2500)
-    plan = ctx.plan(u, 16)
+    plan = ctx.plan(u)  # the default tiling, as transform_full uses
     for prec in (0, 1):
         a = plan.transform(amp, phase, v, -1, LAM, prec)
         b = ctx.transform_full(amp, phase, u, v, -1, LAM, prec)
