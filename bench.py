@@ -172,17 +172,22 @@ def main():
             log(f"[bench] fp32-split: {out['fp32_split']}")
         if "mc" in sections:
             mc = {}
-            for K in (256, 1024, 4096):
-                ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
-                torch.cuda.synchronize(dev)
-                t0 = time.perf_counter()
-                ymc = ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam, holo.FP32_SPLIT)
-                torch.cuda.synchronize(dev)
-                tk = time.perf_counter() - t0
-                mc[str(K)] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk}
-                if f64 is not None:
-                    mc[str(K)]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
-            out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0; "
+            for shared in (False, True):
+                for K in (256, 1024, 4096):
+                    run = lambda: ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam,
+                                                   holo.FP32_SPLIT, shared=shared)
+                    run()
+                    torch.cuda.synchronize(dev)
+                    t0 = time.perf_counter()
+                    ymc = run()
+                    torch.cuda.synchronize(dev)
+                    tk = time.perf_counter() - t0
+                    key = ("batched_" if shared else "") + str(K)
+                    mc[key] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk}
+                    if f64 is not None:
+                        mc[key]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
+            out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0; K draws "
+                                              "per target, or per group of 1024 targets (batched); "
                                               "expected rel. error sqrt(N/K) for a random-phase field",
                                   "K": mc}
             log(f"[bench] monte carlo: {mc}")

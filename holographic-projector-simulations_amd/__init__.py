@@ -32,6 +32,7 @@ FP32_SPLIT = 1
 FORWARDS = +1
 BACKWARDS = -1
 DEVICE_POINTERS = 0x1
+MC_SHARED = 0x2
 
 
 class Direction(enum.IntEnum):
@@ -254,7 +255,9 @@ class Context:
 
     # -- Monte Carlo -----------------------------------------------------------
     def transform_mc(self, amp, phase, u, v, K: int, seed: int = 0, sign: int = FORWARDS,
-                     lam: float = LAMBDA, precision: int = FP32_SPLIT):
+                     lam: float = LAMBDA, precision: int = FP32_SPLIT, shared: bool = False):
+        """Monte Carlo estimate with K draws per target (shared: per group of 1024)."""
+        extra = MC_SHARED if shared else 0
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)
         N, M = _n(amp, 1), _n(v, 3)
@@ -266,17 +269,19 @@ class Context:
             y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
             check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
                                           _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y),
-                                          DEVICE_POINTERS))
+                                          DEVICE_POINTERS | extra))
             return torch.view_as_complex(y)
         y = np.empty(2 * M)
         check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                      _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), 0))
+                                      _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), extra))
         return y.view(np.complex128)
 
-    def mc_indices(self, seed: int, N: int, m0: int, count: int, K: int) -> np.ndarray:
+    def mc_indices(self, seed: int, N: int, m0: int, count: int, K: int,
+                   shared: bool = False) -> np.ndarray:
+        """Indices drawn on the device: per target (or per group with shared)."""
         idx = np.empty(count * K, dtype=np.int64)
         check(lib().holo_mc_indices(self._h, int(seed), int(N), int(m0), int(count), int(K),
-                                    _ptr(idx), 0))
+                                    _ptr(idx), MC_SHARED if shared else 0))
         return idx.reshape(count, K)
 
     # -- time_transform post-processing -------------------------------------------
@@ -485,9 +490,11 @@ def time_transform(x, u, v, direction: Direction = Direction.Forwards,
 
 
 def transform(x, u, v, K: int, seed: int = 0, direction: Direction = Direction.Forwards,
-              lam: float = LAMBDA, precision: int = FP32_SPLIT,
+              lam: float = LAMBDA, precision: int = FP32_SPLIT, batched: bool = False,
               ctx: Optional[Context] = None) -> np.ndarray:
-    """Monte Carlo transform(): K uniform source draws per target, scaled N/K."""
+    """Monte Carlo transform(): K uniform source draws per target ("True MC"),
+    or per group of 1024 targets with batched=True ("Batched MC"), scaled N/K."""
     x = np.asarray(x, dtype=np.float64).reshape(-1, 2)
     c = ctx or default_context()
-    return c.transform_mc(x[:, 0], x[:, 1], u, v, K, seed, int(direction), lam, precision)
+    return c.transform_mc(x[:, 0], x[:, 1], u, v, K, seed, int(direction), lam, precision,
+                          shared=batched)

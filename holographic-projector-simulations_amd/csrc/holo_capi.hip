@@ -586,7 +586,8 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
   HOLO_HIP(holo::launch_mc_pack(fp64, da, dp, du, N, static_cast<double4*>(ctx->pos.ptr),
                                 static_cast<double*>(ctx->ph.ptr), ctx->stream));
   const double Kw = (2 * M_PI) / lambda;
-  HOLO_HIP(holo::launch_mc(sign, fp64, static_cast<const double4*>(ctx->pos.ptr),
+  HOLO_HIP(holo::launch_mc(sign, fp64, flags & HOLO_MC_SHARED,
+                           static_cast<const double4*>(ctx->pos.ptr),
                            static_cast<const double*>(ctx->ph.ptr), N, dv, M, K, seed, Kw,
                            lambda, reinterpret_cast<double2*>(dy), ctx->stream));
   return finish_out(ctx, y, dy, 2 * M, dev);
@@ -602,7 +603,8 @@ holo_status holo_mc_indices(holo_ctx* ctx, uint64_t seed, uint64_t N, size_t m0,
   if (!total) return HOLO_OK;
   int64_t* di;
   HOLO_TRY(out_buf(ctx, ctx->w, idx, total, dev, &di));
-  HOLO_HIP(holo::launch_mc_indices(seed, N, m0, count, K, di, ctx->stream));
+  HOLO_HIP(holo::launch_mc_indices(seed, N, m0, count, K, flags & HOLO_MC_SHARED, di,
+                                   ctx->stream));
   return finish_out(ctx, idx, di, total, dev);
 }
 

@@ -60,7 +60,9 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 // Monte Carlo: y[m] = (N/K) sum_k pair(n_{m,k}, m).
 // FP64: pos = (x,y,z,amp) per source, ph = phase per source.
 // FP32-split: pos = (x,y,z, bitcast float2 weight).
-hipError_t launch_mc(int sign, bool fp64, const double4* pos,
+constexpr unsigned kMcGroup = 1024;  // targets sharing one subset (batched MC)
+
+hipError_t launch_mc(int sign, bool fp64, bool shared, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
                      double2* y, hipStream_t stream);
@@ -69,8 +71,10 @@ hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
                           const double* u, size_t N, double4* pos, double* ph,
                           hipStream_t stream);
 
+// Per-target indices of targets m0..m0+count-1, or (shared) the subsets of
+// groups m0..m0+count-1.
 hipError_t launch_mc_indices(uint64_t seed, uint64_t N, size_t m0,
-                             size_t count, uint32_t K, int64_t* idx,
+                             size_t count, uint32_t K, bool shared, int64_t* idx,
                              hipStream_t stream);
 
 // max |y| into *max_out (device), then amp/phase = polar(y / max) (or y when

@@ -375,6 +375,31 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 // Monte Carlo: target m draws K indices n_{m,k} from Philox4x32-10 keyed by
 // the seed with counter (k/2, m, 0, kMcStream); words (x,y) serve even k,
 // (z,w) odd k; Lemire multiply-shift maps 64 bits into [0, N).
+// Batched ("shared") Monte Carlo: the targets of group g = m / kMcGroup share
+// one subset, drawn with counter (k/2, g, kMcSharedTag, kMcStream); the
+// index is then wave-uniform, Philox runs on the scalar unit and sources
+// arrive through scalar loads.
+
+constexpr uint32_t kMcSharedTag = 0x80000000u;
+
+// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
+// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).
+template <int SIGN>
+__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
+                                             double inv_lambda, float& fr, float& fi) {
+  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+  double d, id;
+  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  const float f = (float)frac_cycle(d * inv_lambda);
+  float sn, cs;
+  sincos_rev_f32(f, sn, cs);
+  if (SIGN < 0) sn = -sn;
+  const float idf = (float)id;
+  const float2 w = __builtin_bit_cast(float2, p.w);
+  const float pr = idf * cs, pi = idf * sn;
+  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
+  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+}
 
 template <int SIGN, bool FP64>
 __global__ __launch_bounds__(kBlock) void k_mc(
@@ -388,7 +413,40 @@ __global__ __launch_bounds__(kBlock) void k_mc(
                vz = v[3 * (size_t)m + 2];
   double ar = 0.0, ai = 0.0;
   float fr = 0.f, fi = 0.f;
-  for (uint32_t k = 0; k < K; k += 2) {
+  // 8 samples per step: 4 Philox blocks, then 8 independent gathers in flight
+  const uint32_t K8 = K & ~7u;
+  for (uint32_t k = 0; k < K8; k += 8) {
+    uint32_t idx[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const U4 o = philox4x32_10(U4{(k >> 1) + h, m, 0u, kMcStream}, seed_lo, seed_hi);
+      idx[2 * h] = lemire_index(o.x, o.y, N);
+      idx[2 * h + 1] = lemire_index(o.z, o.w, N);
+    }
+    double4 p[8];
+    double phv[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      p[h] = pos[idx[h]];
+      if constexpr (FP64) phv[h] = ph[idx[h]];
+    }
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      if constexpr (FP64) {
+        const PairF64 q = pair_f64<SIGN>(p[h].w, phv[h], p[h].x, p[h].y, p[h].z, vx, vy, vz, Kw);
+        accumulate_phasor<0>(q.r, q.arg, ar, ai);
+      } else {
+        mc_pair_f32s<SIGN>(p[h], vx, vy, vz, inv_lambda, fr, fi);
+      }
+    }
+    if constexpr (!FP64) {
+      ar += fr;
+      ai += fi;
+      fr = 0.f;
+      fi = 0.f;
+    }
+  }
+  for (uint32_t k = K8; k < K; k += 2) {
     const U4 o = philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, seed_lo, seed_hi);
     const uint32_t idx[2] = {lemire_index(o.x, o.y, N), lemire_index(o.z, o.w, N)};
 #pragma unroll
@@ -399,41 +457,94 @@ __global__ __launch_bounds__(kBlock) void k_mc(
         const PairF64 q = pair_f64<SIGN>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
         accumulate_phasor<0>(q.r, q.arg, ar, ai);
       } else {
-        const double ddx = vx - p.x, ddy = vy - p.y, ddz = vz - p.z;
-        double d, idd;
-        hm_sqrt_inv(__builtin_fma(ddz, ddz, __builtin_fma(ddy, ddy, ddx * ddx)), d, idd);
-        const float f = (float)frac_cycle(d * inv_lambda);
-        float sn, cs;
-        sincos_rev_f32(f, sn, cs);
-        if (SIGN < 0) sn = -sn;
-        const float id = (float)idd;
-        const float2 w = __builtin_bit_cast(float2, p.w);
-        const float pr = id * cs, pi = id * sn;
-        fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
-        fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
-      }
-    }
-    if constexpr (!FP64) {
-      if ((k & 31) == 30 || k + 2 >= K) {  // flush FP32 partials every 32 draws
-        ar += fr;
-        ai += fi;
-        fr = 0.f;
-        fi = 0.f;
+        mc_pair_f32s<SIGN>(p, vx, vy, vz, inv_lambda, fr, fi);
       }
     }
   }
+  ar += fr;
+  ai += fi;
   y[m] = make_double2(ar * scale, ai * scale);
 }
 
-hipError_t launch_mc(int sign, bool fp64, const double4* pos,
+// Batched Monte Carlo: one workgroup = one group of kMcGroup targets
+// (kBlock lanes x 4 targets each) sharing the subset n_{g,k}.
+template <int SIGN, bool FP64>
+__global__ __launch_bounds__(kBlock) void k_mc_shared(
+    const double4* __restrict__ pos, const double* __restrict__ ph,
+    uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
+    uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
+    double scale, double2* __restrict__ y) {
+  constexpr int R = kMcGroup / kBlock;
+  const uint32_t g = blockIdx.x;
+  double vx[R], vy[R], vz[R], ar[R], ai[R];
+  float fr[R], fi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = g * kMcGroup + r * kBlock + threadIdx.x;
+    const size_t mm = m < M ? m : M - 1;
+    vx[r] = v[3 * mm];
+    vy[r] = v[3 * mm + 1];
+    vz[r] = v[3 * mm + 2];
+    ar[r] = ai[r] = 0.0;
+    fr[r] = fi[r] = 0.f;
+  }
+  for (uint32_t k = 0; k < K; k += 2) {
+    const U4 o = philox4x32_10(U4{k >> 1, g, kMcSharedTag, kMcStream}, seed_lo, seed_hi);
+    const uint32_t idx[2] = {lemire_index(o.x, o.y, N), lemire_index(o.z, o.w, N)};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1 && k + 1 >= K) break;
+      const double4 p = pos[idx[h]];
+      if constexpr (FP64) {
+        const double phv = ph[idx[h]];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const PairF64 q = pair_f64<SIGN>(p.w, phv, p.x, p.y, p.z, vx[r], vy[r], vz[r], Kw);
+          accumulate_phasor<0>(q.r, q.arg, ar[r], ai[r]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) mc_pair_f32s<SIGN>(p, vx[r], vy[r], vz[r], inv_lambda, fr[r], fi[r]);
+      }
+    }
+    if constexpr (!FP64) {
+      if ((k & 62) == 62 || k + 2 >= K) {  // flush FP32 partials every 64 draws
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          ar[r] += fr[r];
+          ai[r] += fi[r];
+          fr[r] = fi[r] = 0.f;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const uint32_t m = g * kMcGroup + r * kBlock + threadIdx.x;
+    if (m < M) y[m] = make_double2(ar[r] * scale, ai[r] * scale);
+  }
+}
+
+hipError_t launch_mc(int sign, bool fp64, bool shared, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
                      double2* y, hipStream_t stream) {
   if (!M) return hipSuccess;
-  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
   const double scale = (double)N / (double)K;
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
   const double il = 1.0 / lambda;
+  if (shared) {
+    const unsigned blocks = (unsigned)((M + kMcGroup - 1) / kMcGroup);
+    if (sign > 0) {
+      if (fp64) k_mc_shared<+1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+      else k_mc_shared<+1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+    } else {
+      if (fp64) k_mc_shared<-1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+      else k_mc_shared<-1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+    }
+    return hipGetLastError();
+  }
+  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
   if (sign > 0) {
     if (fp64) k_mc<+1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
     else k_mc<+1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
@@ -478,26 +589,26 @@ hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
 
 __global__ __launch_bounds__(kBlock) void k_mc_indices(
     uint32_t seed_lo, uint32_t seed_hi, uint64_t N, uint32_t m0,
-    uint32_t count, uint32_t K, int64_t* __restrict__ idx) {
+    uint32_t count, uint32_t K, uint32_t tag, int64_t* __restrict__ idx) {
   const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
   if (j >= count) return;
   const uint32_t m = m0 + j;
   for (uint32_t k = 0; k < K; k += 2) {
-    const U4 o = philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, seed_lo, seed_hi);
+    const U4 o = philox4x32_10(U4{k >> 1, m, tag, kMcStream}, seed_lo, seed_hi);
     idx[(size_t)j * K + k] = lemire_index(o.x, o.y, N);
     if (k + 1 < K) idx[(size_t)j * K + k + 1] = lemire_index(o.z, o.w, N);
   }
 }
 
 hipError_t launch_mc_indices(uint64_t seed, uint64_t N, size_t m0,
-                             size_t count, uint32_t K, int64_t* idx,
+                             size_t count, uint32_t K, bool shared, int64_t* idx,
                              hipStream_t stream) {
   if (!count || !K) return hipSuccess;
   const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
   k_mc_indices<<<blocks, kBlock, 0, stream>>>((uint32_t)seed,
                                              (uint32_t)(seed >> 32), N,
                                              (uint32_t)m0, (uint32_t)count, K,
-                                             idx);
+                                             shared ? kMcSharedTag : 0u, idx);
   return hipGetLastError();
 }
 

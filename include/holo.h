@@ -55,6 +55,7 @@ typedef enum holo_precision {
 #define HOLO_BACKWARDS (-1) /* phase - 2 pi d / lambda (Direction::Backwards) */
 
 #define HOLO_DEVICE_POINTERS 0x1u /* all array arguments are device pointers */
+#define HOLO_MC_SHARED 0x2u       /* Monte Carlo: groups of 1024 targets share a subset */
 
 typedef struct holo_ctx holo_ctx;
 typedef struct holo_plan holo_plan;
@@ -102,7 +103,11 @@ holo_status holo_transform_full(holo_ctx *ctx, holo_precision precision,
  * Indices come from Philox4x32-10 keyed by `seed`, counter
  * (k/2, m_lo, m_hi, 0x4d43); words (0,1) serve even k, (2,3) odd k; the
  * 64-bit word r maps to floor(r N / 2^64).  They depend only on
- * (seed, m, k, N), not on the launch. */
+ * (seed, m, k, N), not on the launch.
+ * With HOLO_MC_SHARED ("Batched MC", py/plot.py:787-804): the targets of
+ * group g = m / 1024 share the subset n_{g,k} drawn with counter
+ * (k/2, g, 0x80000000, 0x4d43); each target's estimate is still unbiased
+ * with the same variance, but errors are correlated within a group. */
 holo_status holo_transform_mc(holo_ctx *ctx, holo_precision precision,
                               int sign, double lambda, const double *amp,
                               const double *phase, const double *u, size_t N,
@@ -110,7 +115,8 @@ holo_status holo_transform_mc(holo_ctx *ctx, holo_precision precision,
                               uint64_t seed, double *y, unsigned flags);
 
 /* Sample indices n_{m,k} for targets m0 .. m0+count-1 (row-major count x K,
- * int64), as drawn on the device by holo_transform_mc. */
+ * int64), as drawn on the device by holo_transform_mc; with HOLO_MC_SHARED,
+ * the subsets of groups m0 .. m0+count-1. */
 holo_status holo_mc_indices(holo_ctx *ctx, uint64_t seed, uint64_t N,
                             size_t m0, size_t count, uint32_t K,
                             int64_t *idx, unsigned flags);

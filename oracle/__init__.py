@@ -60,10 +60,12 @@ def lib() -> ctypes.CDLL:
         L.oracle_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
         L.oracle_mc_index.restype = ctypes.c_uint64
         L.oracle_mc_index.argtypes = [ctypes.c_uint64] * 4
+        L.oracle_mc_shared_index.restype = ctypes.c_uint64
+        L.oracle_mc_shared_index.argtypes = [ctypes.c_uint64] * 4
         L.oracle_transform_mc.restype = None
         L.oracle_transform_mc.argtypes = [
             ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_size_t,
-            _dp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64,
+            _dp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
             _i64p, ctypes.c_size_t, _dp, ctypes.c_int]
         L.oracle_adjoint.restype = None
         L.oracle_adjoint.argtypes = [
@@ -88,10 +90,8 @@ def wavenumber(lam: float) -> float:
 def pair(sign: int, lam: float, a: float, phi: float, u, v):
     """One pair -> (re, im, d, arg) with the pinned operation order."""
     u = _f64(u); v = _f64(v)
-    out = (ctypes.c_double * 4)()
     p = [ctypes.pointer(ctypes.c_double(0)) for _ in range(4)]
     lib().oracle_pair(sign, wavenumber(lam), a, phi, _d(u), _d(v), *p)
-    del out
     return tuple(x.contents.value for x in p)
 
 
@@ -142,8 +142,12 @@ def mc_index(seed: int, m: int, k: int, N: int) -> int:
     return int(lib().oracle_mc_index(seed, m, k, N))
 
 
+def mc_shared_index(seed: int, g: int, k: int, N: int) -> int:
+    return int(lib().oracle_mc_shared_index(seed, g, k, N))
+
+
 def transform_mc(sign: int, lam: float, amp, phase, u, v, K: int, seed: int,
-                 targets=None, nthreads: int = 0) -> np.ndarray:
+                 targets=None, nthreads: int = 0, shared: bool = False) -> np.ndarray:
     amp = _f64(amp); phase = _f64(phase); u = _f64(u).reshape(-1); v = _f64(v).reshape(-1)
     N, M = amp.size, v.size // 3
     if targets is not None:
@@ -153,7 +157,7 @@ def transform_mc(sign: int, lam: float, amp, phase, u, v, K: int, seed: int,
         cnt, tp = M, None
     y = np.zeros(2 * cnt)
     lib().oracle_transform_mc(sign, lam, _d(amp), _d(phase), _d(u), N, _d(v), M,
-                              K, seed, tp, cnt, _d(y), nthreads)
+                              K, seed, 1 if shared else 0, tp, cnt, _d(y), nthreads)
     return y.view(np.complex128)
 
 

@@ -213,12 +213,30 @@ uint64_t oracle_mc_index(uint64_t seed, uint64_t m, uint64_t k, uint64_t N)
     return (uint64_t)(((unsigned __int128)r * N) >> 64);
 }
 
-/* MC estimator in FP64 with the pinned pair: (N/K) sum_k pair(n_{m,k}, m). */
+/* Shared-subset ("Batched MC") index: group g's k-th draw, counter
+ * (k/2, g, 0x80000000, 0x4d43). */
+#define ORACLE_MC_SHARED_TAG 0x80000000u
+#define ORACLE_MC_GROUP 1024u
+
+uint64_t oracle_mc_shared_index(uint64_t seed, uint64_t g, uint64_t k, uint64_t N)
+{
+    const uint32_t ctr[4] = {(uint32_t)(k >> 1), (uint32_t)g, ORACLE_MC_SHARED_TAG,
+                             ORACLE_MC_STREAM};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    oracle_philox4x32_10(ctr, key, o);
+    const uint64_t r = (k & 1) ? ((uint64_t)o[3] << 32 | o[2])
+                               : ((uint64_t)o[1] << 32 | o[0]);
+    return (uint64_t)(((unsigned __int128)r * N) >> 64);
+}
+
+/* MC estimator in FP64 with the pinned pair: (N/K) sum_k pair(n_{m,k}, m).
+ * shared != 0 uses the group subsets n_{m/1024, k}. */
 void oracle_transform_mc(int sign, double lambda,
                          const double *amp, const double *phase,
                          const double *u, size_t N,
                          const double *v, size_t M,
-                         uint32_t K_samples, uint64_t seed,
+                         uint32_t K_samples, uint64_t seed, int shared,
                          const int64_t *targets, size_t M_sel,
                          double *y, int nthreads)
 {
@@ -233,7 +251,8 @@ void oracle_transform_mc(int sign, double lambda,
         const size_t m = targets ? (size_t)targets[j] : j;
         double sr = 0.0, si = 0.0;
         for (uint32_t k = 0; k < K_samples; ++k) {
-            const uint64_t n = oracle_mc_index(seed, m, k, N);
+            const uint64_t n = shared ? oracle_mc_shared_index(seed, m / ORACLE_MC_GROUP, k, N)
+                                      : oracle_mc_index(seed, m, k, N);
             double re, im;
             oracle_pair(sign, K, amp[n], phase[n], &u[3 * n], &v[3 * m],
                         &re, &im, NULL, NULL);

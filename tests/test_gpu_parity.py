@@ -287,9 +287,24 @@ def test_plan_info_grid(pkg, ctx, workloads):
 def test_mc_indices_match_host_replica(pkg, ctx, oracle):
     for seed, N in ((0, 1 << 20), (123456789012345, 1000), (2 ** 63 + 5, 7)):
         idx = ctx.mc_indices(seed, N, 1000, 64, 33)
+        sidx = ctx.mc_indices(seed, N, 3, 5, 33, shared=True)
         for j in range(0, 64, 7):
             for k in range(33):
                 assert idx[j, k] == oracle.mc_index(seed, 1000 + j, k, N)
+        for j in range(5):
+            for k in range(33):
+                assert sidx[j, k] == oracle.mc_shared_index(seed, 3 + j, k, N)
+
+
+@pytest.mark.parametrize("K", [1, 7, 8, 257])
+@pytest.mark.parametrize("shared", [False, True])
+def test_mc_fp64_ragged_k_matches_oracle(pkg, ctx, oracle, K, shared):
+    """K below, at and off the 8-sample batch; shared groups of 1024 targets."""
+    amp, phase, u, v = rand_geom(19, 3000, 2500)
+    y = ctx.transform_mc(amp, phase, u, v, K, 11, -1, LAM, pkg.FP64, shared=shared)
+    sub = np.arange(0, 2500, 13)
+    ref = oracle.transform_mc(-1, LAM, amp, phase, u, v, K, 11, targets=sub, shared=shared)
+    assert rel_l2(y[sub], ref) <= 1e-12
 
 
 @pytest.mark.parametrize("sign", [+1, -1])
@@ -301,21 +316,30 @@ def test_mc_fp64_matches_oracle(pkg, ctx, oracle, sign):
     assert rel_l2(y[sub], ref) <= 1e-12
 
 
-def test_mc_fp32_split_matches_fp64(pkg, ctx):
+@pytest.mark.parametrize("shared", [False, True])
+def test_mc_fp32_split_matches_fp64(pkg, ctx, shared):
     amp, phase, u, v = rand_geom(14, 5000, 3000)
-    y64 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP64)
-    y32 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP32_SPLIT)
+    y64 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP64, shared=shared)
+    y32 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP32_SPLIT, shared=shared)
     assert rel_l2(y32, y64) <= 1e-6
 
 
-def test_mc_statistics_vs_brute_force(pkg, ctx):
+@pytest.mark.parametrize("shared", [False, True])
+def test_mc_statistics_vs_brute_force(pkg, ctx, shared):
     """Error ~ 1/sqrt(K): the relative L2 error against brute force falls by
-    ~2 for every 4x more samples (tolerances from the estimator's variance)."""
+    ~2 for every 4x more samples, and matches the estimator's predicted
+    spread sqrt((N sum|t|^2 - |y|^2) / K) / |y| within 10 %."""
     amp, phase, u, v = rand_geom(15, 65536, 4096)
     y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP64)
+    d2 = np.sum((v[:, None, :] - u[None, ::64, :]) ** 2, axis=2)  # strided sample
+    t2 = np.mean(amp[None, ::64] ** 2 / d2, axis=1) * amp.size    # sum_n |t_n|^2 per target
     errs = []
     for K in (256, 1024, 4096):
-        e = rel_l2(ctx.transform_mc(amp, phase, u, v, K, 1, +1, LAM, pkg.FP32_SPLIT), y)
+        e = rel_l2(ctx.transform_mc(amp, phase, u, v, K, 1, +1, LAM, pkg.FP32_SPLIT,
+                                    shared=shared), y)
+        pred = np.sqrt(np.sum((amp.size * t2 - np.abs(y) ** 2) / K)) / np.linalg.norm(y)
+        if not shared:  # independent targets: the L2 error concentrates
+            assert abs(e / pred - 1) < 0.1
         errs.append(e)
     assert 1.7 < errs[0] / errs[1] < 2.3 and 1.7 < errs[1] / errs[2] < 2.3
 

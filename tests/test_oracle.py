@@ -254,6 +254,41 @@ def test_mc_index_matches_python_restatement(oracle):
         assert oracle.mc_index(seed, m, k, N) == py_mc_index(seed, m, k, N)
 
 
+def test_mc_shared_index_matches_python_restatement(oracle):
+    """Batched MC: group g's draws use counter (k/2, g, 0x80000000, 0x4d43)."""
+    g = np.random.default_rng(12)
+    for _ in range(200):
+        seed = int(g.integers(0, 2 ** 63))
+        grp = int(g.integers(0, 2 ** 21))
+        k = int(g.integers(0, 2 ** 20))
+        N = int(g.integers(1, 2 ** 31))
+        o = py_philox([k >> 1, grp, 0x80000000, 0x4d43], [seed & 0xFFFFFFFF, seed >> 32])
+        r = (o[3] << 32 | o[2]) if (k & 1) else (o[1] << 32 | o[0])
+        assert oracle.mc_shared_index(seed, grp, k, N) == (r * N) >> 64
+    # shared and per-target streams differ
+    assert [oracle.mc_shared_index(5, 0, k, 1 << 30) for k in range(8)] != \
+           [oracle.mc_index(5, 0, k, 1 << 30) for k in range(8)]
+
+
+def test_mc_shared_estimator_unbiased(oracle):
+    amp, phase, u, v = rand_geom(18, 64, 3)
+    y = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    K, S = 16, 400
+    est = np.stack([oracle.transform_mc(+1, LAM, amp, phase, u, v, K, seed, shared=True)
+                    for seed in range(S)])
+    mean = est.mean(axis=0)
+    sem = est.std(axis=0) / math.sqrt(S)
+    assert np.all(np.abs(mean - y) < 5 * np.abs(sem) + 1e-12)
+    # all targets of one group use the same indices: the estimate of a target
+    # equals (N/K) sum over the group's subset
+    idx = [oracle.mc_shared_index(3, 0, k, 64) for k in range(K)]
+    e = oracle.transform_mc(+1, LAM, amp, phase, u, v, K, 3, shared=True)
+    for m in range(3):
+        ref = sum(oracle.transform_full(+1, LAM, amp[[n]], phase[[n]], u[[n]], v[[m]])[0]
+                  for n in idx)
+        assert abs(e[m] - 64 / K * ref) <= 1e-12 * abs(e[m])
+
+
 def test_mc_indices_uniform(oracle):
     N = 97
     counts = np.zeros(N)
