@@ -2,8 +2,9 @@
 
 Projector phases theta (amplitude fixed at 1) on an n_proj^2 plane are
 propagated to an n_img^2 image plane; the loss is the MSE between the
-normalised intensity I / sum(I) and a normalised target T / sum(T) (sum
-normalisation keeps the loss differentiable everywhere, unlike a max).
+normalised intensity I / mean(I) and the target T / mean(T) (mean
+normalisation keeps the loss differentiable everywhere, unlike a max, and of
+order one, so Adam's epsilon does not swamp the gradients).
 Adam (lr 0.05) updates theta; gradients come from the hand-written adjoint.
 
     python -m holo_amd.hologram --steps 200       (via __graft_entry__.load_package)
@@ -22,8 +23,9 @@ from .workloads import hologram as hologram_workload
 
 
 def intensity_loss(z: torch.Tensor, target: torch.Tensor) -> torch.Tensor:
+    """MSE between I / mean(I) and `target` (already divided by its mean)."""
     I = z[:, 0] ** 2 + z[:, 1] ** 2
-    return torch.mean((I / I.sum() - target) ** 2)
+    return torch.mean((I / I.mean() - target) ** 2)
 
 
 @dataclass
@@ -35,14 +37,14 @@ class HologramResult:
 
 def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 1024,
              n_img: int = 512, precision: int = FP32_SPLIT, device: int = 0,
-             warmup: int = 1, log_every: int = 0) -> HologramResult:
-    wl = hologram_workload(seed, n_proj, n_img)
+             warmup: int = 1, log_every: int = 0, z: float = 0.35) -> HologramResult:
+    wl = hologram_workload(seed, n_proj, n_img, z)
     st = wl.stages[0]
     dev = torch.device("cuda", device)
     u = torch.from_numpy(st.u).to(dev)
     v = torch.from_numpy(st.v).to(dev)
     prop = Propagator(u, v, st.sign, st.lam, precision)
-    target = torch.from_numpy(wl.extra["target"] / wl.extra["target"].sum()).to(dev)
+    target = torch.from_numpy(wl.extra["target"] / wl.extra["target"].mean()).to(dev)
     amp = torch.ones(u.shape[0], dtype=torch.float64, device=dev)
     theta = torch.from_numpy(st.phase.copy()).to(dev).requires_grad_(True)
     opt = torch.optim.Adam([theta], lr=lr)
@@ -62,12 +64,12 @@ def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 10
     for i in range(steps):
         loss = step()
         if log_every and i % log_every == 0:
-            losses.append(float(loss))
+            losses.append(float(loss.detach()))
             print(f"step {i}: loss {losses[-1]:.4e}", flush=True)
     torch.cuda.synchronize(dev)
     dt = time.perf_counter() - t0
     if not log_every:
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     return HologramResult(losses, 1e3 * dt / max(steps, 1), theta.detach())
 
 

@@ -110,7 +110,7 @@ def plane_1024(seed: int = 0, n_src: int = 1024, n_tgt: int = 1024) -> Workload:
                            grid(n_tgt, n_tgt, 7.56e-6, 0.35), +1)])
 
 
-def hologram(seed: int = 0, n_proj: int = 1024, n_img: int = 512) -> Workload:
+def hologram(seed: int = 0, n_proj: int = 1024, n_img: int = 512, z: float = 0.35) -> Workload:
     """Phase-only hologram: theta on a 1024^2 projector (z=0, 7.56 um, amp 1,
     theta U[0,2pi)) -> 512^2 image plane (z=0.35 m, 10 um); target intensity
     = 64 Gaussian spots (sigma 3 px)."""
@@ -118,7 +118,7 @@ def hologram(seed: int = 0, n_proj: int = 1024, n_img: int = 512) -> Workload:
     theta = g.random(n_proj * n_proj) * 2 * np.pi
     target = gaussian_spots(n_img, 64, 3.0, seed + 1)
     st = Stage(np.ones(n_proj * n_proj), theta, grid(n_proj, n_proj, 7.56e-6, 0.0),
-               grid(n_img, n_img, 10e-6, 0.35), +1)
+               grid(n_img, n_img, 10e-6, z), +1)
     return Workload(f"hologram_{n_proj}^2_to_{n_img}^2", [st], {"target": target})
 
 
