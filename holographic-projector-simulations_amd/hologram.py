@@ -11,6 +11,7 @@ Adam (lr 0.05) updates theta; gradients come from the hand-written adjoint.
 """
 from __future__ import annotations
 
+import sys
 import time
 from dataclasses import dataclass
 
@@ -65,7 +66,7 @@ def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 10
         loss = step()
         if log_every and i % log_every == 0:
             losses.append(float(loss.detach()))
-            print(f"step {i}: loss {losses[-1]:.4e}", flush=True)
+            print(f"step {i}: loss {losses[-1]:.4e}", file=sys.stderr, flush=True)
     torch.cuda.synchronize(dev)
     dt = time.perf_counter() - t0
     if not log_every:
