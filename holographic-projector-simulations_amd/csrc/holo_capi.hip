@@ -212,7 +212,15 @@ void emit_tile(const double* u, const int32_t* idx, size_t cnt, unsigned T,
     }
   double c[3];
   for (int d = 0; d < 3; ++d) c[d] = 0.5 * (blo[d] + bhi[d]);
-  out.centre.push_back(double4{c[0], c[1], c[2], 0.0});
+  double rho_xy = 0.0, rho_z = 0.0;
+  for (size_t s = 0; s < cnt; ++s) {
+    const double* p = &u[3 * (size_t)idx[s]];
+    rho_xy = std::max(rho_xy, std::hypot(p[0] - c[0], p[1] - c[1]));
+    rho_z = std::max(rho_z, std::fabs(p[2] - c[2]));
+  }
+  // radii rounded up to floats, packed into .w
+  const float2 rho{std::nextafter((float)rho_xy, INFINITY), std::nextafter((float)rho_z, INFINITY)};
+  out.centre.push_back(double4{c[0], c[1], c[2], __builtin_bit_cast(double, rho)});
   for (size_t s = 0; s < T; ++s) {
     if (s < cnt) {
       const int32_t n = idx[s];
@@ -403,6 +411,7 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     ts.centre = static_cast<const double4*>(p->centre.ptr);
     ts.geo = static_cast<const float4*>(p->geo.ptr);
     ts.perm = static_cast<const int32_t*>(p->perm.ptr);
+    ts.u = static_cast<const double*>(p->u.ptr);
     ts.n_tiles = (uint32_t)p->n_tiles;
     ts.tile = p->tile;
     ts.planar = p->planar;

@@ -15,9 +15,10 @@ enum WeightForm { kPolar = 0, kCartesian = 1 };
 // relative to the tile centre centre[t] = (cx, cy, cz, 0); perm[slot] is the
 // caller's source index or -1 for a padding slot (weight 0).
 struct TileSet {
-  const double4* centre = nullptr;
+  const double4* centre = nullptr;  // .w: bit-cast float2 (lateral, axial radius)
   const float4* geo = nullptr;
   const int32_t* perm = nullptr;
+  const double* u = nullptr;  // the sources' FP64 positions (AoS), for exact tiles
   uint32_t n_tiles = 0;
   uint32_t tile = 0;
   bool planar = false;  // every slot has delta_z == 0

@@ -194,10 +194,56 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
 // Valid in the far field of each tile: |q| / d_ref^2 << 1 (tile radius much
 // smaller than the distance) and |t| of order one cycle.
 
+// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
+// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).
+template <int SIGN>
+__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
+                                             double inv_lambda, float& fr, float& fi) {
+  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+  double d, id;
+  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  const float f = (float)frac_cycle(d * inv_lambda);
+  float sn, cs;
+  sincos_rev_f32(f, sn, cs);
+  if (SIGN < 0) sn = -sn;
+  const float idf = (float)id;
+  const float2 w = __builtin_bit_cast(float2, p.w);
+  const float pr = idf * cs, pi = idf * sn;
+  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
+  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+}
+
+// Accuracy guard: per (target, tile) the bound q_max of |q| over the tile
+// (from the tile's lateral and axial radii) gives the FP32 rounding error of
+// t, ~eps32 q_max / (2 d lambda) cycles, and the truncation of the q-series,
+// q_max^3 / (16 d^5 lambda) cycles.  If either exceeds kGuardCycles (thick
+// 3-D tiles, near field) that target evaluates the tile exactly: FP64
+// distance from the source's FP64 position, FP64 fractional cycle, FP32
+// sin/cos and weights.
+constexpr float kGuardCycles = 2e-7f;
+
+template <int SIGN>
+__device__ __forceinline__ void exact_tile_pairs(
+    const double* __restrict__ u, const int32_t* __restrict__ perm,
+    const float2* __restrict__ w, int T, double vx, double vy, double vz,
+    double inv_lambda, double& ar, double& ai) {
+  float fr = 0.f, fi = 0.f;
+  for (int j = 0; j < T; ++j) {
+    const int32_t n = perm[j];
+    if (n < 0) continue;
+    const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
+                                   __builtin_bit_cast(double, w[j]));
+    mc_pair_f32s<SIGN>(p, vx, vy, vz, inv_lambda, fr, fi);
+  }
+  ar += fr;
+  ai += fi;
+}
+
 template <int SIGN, int R, int T, bool PLANAR>
 __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     const double4* __restrict__ centre, const float4* __restrict__ geo,
-    const float2* __restrict__ wc, uint32_t n_tiles, uint32_t chunk_tiles,
+    const float2* __restrict__ wc, const int32_t* __restrict__ perm,
+    const double* __restrict__ u, uint32_t n_tiles, uint32_t chunk_tiles,
     const double* __restrict__ v, uint32_t M, double lambda,
     double inv_lambda, double2* __restrict__ out) {
   const uint32_t t0 = blockIdx.y * chunk_tiles;
@@ -216,13 +262,27 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
   }
   for (uint32_t t = t0; t < t1; ++t) {
     const double4 c = centre[t];
+    const float2 rho = __builtin_bit_cast(float2, c.w);  // lateral, axial tile radius
     float m2rx[R], m2ry[R], m2rz[R], alpha[R], c2[R], gq[R], invd[R];
     float ct[R], st[R], sr[R], si[R];
+    bool near[R];
+    bool any_near = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const double rx = vx[r] - c.x, ry = vy[r] - c.y, rz = vz[r] - c.z;
       double dref, id;
       hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      {
+        const float rxy = __builtin_sqrtf((float)__builtin_fma(ry, ry, rx * rx));
+        const float qmax = 2.f * (rxy * rho.x + __builtin_fabsf((float)rz) * rho.y) +
+                           rho.x * rho.x + rho.y * rho.y;
+        const float ia = (float)(id * inv_lambda);  // 1 / (d lambda)
+        const float idf = (float)id;
+        const float round_err = 3e-8f * qmax * ia;
+        const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
+        near[r] = round_err > kGuardCycles || trunc_err > 0.05f * kGuardCycles;
+        any_near |= near[r];
+      }
       const float f = (float)frac_cycle(dref * inv_lambda);
       sincos_rev_f32(f, st[r], ct[r]);
       m2rx[r] = (float)(-2.0 * rx);
@@ -265,10 +325,17 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const double xr = sr[r], xi = si[r];
+      const double xr = near[r] ? 0.0 : sr[r], xi = near[r] ? 0.0 : si[r];
       const double cr = ct[r], cs = SIGN > 0 ? (double)st[r] : -(double)st[r];
       ar[r] += xr * cr - xi * cs;
       ai[r] += xr * cs + xi * cr;
+    }
+    if (any_near) {  // rare: thick tiles or near field
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (near[r])
+          exact_tile_pairs<SIGN>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r], inv_lambda,
+                                 ar[r], ai[r]);
     }
   }
   double2* dst = out + (size_t)blockIdx.y * M;
@@ -288,15 +355,15 @@ static void launch_f32s_r(int R, dim3 grid, const TileSet& ts,
   switch (R) {
     case 1:
       k_superpose_f32s<SIGN, 1, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
     case 4:
       k_superpose_f32s<SIGN, 4, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
     default:
       k_superpose_f32s<SIGN, 2, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
+          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
       break;
   }
 }
@@ -381,25 +448,6 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 // arrive through scalar loads.
 
 constexpr uint32_t kMcSharedTag = 0x80000000u;
-
-// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
-// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).
-template <int SIGN>
-__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
-                                             double inv_lambda, float& fr, float& fi) {
-  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
-  double d, id;
-  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
-  const float f = (float)frac_cycle(d * inv_lambda);
-  float sn, cs;
-  sincos_rev_f32(f, sn, cs);
-  if (SIGN < 0) sn = -sn;
-  const float idf = (float)id;
-  const float2 w = __builtin_bit_cast(float2, p.w);
-  const float pr = idf * cs, pi = idf * sn;
-  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
-  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
-}
 
 template <int SIGN, bool FP64>
 __global__ __launch_bounds__(kBlock) void k_mc(

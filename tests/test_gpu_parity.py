@@ -272,6 +272,35 @@ def test_fp32_split_random_cloud(pkg, ctx, oracle):
     assert rel_l2(y, ref) <= 1e-6
 
 
+def test_fp32_split_thick_cloud_uses_exact_tiles(pkg, ctx, oracle):
+    """Sources in a 100 um thick slab: the FP32 error of the per-pair offset
+    would reach ~1e-5 (the earlier failure); the accuracy guard evaluates
+    those tiles exactly and the result meets the FP32-split bar."""
+    g = np.random.default_rng(12)
+    N, M = 20000, 300
+    u = (g.random((N, 3)) - 0.5) * np.array([4e-3, 4e-3, 1e-4])
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * 4e-3
+    v[:, 2] = 0.3
+    amp, phase = g.random(N), g.random(N) * 2 * np.pi
+    y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("z", [1e-3, 5e-3])
+def test_fp32_split_near_field(pkg, ctx, oracle, workloads, z):
+    """Targets a few mm from a 10 um lattice: the q-series would be truncated
+    far above the bar; the guard switches those tiles to exact pairs."""
+    u = workloads.grid(96, 96, 10e-6, 0.0)
+    v = workloads.grid(40, 40, 25e-6, z)
+    g = np.random.default_rng(22)
+    amp, phase = g.random(u.shape[0]), g.random(u.shape[0]) * 2 * np.pi
+    y = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(-1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-6
+
+
 def test_plan_info_grid(pkg, ctx, workloads):
     st = workloads.plane_1024(0).stages[0]
     info = ctx.plan(st.u, 16).info()
