@@ -135,3 +135,27 @@ def test_generated_mode_writes_reference_files(tmp_path):
     assert w.shape == (1024, 3) and np.allclose(w[:, 2], 0.35)
     u = np.fromfile(out / "u0.dat", "<f8").reshape(-1, 3)
     assert np.allclose(np.hypot(u[:, 0], u[:, 1]), 0.5e-3)  # circle of width 1 mm
+
+
+@pytest.mark.gpu
+def test_reference_wave_mode(tmp_path, oracle):
+    """HOLO_PROJECT_PHASE=0 (the reference's PROJECT_PHASE=0 build,
+    cuda/transform.cu:379-395): y is normalised, a unit point source 0.25 m
+    behind the v plane adds a normalised reference wave, and the projector
+    keeps |y + y_ref| with the reference wave's phase."""
+    stage_golden(tmp_path)
+    v = np.fromfile(tmp_path / "v0.dat", "<f8")
+    M = v.size // 3
+    r = run(["-f", str(tmp_path), "-y", str(M)], env={"HOLO_PROJECT_PHASE": "0"})
+    assert r.returncode == 0, r.stderr
+    ya = np.fromfile(tmp_path / "y0_amp.dat", "<f8")
+    yp = np.fromfile(tmp_path / "y0_phase.dat", "<f8")
+    amp = np.fromfile(tmp_path / "x0_amp.dat", "<f8")
+    phase = np.fromfile(tmp_path / "x0_phase.dat", "<f8")
+    u = np.fromfile(tmp_path / "u0.dat", "<f8")
+    y = oracle.transform_full(-1, LAM_REF, amp, phase, u, v)
+    y = y / np.abs(y).max()
+    ref = oracle.transform_full(+1, LAM_REF, [1.0], [0.0], [0.0, 0.0, v[2] - 0.25], v)
+    ref = ref / np.abs(ref).max()
+    assert np.max(np.abs(ya - np.abs(y + ref))) < 1e-12
+    assert np.max(np.abs(np.angle(np.exp(1j * (yp - np.angle(ref)))))) < 1e-9
