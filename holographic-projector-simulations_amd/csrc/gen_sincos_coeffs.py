@@ -1,7 +1,7 @@
 """Derives the FP64 sin/cos polynomial coefficients used in holo_math.hpp.
 
 sin r = r + r^3 * P(r^2),  cos r = 1 + r^2 * Q(r^2)  on |r| <= pi/2 (+ margin),
-with P of degree 6 and Q of degree 7 in u = r^2, fitted by weighted least
+with P of degree 6 and Q of degree 6 in u = r^2, fitted by weighted least
 squares at 50-digit precision on Chebyshev nodes in u, weights chosen so
 that the fitted quantity is the absolute error of sin r and cos r.  Prints
 the coefficients as hex doubles and the maximum absolute error of the
@@ -42,7 +42,7 @@ def cos_target(u):  # (cos r - 1) / r^2
 
 def main():
     P = fit(sin_target, lambda u: mp.sqrt(u) ** 3, 6)
-    Q = fit(cos_target, lambda u: u, 7)
+    Q = fit(cos_target, lambda u: u, 6)
     Pd = [float(c) for c in P]
     Qd = [float(c) for c in Q]
     err_s = err_c = mp.mpf(0)
@@ -54,7 +54,7 @@ def main():
         err_s = max(err_s, abs(r + r ** 3 * ps - mp.sin(r)))
         err_c = max(err_c, abs(1 + u * pc - mp.cos(r)))
     print("sin P (u^0 .. u^6):", ", ".join(c.hex() for c in Pd))
-    print("cos Q (u^0 .. u^7):", ", ".join(c.hex() for c in Qd))
+    print("cos Q (u^0 .. u^6):", ", ".join(c.hex() for c in Qd))
     print(f"max abs error with double coefficients: sin {float(err_s):.3e}, cos {float(err_c):.3e}")
 
 

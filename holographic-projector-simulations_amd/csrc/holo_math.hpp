@@ -8,9 +8,9 @@
 //   sin r = r + r^3 P(r^2),  cos r = 1 + r^2 Q(r^2)
 //   sin x = (-1)^k sin r,    cos x = (-1)^k cos r
 // PI1 = RN(pi), PI2 = RN(pi - PI1); the neglected third term is 3e-33, so for
-// |x| < 2^50 the reduced argument is within ~1.2e-16 of x - k pi.  P (degree
-// 6) and Q (degree 7) are least-squares fits on Chebyshev nodes computed by
-// gen_sincos_coeffs.py; their absolute error is 1.2e-16 (sin) and 3e-17
+// |x| < 2^50 the reduced argument is within ~1.2e-16 of x - k pi.  P and Q
+// (degree 6 in r^2) are least-squares fits on Chebyshev nodes computed by
+// gen_sincos_coeffs.py; their absolute error is 1.2e-16 (sin) and 2.2e-15
 // (cos) on [-pi/2, pi/2].  Reducing by pi instead of pi/2 leaves a common
 // sign (-1)^k and no sin/cos swap: hm_sincos_unsigned returns that sign as a
 // bit mask for the caller to fold into an amplitude with one xor.  Only
@@ -44,14 +44,13 @@ HM_HD void hm_sincos_unsigned(double x, double& s, double& c,
   p = __builtin_fma(p, u, 0x1.1111111106d2cp-7);
   p = __builtin_fma(p, u, -0x1.5555555555412p-3);
   s = __builtin_fma(r * u, p, r);
-  double q = 0x1.9f3d757541f50p-45;
-  q = __builtin_fma(q, u, -0x1.9351b15a30d5cp-37);
-  q = __builtin_fma(q, u, 0x1.1eece3c299f48p-29);
-  q = __builtin_fma(q, u, -0x1.27e4f98df5d2cp-22);
-  q = __builtin_fma(q, u, 0x1.a01a0199b870dp-16);
-  q = __builtin_fma(q, u, -0x1.6c16c16c0af4ep-10);
-  q = __builtin_fma(q, u, 0x1.5555555555403p-5);
-  q = __builtin_fma(q, u, -0x1.ffffffffffffdp-2);
+  double q = -0x1.837194dec695cp-37;
+  q = __builtin_fma(q, u, 0x1.1eadefa1f8550p-29);
+  q = __builtin_fma(q, u, -0x1.27e3f6aae8f33p-22);
+  q = __builtin_fma(q, u, 0x1.a019fd06a0196p-16);
+  q = __builtin_fma(q, u, -0x1.6c16c160f5ed5p-10);
+  q = __builtin_fma(q, u, 0x1.555555553bbacp-5);
+  q = __builtin_fma(q, u, -0x1.ffffffffffafdp-2);
   c = __builtin_fma(q, u, 1.0);
 }
 
@@ -67,12 +66,13 @@ HM_HD void hm_sincos(double x, double& s, double& c) {
   c = hm_apply_sign(c, m);
 }
 
-// d = sqrt(s), correctly rounded, and inv = 1/d to ~2e-15 relative, for
-// normal s.  Goldschmidt iteration from the hardware reciprocal square root,
-// then two Newton corrections of d using h ~ 1/(2 sqrt s) (Markstein's final
-// step) and one more refinement of h.  On gfx950, d matched the host's IEEE
-// sqrt on 1.7e7 random s in [1e-8, 20]; inv is used only for amplitudes.
-// On the host the seed comes from 1/sqrt(s).
+// d = sqrt(s), correctly rounded, and inv = 1/d to about an ulp, for normal
+// s.  Goldschmidt iteration from the hardware reciprocal square root, then
+// two Newton corrections of d using h ~ 1/(2 sqrt s) (Markstein's final
+// step), then one Newton step of the reciprocal 1/d from 2h:
+// inv = 2h + 2h (1 - 2 d h).  On gfx950, d matched the host's IEEE sqrt on
+// 1.7e7 random s in [1e-8, 20]; inv is used only for amplitudes.  On the host
+// the seed comes from 1/sqrt(s).
 HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
 #pragma clang fp contract(off)
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -89,10 +89,10 @@ HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
   g = __builtin_fma(rr, h, g);
   rr = __builtin_fma(-g, g, s);
   g = __builtin_fma(rr, h, g);
-  e = __builtin_fma(-g, h, 0.5);
-  h = __builtin_fma(h, e, h);
+  const double h2 = h + h;
+  e = __builtin_fma(-g, h2, 1.0);
   d = g;
-  inv = h + h;
+  inv = __builtin_fma(h2, e, h2);
 }
 
 #undef HM_HD
