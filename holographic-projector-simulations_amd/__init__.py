@@ -33,6 +33,7 @@ FORWARDS = +1
 BACKWARDS = -1
 DEVICE_POINTERS = 0x1
 MC_SHARED = 0x2
+OBLIQUITY = 0x4             # amplitude x |dz|/d, matlab/holofunctions.m:226-227
 
 
 class Direction(enum.IntEnum):
@@ -234,8 +235,10 @@ class Context:
 
     # -- brute force ---------------------------------------------------------
     def transform_full(self, amp, phase, u, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                       precision: int = FP64):
-        """y[m] = sum_n amp_n/d exp(i(phase_n + sign 2 pi d/lam)); complex (M,)."""
+                       precision: int = FP64, obliquity: bool = False):
+        """y[m] = sum_n amp_n/d exp(i(phase_n + sign 2 pi d/lam)); complex (M,).
+        obliquity: each term also carries the factor |dz|/d."""
+        extra = OBLIQUITY if obliquity else 0
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)
         N, M = _n(amp, 1), _n(v, 3)
@@ -246,18 +249,20 @@ class Context:
             self._torch_stream()
             y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
             check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                            _ptr(u), N, _ptr(v), M, _ptr(y), DEVICE_POINTERS))
+                                            _ptr(u), N, _ptr(v), M, _ptr(y),
+                                            DEVICE_POINTERS | extra))
             return torch.view_as_complex(y)
         y = np.empty(2 * M)
         check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                        _ptr(u), N, _ptr(v), M, _ptr(y), 0))
+                                        _ptr(u), N, _ptr(v), M, _ptr(y), extra))
         return y.view(np.complex128)
 
     # -- Monte Carlo -----------------------------------------------------------
     def transform_mc(self, amp, phase, u, v, K: int, seed: int = 0, sign: int = FORWARDS,
-                     lam: float = LAMBDA, precision: int = FP32_SPLIT, shared: bool = False):
+                     lam: float = LAMBDA, precision: int = FP32_SPLIT, shared: bool = False,
+                     obliquity: bool = False):
         """Monte Carlo estimate with K draws per target (shared: per group of 1024)."""
-        extra = MC_SHARED if shared else 0
+        extra = (MC_SHARED if shared else 0) | (OBLIQUITY if obliquity else 0)
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)
         N, M = _n(amp, 1), _n(v, 3)
@@ -348,7 +353,7 @@ class Plan:
         return np.empty(2 * M)
 
     def transform(self, amp, phase, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                  precision: int = FP64, out=None):
+                  precision: int = FP64, out=None, obliquity: bool = False):
         device = _is_torch(v) and v.is_cuda
         amp, phase, v = _prep([amp, phase, v], device)
         M = _n(v, 3)
@@ -357,16 +362,16 @@ class Plan:
         y = out if out is not None else self._out(v, M, device)
         if device:
             self.ctx._torch_stream()
+        flags = (DEVICE_POINTERS if device else 0) | (OBLIQUITY if obliquity else 0)
         check(lib().holo_plan_transform(self.ctx.handle, self._h, precision, sign, lam,
-                                        _ptr(amp), _ptr(phase), _ptr(v), M, _ptr(y),
-                                        DEVICE_POINTERS if device else 0))
+                                        _ptr(amp), _ptr(phase), _ptr(v), M, _ptr(y), flags))
         if device:
             import torch
             return torch.view_as_complex(y) if y.dim() == 2 else y
         return y.view(np.complex128)
 
     def transform_complex(self, w, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                          precision: int = FP64):
+                          precision: int = FP64, obliquity: bool = False):
         device = _is_torch(v) and v.is_cuda
         w, v = _prep([w, v], device)
         M = _n(v, 3)
@@ -375,16 +380,16 @@ class Plan:
         y = self._out(v, M, device)
         if device:
             self.ctx._torch_stream()
+        flags = (DEVICE_POINTERS if device else 0) | (OBLIQUITY if obliquity else 0)
         check(lib().holo_plan_transform_complex(self.ctx.handle, self._h, precision, sign, lam,
-                                                _ptr(w), _ptr(v), M, _ptr(y),
-                                                DEVICE_POINTERS if device else 0))
+                                                _ptr(w), _ptr(v), M, _ptr(y), flags))
         if device:
             import torch
             return torch.view_as_complex(y)
         return y.view(np.complex128)
 
     def adjoint(self, g, amp, theta, u, sign: int = FORWARDS, lam: float = LAMBDA,
-                precision: int = FP64):
+                precision: int = FP64, obliquity: bool = False):
         """(dL/damp, dL/dtheta) for z = transform(amp, theta; u -> plan points)."""
         device = _is_torch(u) and u.is_cuda
         g, amp, theta, u = _prep([g, amp, theta, u], device)
@@ -400,7 +405,8 @@ class Plan:
             ga, gt = np.empty(N), np.empty(N)
         check(lib().holo_plan_adjoint(self.ctx.handle, self._h, precision, sign, lam, _ptr(g),
                                       _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt),
-                                      DEVICE_POINTERS if device else 0))
+                                      (DEVICE_POINTERS if device else 0)
+                                      | (OBLIQUITY if obliquity else 0)))
         return ga, gt
 
 

@@ -115,6 +115,11 @@ holo_status check_common(holo_ctx* ctx, int precision, int sign, double lambda,
   return HOLO_OK;
 }
 
+holo_status check_flags(unsigned flags, unsigned allowed) {
+  if (flags & ~allowed) return fail(HOLO_EINVAL, "unsupported flag bits for this call");
+  return HOLO_OK;
+}
+
 // Device view of an input array: the caller's pointer in device mode, a
 // staged copy otherwise.
 template <typename T>
@@ -392,14 +397,15 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
 
 // y (device) = sum over the plan's sources; weights already device-resident.
 holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
-                     double lambda, holo::WeightForm wf, const double* w0,
+                     bool obliquity, double lambda, holo::WeightForm wf, const double* w0,
                      const double* w1, const double* v, size_t M, double* y) {
   if (M == 0) return HOLO_OK;
   if (precision == HOLO_FP64) {
     const size_t pe = split_elems_f64(M, p->N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    HOLO_HIP(holo::launch_superpose_f64(sign, wf, w0, w1, static_cast<const double*>(p->u.ptr),
+    HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, w0, w1,
+                                        static_cast<const double*>(p->u.ptr),
                                         p->N, v, M, K, reinterpret_cast<double2*>(y),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));
@@ -417,7 +423,8 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     ts.planar = p->planar;
     const size_t pe = split_elems_f32(M, p->n_tiles);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
-    HOLO_HIP(holo::launch_superpose_f32s(sign, ts, static_cast<const float2*>(p->wc.ptr), v, M,
+    HOLO_HIP(holo::launch_superpose_f32s(sign, obliquity, ts,
+                                         static_cast<const float2*>(p->wc.ptr), v, M,
                                          lambda, reinterpret_cast<double2*>(y),
                                          static_cast<double2*>(ctx->partial.ptr), pe,
                                          ctx->stream));
@@ -540,8 +547,10 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
                                 const double* u, size_t N, const double* v, size_t M,
                                 double* y, unsigned flags) {
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
   const bool dev = flags & HOLO_DEVICE_POINTERS;
+  const bool obl = flags & HOLO_OBLIQUITY;
   if (M == 0) return HOLO_OK;
   double* dy;
   HOLO_TRY(out_buf(ctx, ctx->y, y, 2 * M, dev, &dy));
@@ -559,14 +568,15 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
     const size_t pe = split_elems_f64(M, N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, da, dp, du, N, dv, M, K,
+    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, da, dp, du, N, dv, M, K,
                                         reinterpret_cast<double2*>(dy),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));
   } else {
     holo_plan plan;
     HOLO_TRY(plan_build(ctx, &plan, u, N, 0, dev));
-    HOLO_TRY(plan_run(ctx, &plan, precision, sign, lambda, holo::kPolar, da, dp, dv, M, dy));
+    HOLO_TRY(plan_run(ctx, &plan, precision, sign, obl, lambda, holo::kPolar, da, dp, dv, M,
+                      dy));
     HOLO_HIP(hipStreamSynchronize(ctx->stream));  // plan buffers die here
   }
   return finish_out(ctx, y, dy, 2 * M, dev);
@@ -577,6 +587,7 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
                               const double* u, size_t N, const double* v, size_t M,
                               uint32_t K, uint64_t seed, double* y, unsigned flags) {
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_MC_SHARED | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
   if (K == 0) return fail(HOLO_EINVAL, "K must be positive");
   if (N == 0) return fail(HOLO_EINVAL, "Monte Carlo needs at least one source");
@@ -595,7 +606,7 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
   HOLO_HIP(holo::launch_mc_pack(fp64, da, dp, du, N, static_cast<double4*>(ctx->pos.ptr),
                                 static_cast<double*>(ctx->ph.ptr), ctx->stream));
   const double Kw = (2 * M_PI) / lambda;
-  HOLO_HIP(holo::launch_mc(sign, fp64, flags & HOLO_MC_SHARED,
+  HOLO_HIP(holo::launch_mc(sign, fp64, flags & HOLO_MC_SHARED, flags & HOLO_OBLIQUITY,
                            static_cast<const double4*>(ctx->pos.ptr),
                            static_cast<const double*>(ctx->ph.ptr), N, dv, M, K, seed, Kw,
                            lambda, reinterpret_cast<double2*>(dy), ctx->stream));
@@ -604,6 +615,7 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
 
 holo_status holo_mc_indices(holo_ctx* ctx, uint64_t seed, uint64_t N, size_t m0,
                             size_t count, uint32_t K, int64_t* idx, unsigned flags) {
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_MC_SHARED));
   HOLO_TRY(use_device(ctx));
   if (N == 0 || N > kMaxCount) return fail(HOLO_EINVAL, "N must be in [1, 2^31)");
   if (m0 + count > kMaxCount) return fail(HOLO_EINVAL, "target range beyond 2^31");
@@ -619,6 +631,7 @@ holo_status holo_mc_indices(holo_ctx* ctx, uint64_t seed, uint64_t N, size_t m0,
 
 holo_status holo_normalize_polar(holo_ctx* ctx, const double* y, size_t M, double* amp,
                                  double* phase, double* max_amp, unsigned flags) {
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS));
   HOLO_TRY(use_device(ctx));
   if (M > kMaxCount) return fail(HOLO_EINVAL, "M must be below 2^31");
   const bool dev = flags & HOLO_DEVICE_POINTERS;
@@ -649,6 +662,7 @@ holo_status holo_normalize_polar(holo_ctx* ctx, const double* y, size_t M, doubl
 
 holo_status holo_plan_create(holo_ctx* ctx, holo_plan** plan, const double* u, size_t N,
                              unsigned tile, unsigned flags) {
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS));
   HOLO_TRY(use_device(ctx));
   if (!plan) return fail(HOLO_EINVAL, "null plan pointer");
   *plan = nullptr;
@@ -681,6 +695,7 @@ static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int 
                                       size_t M, double* y, unsigned flags) {
   if (!plan) return fail(HOLO_EINVAL, "null plan");
   HOLO_TRY(check_common(ctx, precision, sign, lambda, plan->N, M));
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
   if (plan->device != ctx->device) return fail(HOLO_EINVAL, "plan belongs to another device");
   const bool dev = flags & HOLO_DEVICE_POINTERS;
@@ -701,7 +716,8 @@ static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int 
   } else {
     HOLO_TRY(stage_in(ctx, ctx->amp, w0, 2 * N, dev, &d0));
   }
-  HOLO_TRY(plan_run(ctx, plan, precision, sign, lambda, wf, d0, d1, dv, M, dy));
+  HOLO_TRY(plan_run(ctx, plan, precision, sign, flags & HOLO_OBLIQUITY, lambda, wf, d0, d1, dv,
+                    M, dy));
   return finish_out(ctx, y, dy, 2 * M, dev);
 }
 
@@ -728,6 +744,7 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
                               unsigned flags) {
   if (!plan_targets) return fail(HOLO_EINVAL, "null plan");
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, plan_targets->N));
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
   if (plan_targets->device != ctx->device)
     return fail(HOLO_EINVAL, "plan belongs to another device");
@@ -748,8 +765,9 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
     HOLO_HIP(holo::launch_zero(db, 2 * N, ctx->stream));
   } else {
     // b_n: the forward targets carry g and radiate back with the sign flipped
-    HOLO_TRY(plan_run(ctx, plan_targets, precision, -sign, lambda, holo::kCartesian, dg,
-                      nullptr, du, N, db));
+    // (the obliquity factor |dz| / d is symmetric in the pair)
+    HOLO_TRY(plan_run(ctx, plan_targets, precision, -sign, flags & HOLO_OBLIQUITY, lambda,
+                      holo::kCartesian, dg, nullptr, du, N, db));
   }
   HOLO_HIP(holo::launch_adjoint_epilogue(reinterpret_cast<const double2*>(db), da, dt, N, dga,
                                          dgt, ctx->stream));

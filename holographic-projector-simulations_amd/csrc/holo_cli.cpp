@@ -102,6 +102,7 @@ void show_help(const char* p) {  // cuda/input.h:7-39
          "\t HOLO_LAMBDA \t Wavelength in m (default 0.65e-6)\n"
          "\t HOLO_PROJECT_PHASE \t 0: amplitude-only projector with reference wave (default 1)\n"
          "\t HOLO_PRECISION \t fp64 (default) or fp32split\n"
+         "\t HOLO_OBLIQUITY \t 1: weight every pair by |dz|/d (default 0)\n"
          "\n", p);
 }
 
@@ -394,6 +395,11 @@ int main(int argc, char** argv) {
     if (v == "fp64") holo::settings().precision = HOLO_FP64;
     else if (v == "fp32split") holo::settings().precision = HOLO_FP32_SPLIT;
     else die("HOLO_PRECISION must be fp64 or fp32split");
+  }
+  if (const char* ob = getenv("HOLO_OBLIQUITY")) {
+    const std::string v(ob);
+    if (v == "1") holo::settings().obliquity = true;
+    else if (v != "0") die("HOLO_OBLIQUITY must be 0 or 1");
   }
   const char* pp = getenv("HOLO_PROJECT_PHASE");
   const bool project_phase = !(pp && std::string(pp) == "0");

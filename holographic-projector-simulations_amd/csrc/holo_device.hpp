@@ -27,7 +27,9 @@ struct PairF64 {
 
 // MATH = 0: hm_sqrt_inv (d correctly rounded, 1/d to ~1 ulp); MATH = 1: the
 // compiler's sqrt and IEEE division.  Both give the same d and arg bits.
-template <int SIGN, int MATH = 0>
+// OBL: amplitude times the obliquity factor |dz| / d
+// (matlab/holofunctions.m:226-227), r = (a / d) (|dz| / d).
+template <int SIGN, int MATH = 0, bool OBL = false>
 __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
                                             double uy, double uz, double vx,
                                             double vy, double vz, double K) {
@@ -42,10 +44,12 @@ __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
     hm_sqrt_inv(s, d, inv);
     p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);
     p.r = a * inv;
+    if constexpr (OBL) p.r *= __builtin_fabs(dz) * inv;
   } else {
     const double d = __builtin_sqrt(s);
     p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);
     p.r = a / d;
+    if constexpr (OBL) p.r *= __builtin_fabs(dz) / d;
   }
   return p;
 }

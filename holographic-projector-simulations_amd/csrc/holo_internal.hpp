@@ -40,15 +40,16 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
 
 // y (interleaved complex, M) = sum_n pair(n, m).  Polar weights: w0 = amp,
 // w1 = phase.  Cartesian: w0 = interleaved complex weights, w1 unused.
-// `partial` must hold S*M double2 when S > 1 (see partial_elems()).
-hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
+// obliquity: every pair's amplitude is also multiplied by |dz| / d.
+// `partial` holds partial_cap double2; S source splits need S*M of them.
+hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const double* w0,
                                 const double* w1, const double* u, size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream);
 
 // FP32-split: weights wc (float2 per slot), tiles as above.
-hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
+hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
                                  const float2* wc, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
                                  size_t partial_cap, hipStream_t stream);
@@ -63,7 +64,7 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 // FP32-split: pos = (x,y,z, bitcast float2 weight).
 constexpr unsigned kMcGroup = 1024;  // targets sharing one subset (batched MC)
 
-hipError_t launch_mc(int sign, bool fp64, bool shared, const double4* pos,
+hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
                      double2* y, hipStream_t stream);

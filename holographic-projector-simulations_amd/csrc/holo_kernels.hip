@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "holo_device.hpp"
 #include "holo_internal.hpp"
 
@@ -37,10 +39,36 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units) {
   return (int)std::max<size_t>(S, 1);
 }
 
+// Run-time -> compile-time dispatch: each helper calls f with a
+// std::integral_constant carrying the value.
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+static void with_bool(bool b, F&& f) {
+  if (b) f(IC<1>{});
+  else f(IC<0>{});
+}
+
+template <typename F>
+static void with_sign(int sign, F&& f) {
+  if (sign > 0) f(IC<+1>{});
+  else f(IC<-1>{});
+}
+
+template <typename F>
+static void with_r(int R, F&& f) {
+  switch (R) {
+    case 1: f(IC<1>{}); break;
+    case 4: f(IC<4>{}); break;
+    default: f(IC<2>{}); break;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 brute force.
 
-template <int SIGN, int R, int WF, int MATH>
+template <int SIGN, int R, int WF, int MATH, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     const double* __restrict__ w0, const double* __restrict__ w1,
     const double* __restrict__ u, uint32_t N, uint32_t chunk,
@@ -69,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
       const double phi = w1[n];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const PairF64 p = pair_f64<SIGN, MATH>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
+        const PairF64 p = pair_f64<SIGN, MATH, OBL>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
         accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r]);
       }
     } else {
@@ -80,6 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
         const double dx = vx[r] - ux, dy = vy[r] - uy, dz = vz[r] - uz;
         double d, id;
         hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+        if constexpr (OBL) id *= __builtin_fabs(dz) * id;
         const double arg = (SIGN > 0 ? d : -d) * K;
         double s, c;
         if constexpr (MATH == 0) {
@@ -125,32 +154,7 @@ static hipError_t reduce_if_split(int S, size_t M, const double2* partial,
   return hipGetLastError();
 }
 
-template <int SIGN, int R, int WF>
-static void launch_f64_t(dim3 grid, const double* w0, const double* w1,
-                         const double* u, size_t N, uint32_t chunk,
-                         const double* v, size_t M, double K, double2* out,
-                         hipStream_t stream) {
-  if (tuning().f64_libm)
-    k_superpose_f64<SIGN, R, WF, 1><<<grid, kBlock, 0, stream>>>(
-        w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
-  else
-    k_superpose_f64<SIGN, R, WF, 0><<<grid, kBlock, 0, stream>>>(
-        w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
-}
-
-template <int SIGN, int WF>
-static void launch_f64_r(int R, dim3 grid, const double* w0, const double* w1,
-                         const double* u, size_t N, uint32_t chunk,
-                         const double* v, size_t M, double K, double2* out,
-                         hipStream_t stream) {
-  switch (R) {
-    case 1: launch_f64_t<SIGN, 1, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
-    case 4: launch_f64_t<SIGN, 4, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
-    default: launch_f64_t<SIGN, 2, WF>(grid, w0, w1, u, N, chunk, v, M, K, out, stream); break;
-  }
-}
-
-hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
+hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const double* w0,
                                 const double* w1, const double* u, size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
@@ -161,16 +165,22 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
   int S = N ? choose_splits(M, R, N, 1024) : 1;
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = N ? (uint32_t)((N + S - 1) / S) : 0;
-  dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
-            (unsigned)S);
+  const dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
+                  (unsigned)S);
   double2* out = S > 1 ? partial : y;
-  if (sign > 0) {
-    if (wf == kPolar) launch_f64_r<+1, kPolar>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
-    else launch_f64_r<+1, kCartesian>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
-  } else {
-    if (wf == kPolar) launch_f64_r<-1, kPolar>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
-    else launch_f64_r<-1, kCartesian>(R, grid, w0, w1, u, N, chunk, v, M, K, out, stream);
-  }
+  with_sign(sign, [&](auto sg) {
+    with_r(R, [&](auto rr) {
+      with_bool(wf == kCartesian, [&](auto cart) {
+        with_bool(tuning().f64_libm, [&](auto libm) {
+          with_bool(obliquity, [&](auto obl) {
+            k_superpose_f64<sg.value, rr.value, cart.value ? kCartesian : kPolar, libm.value,
+                            (bool)obl.value><<<grid, kBlock, 0, stream>>>(
+                w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
+          });
+        });
+      });
+    });
+  });
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return reduce_if_split(S, M, partial, y, stream);
@@ -195,8 +205,9 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, const double* w0,
 // smaller than the distance) and |t| of order one cycle.
 
 // One sample of the FP32-split estimator: d and the fractional cycle in FP64,
-// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).
-template <int SIGN>
+// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).  OBL:
+// amplitude |dz| / d^2 instead of 1 / d.
+template <int SIGN, bool OBL = false>
 __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
                                              double inv_lambda, float& fr, float& fi) {
   const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
@@ -206,7 +217,7 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
   float sn, cs;
   sincos_rev_f32(f, sn, cs);
   if (SIGN < 0) sn = -sn;
-  const float idf = (float)id;
+  const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
   const float2 w = __builtin_bit_cast(float2, p.w);
   const float pr = idf * cs, pi = idf * sn;
   fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
@@ -222,7 +233,7 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
 // sin/cos and weights.
 constexpr float kGuardCycles = 2e-7f;
 
-template <int SIGN>
+template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_tile_pairs(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
     const float2* __restrict__ w, int T, double vx, double vy, double vz,
@@ -233,13 +244,13 @@ __device__ __forceinline__ void exact_tile_pairs(
     if (n < 0) continue;
     const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
                                    __builtin_bit_cast(double, w[j]));
-    mc_pair_f32s<SIGN>(p, vx, vy, vz, inv_lambda, fr, fi);
+    mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
   }
   ar += fr;
   ai += fi;
 }
 
-template <int SIGN, int R, int T, bool PLANAR>
+template <int SIGN, int R, int T, bool PLANAR, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     const double4* __restrict__ centre, const float4* __restrict__ geo,
     const float2* __restrict__ wc, const int32_t* __restrict__ perm,
@@ -263,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
   for (uint32_t t = t0; t < t1; ++t) {
     const double4 c = centre[t];
     const float2 rho = __builtin_bit_cast(float2, c.w);  // lateral, axial tile radius
-    float m2rx[R], m2ry[R], m2rz[R], alpha[R], c2[R], gq[R], invd[R];
+    float m2rx[R], m2ry[R], m2rz[R], alpha[R], c2[R], gq[R], invd[R], rzf[R];
     float ct[R], st[R], sr[R], si[R];
     bool near[R];
     bool any_near = false;
@@ -292,8 +303,18 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
       const double a = 0.5 * id * inv_lambda;
       alpha[r] = (float)a;
       c2[r] = (float)(-0.25 * a * id2);
-      gq[r] = (float)(-0.5 * id2 * id);
-      invd[r] = (float)id;
+      if constexpr (OBL) {
+        // |dz| / d^2 = |dz| (1/d_ref^2 - q/d_ref^4 + O(q^2)); planar tiles
+        // have dz = rz for every slot, so |rz| folds into the coefficients.
+        const double zs = PLANAR ? __builtin_fabs(rz) : 1.0;
+        invd[r] = (float)(zs * id2);
+        gq[r] = (float)(-zs * id2 * id2);
+        rzf[r] = (float)rz;
+      } else {
+        gq[r] = (float)(-0.5 * id2 * id);
+        invd[r] = (float)id;
+        rzf[r] = 0.f;
+      }
       sr[r] = 0.f;
       si[r] = 0.f;
     }
@@ -311,7 +332,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
         const float tt = q * __builtin_fmaf(c2[r], q, alpha[r]);
         float sn, cs;
         sincos_rev_f32(tt, sn, cs);
-        const float idd = __builtin_fmaf(q, gq[r], invd[r]);
+        float idd = __builtin_fmaf(q, gq[r], invd[r]);
+        if constexpr (OBL && !PLANAR) idd *= __builtin_fabsf(rzf[r] - gj.z);
         const float pr = idd * cs;
         const float pi = idd * sn;
         if constexpr (SIGN > 0) {
@@ -334,8 +356,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (near[r])
-          exact_tile_pairs<SIGN>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r], inv_lambda,
-                                 ar[r], ai[r]);
+          exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r],
+                                      inv_lambda, ar[r], ai[r]);
     }
   }
   double2* dst = out + (size_t)blockIdx.y * M;
@@ -346,43 +368,18 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
   }
 }
 
-template <int SIGN, int T, bool PLANAR>
-static void launch_f32s_r(int R, dim3 grid, const TileSet& ts,
-                          const float2* wc, uint32_t chunk, const double* v,
-                          size_t M, double lambda, double2* out,
-                          hipStream_t stream) {
-  const double il = 1.0 / lambda;
-  switch (R) {
-    case 1:
-      k_superpose_f32s<SIGN, 1, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
-      break;
-    case 4:
-      k_superpose_f32s<SIGN, 4, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
-      break;
-    default:
-      k_superpose_f32s<SIGN, 2, T, PLANAR><<<grid, kBlock, 0, stream>>>(
-          ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles, chunk, v, (uint32_t)M, lambda, il, out);
-      break;
+template <typename F>
+static bool with_tile(unsigned T, F&& f) {
+  switch (T) {
+    case 8: f(IC<8>{}); return true;
+    case 16: f(IC<16>{}); return true;
+    case 32: f(IC<32>{}); return true;
+    case 64: f(IC<64>{}); return true;
+    default: return false;
   }
 }
 
-template <int T>
-static void launch_f32s_t(int sign, bool planar, int R, dim3 grid,
-                          const TileSet& ts, const float2* wc, uint32_t chunk,
-                          const double* v, size_t M, double lambda,
-                          double2* out, hipStream_t stream) {
-  if (sign > 0) {
-    if (planar) launch_f32s_r<+1, T, true>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-    else launch_f32s_r<+1, T, false>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-  } else {
-    if (planar) launch_f32s_r<-1, T, true>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-    else launch_f32s_r<-1, T, false>(R, grid, ts, wc, chunk, v, M, lambda, out, stream);
-  }
-}
-
-hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
+hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
                                  const float2* wc, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
                                  size_t partial_cap, hipStream_t stream) {
@@ -392,16 +389,24 @@ hipError_t launch_superpose_f32s(int sign, const TileSet& ts,
   int S = ts.n_tiles ? choose_splits(M, R, ts.n_tiles, 64) : 1;
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = ts.n_tiles ? (ts.n_tiles + S - 1) / S : 0;
-  dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
-            (unsigned)S);
+  const dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
+                  (unsigned)S);
   double2* out = S > 1 ? partial : y;
-  switch (ts.tile) {
-    case 8: launch_f32s_t<8>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
-    case 16: launch_f32s_t<16>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
-    case 32: launch_f32s_t<32>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
-    case 64: launch_f32s_t<64>(sign, ts.planar, R, grid, ts, wc, chunk, v, M, lambda, out, stream); break;
-    default: return hipErrorInvalidValue;
-  }
+  const double il = 1.0 / lambda;
+  const bool ok = with_tile(ts.tile, [&](auto tt) {
+    with_sign(sign, [&](auto sg) {
+      with_r(R, [&](auto rr) {
+        with_bool(ts.planar, [&](auto pl) {
+          with_bool(obliquity, [&](auto obl) {
+            k_superpose_f32s<sg.value, rr.value, tt.value, (bool)pl.value, (bool)obl.value>
+                <<<grid, kBlock, 0, stream>>>(ts.centre, ts.geo, wc, ts.perm, ts.u, ts.n_tiles,
+                                              chunk, v, (uint32_t)M, lambda, il, out);
+          });
+        });
+      });
+    });
+  });
+  if (!ok) return hipErrorInvalidValue;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   return reduce_if_split(S, M, partial, y, stream);
@@ -449,7 +454,7 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 
 constexpr uint32_t kMcSharedTag = 0x80000000u;
 
-template <int SIGN, bool FP64>
+template <int SIGN, bool FP64, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_mc(
     const double4* __restrict__ pos, const double* __restrict__ ph,
     uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
@@ -481,10 +486,11 @@ __global__ __launch_bounds__(kBlock) void k_mc(
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
       if constexpr (FP64) {
-        const PairF64 q = pair_f64<SIGN>(p[h].w, phv[h], p[h].x, p[h].y, p[h].z, vx, vy, vz, Kw);
+        const PairF64 q =
+            pair_f64<SIGN, 0, OBL>(p[h].w, phv[h], p[h].x, p[h].y, p[h].z, vx, vy, vz, Kw);
         accumulate_phasor<0>(q.r, q.arg, ar, ai);
       } else {
-        mc_pair_f32s<SIGN>(p[h], vx, vy, vz, inv_lambda, fr, fi);
+        mc_pair_f32s<SIGN, OBL>(p[h], vx, vy, vz, inv_lambda, fr, fi);
       }
     }
     if constexpr (!FP64) {
@@ -502,10 +508,10 @@ __global__ __launch_bounds__(kBlock) void k_mc(
       if (h == 1 && k + 1 >= K) break;
       const double4 p = pos[idx[h]];
       if constexpr (FP64) {
-        const PairF64 q = pair_f64<SIGN>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
+        const PairF64 q = pair_f64<SIGN, 0, OBL>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
         accumulate_phasor<0>(q.r, q.arg, ar, ai);
       } else {
-        mc_pair_f32s<SIGN>(p, vx, vy, vz, inv_lambda, fr, fi);
+        mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
       }
     }
   }
@@ -516,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
 
 // Batched Monte Carlo: one workgroup = one group of kMcGroup targets
 // (kBlock lanes x 4 targets each) sharing the subset n_{g,k}.
-template <int SIGN, bool FP64>
+template <int SIGN, bool FP64, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_mc_shared(
     const double4* __restrict__ pos, const double* __restrict__ ph,
     uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
@@ -547,12 +553,14 @@ __global__ __launch_bounds__(kBlock) void k_mc_shared(
         const double phv = ph[idx[h]];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const PairF64 q = pair_f64<SIGN>(p.w, phv, p.x, p.y, p.z, vx[r], vy[r], vz[r], Kw);
+          const PairF64 q =
+              pair_f64<SIGN, 0, OBL>(p.w, phv, p.x, p.y, p.z, vx[r], vy[r], vz[r], Kw);
           accumulate_phasor<0>(q.r, q.arg, ar[r], ai[r]);
         }
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) mc_pair_f32s<SIGN>(p, vx[r], vy[r], vz[r], inv_lambda, fr[r], fi[r]);
+        for (int r = 0; r < R; ++r)
+          mc_pair_f32s<SIGN, OBL>(p, vx[r], vy[r], vz[r], inv_lambda, fr[r], fi[r]);
       }
     }
     if constexpr (!FP64) {
@@ -573,7 +581,7 @@ __global__ __launch_bounds__(kBlock) void k_mc_shared(
   }
 }
 
-hipError_t launch_mc(int sign, bool fp64, bool shared, const double4* pos,
+hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
                      double2* y, hipStream_t stream) {
@@ -581,25 +589,20 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, const double4* pos,
   const double scale = (double)N / (double)K;
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
   const double il = 1.0 / lambda;
-  if (shared) {
-    const unsigned blocks = (unsigned)((M + kMcGroup - 1) / kMcGroup);
-    if (sign > 0) {
-      if (fp64) k_mc_shared<+1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-      else k_mc_shared<+1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-    } else {
-      if (fp64) k_mc_shared<-1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-      else k_mc_shared<-1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-    }
-    return hipGetLastError();
-  }
-  const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
-  if (sign > 0) {
-    if (fp64) k_mc<+1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-    else k_mc<+1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-  } else {
-    if (fp64) k_mc<-1, true><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-    else k_mc<-1, false><<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-  }
+  const unsigned blocks = shared ? (unsigned)((M + kMcGroup - 1) / kMcGroup)
+                                 : (unsigned)((M + kBlock - 1) / kBlock);
+  with_sign(sign, [&](auto sg) {
+    with_bool(fp64, [&](auto f64) {
+      with_bool(obliquity, [&](auto obl) {
+        if (shared)
+          k_mc_shared<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
+              pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+        else
+          k_mc<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
+              pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+      });
+    });
+  });
   return hipGetLastError();
 }
 

@@ -10,6 +10,9 @@ Forward runs ``holo_plan_transform`` and backward the hand-written adjoint
 phasors recomputed on the fly), both through the C ABI with device pointers
 on torch's current stream.  z is returned as a real (M, 2) tensor [re, im];
 its gradient (dL/dRe z, dL/dIm z) is exactly what the adjoint consumes.
+With ``obliquity=True`` every term also carries |dz_nm| / d_nm (the MATLAB
+light-field model's cosine factor); the adjoint keeps it, since it is
+symmetric in the pair.
 """
 from __future__ import annotations
 
@@ -21,7 +24,8 @@ from . import FP32_SPLIT, Context, Plan
 class _SuperposeFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, amp, theta, prop):
-        z = prop.plan_u.transform(amp, theta, prop.v, prop.sign, prop.lam, prop.precision)
+        z = prop.plan_u.transform(amp, theta, prop.v, prop.sign, prop.lam, prop.precision,
+                                  obliquity=prop.obliquity)
         ctx.prop = prop
         ctx.save_for_backward(amp, theta)
         return torch.view_as_real(z)
@@ -31,7 +35,7 @@ class _SuperposeFn(torch.autograd.Function):
         amp, theta = ctx.saved_tensors
         prop = ctx.prop
         ga, gt = prop.plan_v.adjoint(gz.contiguous(), amp, theta, prop.u, prop.sign, prop.lam,
-                                     prop.precision)
+                                     prop.precision, obliquity=prop.obliquity)
         return ga, gt, None
 
 
@@ -40,12 +44,13 @@ class Propagator:
 
     def __init__(self, u: torch.Tensor, v: torch.Tensor, sign: int = +1,
                  lam: float = 632.8e-9, precision: int = FP32_SPLIT, tile: int = 64,
-                 ctx: Context = None):
+                 ctx: Context = None, obliquity: bool = False):
         if not (u.is_cuda and v.is_cuda):
             raise ValueError("Propagator needs GPU tensors (no CPU fallback)")
         self.u = u.to(torch.float64).contiguous()
         self.v = v.to(torch.float64).contiguous()
         self.sign, self.lam, self.precision = sign, lam, precision
+        self.obliquity = obliquity
         self.ctx = ctx or Context(u.device.index or 0)
         self.plan_u = Plan(self.ctx, self.u, tile)   # forward sources
         self.plan_v = Plan(self.ctx, self.v, tile)   # adjoint sources

@@ -22,6 +22,7 @@
  *    context's stream without synchronising.
  *  - Errors are returned as holo_status; the thread's last message is
  *    available from holo_last_error().  Nothing calls exit() or abort().
+ *    Flag bits that a call does not accept are rejected with HOLO_EINVAL.
  *  - N and M must be below 2^31.  A source and a target at the same point
  *    (d = 0) yield a non-finite term, as in the reference.
  *  - One context per host thread; calls on one context are not reentrant.
@@ -56,6 +57,12 @@ typedef enum holo_precision {
 
 #define HOLO_DEVICE_POINTERS 0x1u /* all array arguments are device pointers */
 #define HOLO_MC_SHARED 0x2u       /* Monte Carlo: groups of 1024 targets share a subset */
+/* Obliquity-weighted pairs: every pair's amplitude is also multiplied by
+ * cos = |v_z - u_z| / d, the `cosfact = Gz ./ Gdistances` of the MATLAB
+ * light-field model (matlab/holofunctions.m:226-227, which assumes one plane
+ * separation Gz for all pairs).  Accepted by holo_transform_full,
+ * holo_transform_mc, holo_plan_transform(_complex) and holo_plan_adjoint. */
+#define HOLO_OBLIQUITY 0x4u
 
 typedef struct holo_ctx holo_ctx;
 typedef struct holo_plan holo_plan;

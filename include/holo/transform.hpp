@@ -125,6 +125,9 @@ inline int sign_of(Direction d) { return d == Direction::Forwards ? HOLO_FORWARD
 struct Settings {
   double lambda = HOLO_LAMBDA;
   holo_precision precision = HOLO_FP64;
+  // Cosine (obliquity) factor |dz| / d on every pair, as in the MATLAB
+  // light-field model (matlab/holofunctions.m:226-227); the CUDA path has none.
+  bool obliquity = false;
 };
 inline Settings& settings() {
   static Settings s;
@@ -148,7 +151,8 @@ inline std::vector<Wave> superpose(Direction dir, const std::vector<Polar>& x,
   std::copy(v.begin(), v.begin() + std::min(v.size(), vv.size()), vv.begin());
   std::vector<Wave> y(M_out);
   check(holo_transform_full(context(), precision, sign_of(dir), lambda, amp.data(), phase.data(),
-                            u.data(), N, vv.data(), M_out, reinterpret_cast<double*>(y.data()), 0),
+                            u.data(), N, vv.data(), M_out, reinterpret_cast<double*>(y.data()),
+                            settings().obliquity ? HOLO_OBLIQUITY : 0u),
         "holo_transform_full");
   return y;
 }

@@ -45,15 +45,15 @@ def lib() -> ctypes.CDLL:
         L.oracle_wavenumber.argtypes = [ctypes.c_double]
         L.oracle_pair.restype = None
         L.oracle_pair.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                  ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+                                  ctypes.c_double, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
         L.oracle_transform_full.restype = None
         L.oracle_transform_full.argtypes = [
             ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_size_t,
-            _dp, ctypes.c_size_t, _i64p, ctypes.c_size_t, _dp, ctypes.c_int]
+            _dp, ctypes.c_size_t, _i64p, ctypes.c_size_t, _dp, ctypes.c_int, ctypes.c_int]
         L.oracle_transform_cartesian.restype = None
         L.oracle_transform_cartesian.argtypes = [
             ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_size_t,
-            _dp, ctypes.c_size_t, _dp, ctypes.c_int]
+            _dp, ctypes.c_size_t, _dp, ctypes.c_int, ctypes.c_int]
         L.oracle_normalize_polar.restype = ctypes.c_double
         L.oracle_normalize_polar.argtypes = [_dp, ctypes.c_size_t, _dp, _dp]
         L.oracle_philox4x32_10.restype = None
@@ -66,11 +66,11 @@ def lib() -> ctypes.CDLL:
         L.oracle_transform_mc.argtypes = [
             ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_size_t,
             _dp, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int,
-            _i64p, ctypes.c_size_t, _dp, ctypes.c_int]
+            _i64p, ctypes.c_size_t, _dp, ctypes.c_int, ctypes.c_int]
         L.oracle_adjoint.restype = None
         L.oracle_adjoint.argtypes = [
             ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, ctypes.c_size_t,
-            _dp, _dp, ctypes.c_size_t, _dp, _dp, ctypes.c_int]
+            _dp, _dp, ctypes.c_size_t, _dp, _dp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -87,16 +87,16 @@ def wavenumber(lam: float) -> float:
     return lib().oracle_wavenumber(lam)
 
 
-def pair(sign: int, lam: float, a: float, phi: float, u, v):
+def pair(sign: int, lam: float, a: float, phi: float, u, v, obliquity: bool = False):
     """One pair -> (re, im, d, arg) with the pinned operation order."""
     u = _f64(u); v = _f64(v)
     p = [ctypes.pointer(ctypes.c_double(0)) for _ in range(4)]
-    lib().oracle_pair(sign, wavenumber(lam), a, phi, _d(u), _d(v), *p)
+    lib().oracle_pair(sign, wavenumber(lam), a, phi, _d(u), _d(v), int(obliquity), *p)
     return tuple(x.contents.value for x in p)
 
 
 def transform_full(sign: int, lam: float, amp, phase, u, v, targets=None,
-                   nthreads: int = 0) -> np.ndarray:
+                   nthreads: int = 0, obliquity: bool = False) -> np.ndarray:
     """Brute-force sum; returns complex128 (len(targets) or M)."""
     amp = _f64(amp); phase = _f64(phase); u = _f64(u).reshape(-1); v = _f64(v).reshape(-1)
     N, M = amp.size, v.size // 3
@@ -109,16 +109,18 @@ def transform_full(sign: int, lam: float, amp, phase, u, v, targets=None,
         cnt, tp = M, None
     y = np.zeros(2 * cnt, dtype=np.float64)
     lib().oracle_transform_full(sign, lam, _d(amp), _d(phase), _d(u), N, _d(v), M,
-                                tp, cnt, _d(y), nthreads)
+                                tp, cnt, _d(y), nthreads, int(obliquity))
     return y.view(np.complex128)
 
 
-def transform_cartesian(sign: int, lam: float, w, u, v, nthreads: int = 0) -> np.ndarray:
+def transform_cartesian(sign: int, lam: float, w, u, v, nthreads: int = 0,
+                        obliquity: bool = False) -> np.ndarray:
     w = np.ascontiguousarray(w, dtype=np.complex128).view(np.float64)
     u = _f64(u).reshape(-1); v = _f64(v).reshape(-1)
     N, M = u.size // 3, v.size // 3
     y = np.zeros(2 * M, dtype=np.float64)
-    lib().oracle_transform_cartesian(sign, lam, _d(w), _d(u), N, _d(v), M, _d(y), nthreads)
+    lib().oracle_transform_cartesian(sign, lam, _d(w), _d(u), N, _d(v), M, _d(y), nthreads,
+                                     int(obliquity))
     return y.view(np.complex128)
 
 
@@ -147,7 +149,8 @@ def mc_shared_index(seed: int, g: int, k: int, N: int) -> int:
 
 
 def transform_mc(sign: int, lam: float, amp, phase, u, v, K: int, seed: int,
-                 targets=None, nthreads: int = 0, shared: bool = False) -> np.ndarray:
+                 targets=None, nthreads: int = 0, shared: bool = False,
+                 obliquity: bool = False) -> np.ndarray:
     amp = _f64(amp); phase = _f64(phase); u = _f64(u).reshape(-1); v = _f64(v).reshape(-1)
     N, M = amp.size, v.size // 3
     if targets is not None:
@@ -157,16 +160,18 @@ def transform_mc(sign: int, lam: float, amp, phase, u, v, K: int, seed: int,
         cnt, tp = M, None
     y = np.zeros(2 * cnt)
     lib().oracle_transform_mc(sign, lam, _d(amp), _d(phase), _d(u), N, _d(v), M,
-                              K, seed, 1 if shared else 0, tp, cnt, _d(y), nthreads)
+                              K, seed, 1 if shared else 0, tp, cnt, _d(y), nthreads,
+                              int(obliquity))
     return y.view(np.complex128)
 
 
-def adjoint(sign: int, lam: float, amp, theta, u, g, v, nthreads: int = 0):
+def adjoint(sign: int, lam: float, amp, theta, u, g, v, nthreads: int = 0,
+            obliquity: bool = False):
     """(dL/dA, dL/dtheta) for z = sum_n A e^{i theta} G; g = dL/dRe z + i dL/dIm z."""
     amp = _f64(amp); theta = _f64(theta); u = _f64(u).reshape(-1); v = _f64(v).reshape(-1)
     g = np.ascontiguousarray(g, dtype=np.complex128).view(np.float64)
     N, M = amp.size, v.size // 3
     ga = np.empty(N); gt = np.empty(N)
     lib().oracle_adjoint(sign, lam, _d(amp), _d(theta), _d(u), N, _d(g), _d(v), M,
-                         _d(ga), _d(gt), nthreads)
+                         _d(ga), _d(gt), nthreads, int(obliquity))
     return ga, gt

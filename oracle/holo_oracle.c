@@ -54,10 +54,16 @@ double oracle_wavenumber(double lambda) { return ORACLE_TWO_PI / lambda; }
 
 /* One pair: superposition.cu:35-51.  sign = +1 Forwards, -1 Backwards.
  * Also returns d and arg so that tests can check them bit-for-bit.
+ * obliquity != 0 multiplies the amplitude by the obliquity factor
+ * cos = |v2 - u2| / d of matlab/holofunctions.m:226-227 (`cosfact = Gz ./
+ * Gdistances`, which assumes one plane separation Gz; here the per-pair
+ * axial distance): r = (a / d) * (|dz| / d).
  * Pinned by: py/test_util.py:63-64 known answer, the exact-rational pure
- * Python restatement in tests/test_oracle.py, mpmath closed forms. */
+ * Python restatement in tests/test_oracle.py, mpmath closed forms; the
+ * obliquity factor by the closed form a |z| / d^2 and a NumPy restatement of
+ * lightfieldcalc. */
 void oracle_pair(int sign, double K, double a, double phi,
-                 const double *u, const double *v,
+                 const double *u, const double *v, int obliquity,
                  double *re, double *im, double *d_out, double *arg_out)
 {
     const double dx = v[0] - u[0], dy = v[1] - u[1], dz = v[2] - u[2];
@@ -65,7 +71,8 @@ void oracle_pair(int sign, double K, double a, double phi,
     const double d = sqrt(s);
     const double sd = sign > 0 ? d : -d;
     const double arg = fma(sd, K, phi);
-    const double r = a / d;
+    double r = a / d;
+    if (obliquity) r *= fabs(dz) / d;
     *re = r * cos(arg);
     *im = r * sin(arg);
     if (d_out) *d_out = d;
@@ -77,14 +84,19 @@ void oracle_pair(int sign, double K, double a, double phi,
  * (conj of the forward kernel when sign is flipped); no reference
  * counterpart.  Pinned by the dense-matrix identity in tests. */
 void oracle_pair_cartesian(int sign, double K, double w_re, double w_im,
-                           const double *u, const double *v,
+                           const double *u, const double *v, int obliquity,
                            double *re, double *im)
 {
     const double dx = v[0] - u[0], dy = v[1] - u[1], dz = v[2] - u[2];
     const double s = fma(dz, dz, fma(dy, dy, dx * dx));
     const double d = sqrt(s);
     const double arg = (sign > 0 ? d : -d) * K;
-    const double c = cos(arg) / d, sn = sin(arg) / d;
+    double c = cos(arg) / d, sn = sin(arg) / d;
+    if (obliquity) {
+        const double cf = fabs(dz) / d;
+        c *= cf;
+        sn *= cf;
+    }
     *re = w_re * c - w_im * sn;
     *im = w_re * sn + w_im * c;
 }
@@ -93,13 +105,14 @@ void oracle_pair_cartesian(int sign, double K, double w_re, double w_im,
  * y[m] = sum_n pair(n, m), summed in source order.  u, v are AoS xyz
  * (macros.h:48-56), y is interleaved (re, im) like std::vector<WAVE>.
  * Targets listed in `targets` (length M_sel) are evaluated; pass NULL for
- * all M.  nthreads <= 0 uses the OpenMP default. */
+ * all M.  nthreads <= 0 uses the OpenMP default.  obliquity: see
+ * oracle_pair. */
 void oracle_transform_full(int sign, double lambda,
                            const double *amp, const double *phase,
                            const double *u, size_t N,
                            const double *v, size_t M,
                            const int64_t *targets, size_t M_sel,
-                           double *y, int nthreads)
+                           double *y, int nthreads, int obliquity)
 {
     const double K = oracle_wavenumber(lambda);
     const size_t count = targets ? M_sel : M;
@@ -113,7 +126,7 @@ void oracle_transform_full(int sign, double lambda,
         for (size_t n = 0; n < N; ++n) {
             double re, im;
             oracle_pair(sign, K, amp[n], phase[n], &u[3 * n], &v[3 * m],
-                        &re, &im, NULL, NULL);
+                        obliquity, &re, &im, NULL, NULL);
             sr += re;
             si += im;
         }
@@ -130,7 +143,7 @@ void oracle_transform_cartesian(int sign, double lambda,
                                 const double *w /* interleaved, N */,
                                 const double *u, size_t N,
                                 const double *v, size_t M,
-                                double *y, int nthreads)
+                                double *y, int nthreads, int obliquity)
 {
     const double K = oracle_wavenumber(lambda);
 #ifdef _OPENMP
@@ -142,7 +155,7 @@ void oracle_transform_cartesian(int sign, double lambda,
         for (size_t n = 0; n < N; ++n) {
             double re, im;
             oracle_pair_cartesian(sign, K, w[2 * n], w[2 * n + 1],
-                                  &u[3 * n], &v[3 * m], &re, &im);
+                                  &u[3 * n], &v[3 * m], obliquity, &re, &im);
             sr += re;
             si += im;
         }
@@ -238,7 +251,7 @@ void oracle_transform_mc(int sign, double lambda,
                          const double *v, size_t M,
                          uint32_t K_samples, uint64_t seed, int shared,
                          const int64_t *targets, size_t M_sel,
-                         double *y, int nthreads)
+                         double *y, int nthreads, int obliquity)
 {
     const double K = oracle_wavenumber(lambda);
     const size_t count = targets ? M_sel : M;
@@ -255,7 +268,7 @@ void oracle_transform_mc(int sign, double lambda,
                                       : oracle_mc_index(seed, m, k, N);
             double re, im;
             oracle_pair(sign, K, amp[n], phase[n], &u[3 * n], &v[3 * m],
-                        &re, &im, NULL, NULL);
+                        obliquity, &re, &im, NULL, NULL);
             sr += re;
             si += im;
         }
@@ -271,17 +284,20 @@ void oracle_transform_mc(int sign, double lambda,
  *   dL/dA_n    = Re(e^{-i theta_n} b_n)
  *   dL/dtheta_n = A_n Im(e^{-i theta_n} b_n)
  * No reference counterpart.  Pinned by central finite differences and a
- * dense complex-matrix autograd product in tests. */
+ * dense complex-matrix autograd product in tests.  With obliquity, G carries
+ * the factor |dz| / d, which is symmetric in (m, n), so b keeps the same
+ * form. */
 void oracle_adjoint(int sign, double lambda,
                     const double *amp, const double *theta,
                     const double *u, size_t N,
                     const double *g /* interleaved, M */,
                     const double *v, size_t M,
-                    double *grad_amp, double *grad_theta, int nthreads)
+                    double *grad_amp, double *grad_theta, int nthreads,
+                    int obliquity)
 {
     double *b = (double *)malloc(sizeof(double) * 2 * N);
     /* b_n: sources are the targets v carrying weights g, targets are u. */
-    oracle_transform_cartesian(-sign, lambda, g, v, M, u, N, b, nthreads);
+    oracle_transform_cartesian(-sign, lambda, g, v, M, u, N, b, nthreads, obliquity);
     for (size_t n = 0; n < N; ++n) {
         const double c = cos(theta[n]), s = sin(theta[n]);
         /* e^{-i theta} b = (c br + s bi) + i (c bi - s br) */

@@ -59,6 +59,13 @@ def test_missing_and_ragged_inputs(tmp_path):
     assert r.returncode == 1 and "source counts" in r.stderr
 
 
+def test_invalid_obliquity_env_exits_nonzero():
+    r = run(["-h"], env={"HOLO_OBLIQUITY": "2"})
+    assert r.returncode == 0  # -h prints usage before the environment is read
+    r = run(["-x", "1"], env={"HOLO_OBLIQUITY": "yes"})
+    assert r.returncode == 1 and "HOLO_OBLIQUITY" in r.stderr
+
+
 def stage_golden(d):
     for src, dst in (("x_amp", "x0_amp"), ("x_phase", "x0_phase"), ("u", "u0"), ("v", "v0")):
         shutil.copy(os.path.join(GOLDEN, f"plain_small_{src}.dat"), os.path.join(d, f"{dst}.dat"))
@@ -159,3 +166,23 @@ def test_reference_wave_mode(tmp_path, oracle):
     ref = ref / np.abs(ref).max()
     assert np.max(np.abs(ya - np.abs(y + ref))) < 1e-12
     assert np.max(np.abs(np.angle(np.exp(1j * (yp - np.angle(ref)))))) < 1e-9
+
+
+@pytest.mark.gpu
+def test_file_mode_obliquity(tmp_path, oracle):
+    """HOLO_OBLIQUITY=1: every pair also carries |dz| / d (the MATLAB model's
+    cosine factor, matlab/holofunctions.m:226-227)."""
+    stage_golden(tmp_path)
+    v = np.fromfile(tmp_path / "v0.dat", "<f8")
+    M = v.size // 3
+    r = run(["-f", str(tmp_path), "-y", str(M)], env={"HOLO_OBLIQUITY": "1"})
+    assert r.returncode == 0, r.stderr
+    ya = np.fromfile(tmp_path / "y0_amp.dat", "<f8")
+    yp = np.fromfile(tmp_path / "y0_phase.dat", "<f8")
+    amp = np.fromfile(tmp_path / "x0_amp.dat", "<f8")
+    phase = np.fromfile(tmp_path / "x0_phase.dat", "<f8")
+    u = np.fromfile(tmp_path / "u0.dat", "<f8")
+    ra, rp, _ = oracle.normalize_polar(oracle.transform_full(-1, LAM_REF, amp, phase, u, v,
+                                                             obliquity=True))
+    assert np.max(np.abs(ya - ra)) < 1e-12
+    assert np.linalg.norm(ya * np.exp(1j * yp) - ra * np.exp(1j * rp)) / np.linalg.norm(ra) < 1e-12

@@ -392,6 +392,98 @@ def test_adjoint_dense_matrix(oracle):
 
 
 # --------------------------------------------------------------------------
+# obliquity factor (matlab/holofunctions.m:209-233, lightfieldcalc)
+
+
+def lightfieldcalc(obs_x, obs_y, holo_angles, holo_field, scr_x, scr_y, z, lam):
+    """NumPy restatement of MATLAB lightfieldcalc (holofunctions.m:209-233)
+    in double precision: for every screen point k, d = sqrt(dx^2 + dy^2 + z^2),
+    cosfact = z / d, sample += cosfact * (field_k / d) * exp(i (2 pi d / lam
+    + angle_k)).  Observers at height z above the screen (z > 0)."""
+    out = np.zeros(obs_x.size, dtype=np.complex128)
+    for k in range(scr_x.size):
+        d = np.sqrt((obs_x - scr_x[k]) ** 2 + (obs_y - scr_y[k]) ** 2 + z ** 2)
+        out += (z / d) * (holo_field[k] / d) * np.exp(1j * (2 * np.pi * d / lam + holo_angles[k]))
+    return out
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_obliquity_single_source_closed_form(oracle, sign):
+    """One source: |y| = a |dz| / d^2 (mpmath), and the phase is that of the
+    unweighted pair (the cosine is real and positive); below the source
+    plane (dz < 0) the factor stays positive."""
+    for v in ([1e-4, -2e-4, 0.05], [3e-4, 1e-4, -0.02], [0.0, 0.0, LAM_REF]):
+        re, im, d, arg = oracle.pair(sign, LAM, 0.7, 1.3, [0, 0, 0], v, obliquity=True)
+        r0, i0, d0, a0 = oracle.pair(sign, LAM, 0.7, 1.3, [0, 0, 0], v)
+        assert d == d0 and arg == a0
+        dm = mp.sqrt(mp.mpf(v[0]) ** 2 + mp.mpf(v[1]) ** 2 + mp.mpf(v[2]) ** 2)
+        expect = float(mp.mpf("0.7") * abs(mp.mpf(v[2])) / dm ** 2)
+        assert math.hypot(re, im) == pytest.approx(expect, rel=4e-16)
+        ratio = complex(re, im) / complex(r0, i0)
+        assert abs(ratio.imag) <= 1e-15 * abs(ratio) and ratio.real > 0
+        assert ratio.real == pytest.approx(float(abs(mp.mpf(v[2])) / dm), rel=4e-16)
+
+
+def test_obliquity_matches_matlab_lightfieldcalc(oracle):
+    """Forwards sum with obliquity == lightfieldcalc (MATLAB sign: + 2 pi d/lam).
+    The two formulations round the ~5e5 rad argument differently (2 pi d / lam
+    vs d K), about 1e-10 rad, so the bar is 1e-9; a factor of d, a missing
+    |.|, or the wrong sign is off by orders of magnitude."""
+    g = np.random.default_rng(31)
+    n_scr, n_obs, z = 300, 64, 0.05
+    sx, sy = (g.random(n_scr) - 0.5) * 2e-3, (g.random(n_scr) - 0.5) * 2e-3
+    ox, oy = (g.random(n_obs) - 0.5) * 2e-3, (g.random(n_obs) - 0.5) * 2e-3
+    ang, fld = g.random(n_scr) * 2 * np.pi, g.random(n_scr)
+    ref = lightfieldcalc(ox, oy, ang, fld, sx, sy, z, LAM)
+    u = np.stack([sx, sy, np.zeros(n_scr)], 1)
+    v = np.stack([ox, oy, np.full(n_obs, z)], 1)
+    y = oracle.transform_full(+1, LAM, fld, ang, u, v, obliquity=True)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-9
+    # observers below the screen: |dz| keeps the factor positive
+    v[:, 2] = -z
+    y = oracle.transform_full(+1, LAM, fld, ang, u, v, obliquity=True)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-9
+    # and without the factor the sums differ by about (1 - z/d) ~ 1e-3
+    y0 = oracle.transform_full(+1, LAM, fld, ang, u, np.abs(v))
+    assert np.linalg.norm(y0 - ref) / np.linalg.norm(ref) > 1e-5
+
+
+def test_obliquity_cartesian_and_mc_consistent(oracle):
+    """Complex weights w = a e^{i phi} give the polar sum (to the argument
+    rounding); the MC estimator with obliquity is (N/K) sum over its drawn
+    indices of the obliquity pair."""
+    amp, phase, u, v = rand_geom(32, 40, 5, z=2e-3, spread=5e-4)
+    yp = oracle.transform_full(-1, LAM, amp, phase, u, v, obliquity=True)
+    yc = oracle.transform_cartesian(-1, LAM, amp * np.exp(1j * phase), u, v, obliquity=True)
+    # fl(phi - d K) vs phi applied after fl(d K): ~4e-12 rad at d ~ 2 mm
+    assert np.linalg.norm(yc - yp) / np.linalg.norm(yp) < 1e-10
+    K, seed = 9, 5
+    y = oracle.transform_mc(+1, LAM, amp, phase, u, v, K, seed, obliquity=True)
+    for m in range(5):
+        idx = [oracle.mc_index(seed, m, k, 40) for k in range(K)]
+        ref = sum(oracle.transform_full(+1, LAM, amp[[n]], phase[[n]], u[[n]], v[[m]],
+                                        obliquity=True)[0] for n in idx)
+        assert abs(y[m] - 40 / K * ref) <= 1e-12 * abs(y[m])
+
+
+def test_obliquity_adjoint_dense_matrix(oracle):
+    """b = G^H g with G_mn = (|dz|/d) e^{i K d} / d built densely."""
+    g0 = np.random.default_rng(33)
+    amp, theta, u, v = rand_geom(34, 30, 20, z=0.01, spread=2e-3)
+    u[:, 2] = (g0.random(30) - 0.5) * 1e-3  # non-planar sources
+    g = g0.standard_normal(20) + 1j * g0.standard_normal(20)
+    diff = v[:, None, :] - u[None, :, :]
+    d = np.linalg.norm(diff, axis=2)
+    G = np.abs(diff[:, :, 2]) / d * np.exp(1j * 2 * np.pi / LAM * d) / d
+    ga, gt = oracle.adjoint(+1, LAM, amp, theta, u, g, v, obliquity=True)
+    e = np.exp(-1j * theta) * (G.conj().T @ g)
+    assert np.allclose(ga, e.real, rtol=1e-8, atol=1e-8 * np.abs(e).max())
+    assert np.allclose(gt, amp * e.imag, rtol=1e-8, atol=1e-8 * np.abs(e).max())
+    z = oracle.transform_full(+1, LAM, amp, theta, u, v, obliquity=True)
+    assert np.allclose(z, G @ (amp * np.exp(1j * theta)), rtol=1e-8, atol=1e-8 * np.abs(z).max())
+
+
+# --------------------------------------------------------------------------
 # the NumPy restatement of the reference's CPU path
 
 
