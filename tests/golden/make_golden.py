@@ -1,7 +1,18 @@
-"""Writes the golden vectors of the 32x32 -> 64x64 FP64 case (sign +1,
-lambda 632.8 nm) in the reference's .dat layout (raw little-endian float64,
-cuda/util.h:248-337): inputs x0 amp/phase, positions u/v, and the oracle's
-complex field y (interleaved re, im).  Run from the repository root:
+"""Writes the golden vectors in the reference's .dat layout (raw
+little-endian float64, cuda/util.h:248-337):
+
+* plain_small_*: the 32x32 -> 64x64 FP64 case (sign +1, lambda 632.8 nm):
+  inputs x0 amp/phase, positions u/v, and the oracle's complex field y
+  (interleaved re, im);
+* matlab_projector_*: a 32x32 version of the input set that
+  matlab/projector.m:4-75 writes for the holo CLI (one unit point source at
+  (0, 0, 0.5 m); a projector lattice at z = 0 with 7 um pitch that is NOT
+  centred, x_i = i * 7 um, y_j = j * 7 um, flat index i * 32 + j; the
+  projection lattice w at z = 0.5 m), plus the CLI's expected outputs at
+  lambda 0.65 um: y0 = normalised Backwards field of x on v, and z0_0 =
+  normalised Forwards field of x on v (cuda/main.cu:212-221), as amp/phase.
+
+Run from the repository root:
 
     python tests/golden/make_golden.py
 """
@@ -31,9 +42,31 @@ def main():
         "plain_small_v.dat": st.v.reshape(-1),
         "plain_small_y_oracle.dat": y.view(np.float64),
     }
+    out.update(matlab_projector())
     for name, arr in out.items():
         np.ascontiguousarray(arr, dtype="<f8").tofile(os.path.join(HERE, name))
     print("wrote", ", ".join(out))
+
+
+def matlab_projector(n: int = 32, pitch: float = 7e-6, z: float = 0.5, lam: float = 0.65e-6):
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v = np.stack([i.ravel() * pitch, j.ravel() * pitch, np.zeros(n * n)], 1)
+    w = v.copy()
+    w[:, 2] = z
+    amp, phase, u = np.array([1.0]), np.array([0.0]), np.array([0.0, 0.0, z])
+    ya, yp, _ = oracle.normalize_polar(oracle.transform_full(-1, lam, amp, phase, u, v))
+    za, zp, _ = oracle.normalize_polar(oracle.transform_full(+1, lam, amp, phase, u, v))
+    return {
+        "matlab_projector_x0_amp.dat": amp,
+        "matlab_projector_x0_phase.dat": phase,
+        "matlab_projector_u0.dat": u,
+        "matlab_projector_v0.dat": v.reshape(-1),
+        "matlab_projector_w0.dat": w.reshape(-1),
+        "matlab_projector_y0_amp.dat": ya,
+        "matlab_projector_y0_phase.dat": yp,
+        "matlab_projector_z0_0_amp.dat": za,
+        "matlab_projector_z0_0_phase.dat": zp,
+    }
 
 
 if __name__ == "__main__":

@@ -186,3 +186,21 @@ def test_file_mode_obliquity(tmp_path, oracle):
                                                              obliquity=True))
     assert np.max(np.abs(ya - ra)) < 1e-12
     assert np.linalg.norm(ya * np.exp(1j * yp) - ra * np.exp(1j * rp)) / np.linalg.norm(ra) < 1e-12
+
+
+@pytest.mark.gpu
+def test_matlab_projector_fixture(tmp_path):
+    """The input set of matlab/projector.m (32x32 version, tests/golden/
+    matlab_projector_*) through `holo -f DIR -F`: y0 and z0_0 equal the
+    committed expected outputs."""
+    for k in ("x0_amp", "x0_phase", "u0", "v0", "w0"):
+        shutil.copy(os.path.join(GOLDEN, f"matlab_projector_{k}.dat"), tmp_path / f"{k}.dat")
+    r = run(["-f", str(tmp_path), "-y", "1024", "-F"])
+    assert r.returncode == 0, r.stderr
+    for k in ("y0", "z0_0"):
+        a = np.fromfile(tmp_path / f"{k}_amp.dat", "<f8")
+        p = np.fromfile(tmp_path / f"{k}_phase.dat", "<f8")
+        ea = np.fromfile(os.path.join(GOLDEN, f"matlab_projector_{k}_amp.dat"), "<f8")
+        ep = np.fromfile(os.path.join(GOLDEN, f"matlab_projector_{k}_phase.dat"), "<f8")
+        assert np.max(np.abs(a - ea)) < 1e-12
+        assert np.linalg.norm(a * np.exp(1j * p) - ea * np.exp(1j * ep)) / np.linalg.norm(ea) < 1e-12
