@@ -239,11 +239,13 @@ def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
         chain_step(False)
     torch.cuda.synchronize(dev)
     log(f"[bench] timing {a.steps} steps")
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         chain_step(True)
         torch.cuda.synchronize(dev)
         launch_ms += [ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])]
+        step_ms.append(ev[0].elapsed_time(ev[3]))
     torch.cuda.synchronize(dev)
     dt = time.perf_counter() - t0
     ms_per_step = 1e3 * dt / a.steps
@@ -259,13 +261,30 @@ def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
                 "traffic": read_traffic("k_superpose_f64"),
                 "kernel": "k_superpose_f64 (+ partial-sum reduce)",
                 "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_launch,
-                "avg_launch_ms": 1e3 * avg_launch_s}
+                "avg_launch_ms": 1e3 * avg_launch_s,
+                "median_launch_ms": float(np.median(launch_ms)),
+                "std_launch_ms": float(np.std(launch_ms))}
     res = {"value": value, "unit": "pairs/s", "ms_per_step": ms_per_step,
            "config": {"workload": "FP64 brute-force chain x->y->z: 512^2 -> 1024^2 -> 512^2, "
                                   "lambda 632.8 nm, z 0 / 0.35 m, pitch 10 / 7.56 um",
                       "pairs_per_step": pairs_step, "precision": "FP64"},
            "gflops_29": value * FLOP_PER_PAIR / 1e9,
+           "step_ms_median": float(np.median(step_ms)), "step_ms_std": float(np.std(step_ms)),
            "roofline": roofline}
+    # End to end from host memory (the reference's time_transform includes
+    # its host staging): stage 1 through holo_transform_full with host
+    # pointers, i.e. H2D of 5 doubles per source and 3 per target, the
+    # launch, and D2H of the field.
+    log("[bench] stage 1 from host memory")
+    ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam,
+                       holo.FP64)  # first call sizes the staging buffers
+    t0 = time.perf_counter()
+    yh_host = ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam, holo.FP64)
+    th = time.perf_counter() - t0
+    res["host_pointer_stage1"] = {"pairs_per_s": s1.pairs / th, "ms": 1e3 * th,
+                                  "kernel_ms": 1e3 * avg_launch_s,
+                                  "rel_l2_vs_device_path": rel_l2(
+                                      yh_host, torch.view_as_complex(y).cpu().numpy())}
     if cpu is not None:
         cres, sub, ycpu = cpu
         # y holds the last chain step's stage-1 field (same inputs every step)

@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
     L.holo_plan_destroy.restype = None
     L.holo_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
                                  ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(d)]
+    L.holo_plan_lattice.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     L.holo_plan_transform.argtypes = [vp, vp, i, i, d, vp, vp, vp, sz, vp, ctypes.c_uint]
     L.holo_plan_transform_complex.argtypes = [vp, vp, i, i, d, vp, vp, sz, vp, ctypes.c_uint]
     L.holo_plan_adjoint.argtypes = [vp, vp, i, i, d, vp, vp, vp, vp, sz, vp, vp, ctypes.c_uint]
@@ -114,7 +115,7 @@ def lib() -> ctypes.CDLL:
     L.holo_host_sqrt_inv.restype = None
     for name in ("holo_create", "holo_set_stream", "holo_synchronize", "holo_set_option",
                  "holo_transform_full", "holo_transform_mc", "holo_mc_indices",
-                 "holo_normalize_polar", "holo_plan_create", "holo_plan_info",
+                 "holo_normalize_polar", "holo_plan_create", "holo_plan_info", "holo_plan_lattice",
                  "holo_plan_transform", "holo_plan_transform_complex", "holo_plan_adjoint",
                  "holo_read_dat", "holo_write_dat"):
         getattr(L, name).restype = i
@@ -345,6 +346,12 @@ class Plan:
                                    ctypes.byref(off)))
         return {"n_sources": n.value, "n_tiles": t.value, "tile": tile.value,
                 "max_offset": off.value}
+
+    def lattice(self):
+        """(nx, ny) of the product grid the plan detected, or (0, 0)."""
+        nx, ny = ctypes.c_size_t(), ctypes.c_size_t()
+        check(lib().holo_plan_lattice(self._h, ctypes.byref(nx), ctypes.byref(ny)))
+        return nx.value, ny.value
 
     def _out(self, like, M, device):
         if device:

@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
 ]
-LIB_SOURCES = ["holo_kernels.hip", "holo_capi.hip"]
+LIB_SOURCES = ["holo_kernels.hip", "holo_grid.hip", "holo_capi.hip"]
 
 
 def hipcc() -> str:

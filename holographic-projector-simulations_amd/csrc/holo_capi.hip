@@ -92,6 +92,9 @@ struct holo_plan {
   double max_offset = 0.0;
   bool planar = true;
   DevBuf centre, geo, perm, wc;
+  // lattice tiling (product-grid sources, 8 x 8 tiles) for the MFMA kernel
+  size_t grid_nx = 0, grid_ny = 0, grid_tiles = 0;
+  DevBuf g_centre, g_goff, g_perm, g_wc, g_wa, g_wa16, g_ts;
 };
 
 namespace {
@@ -341,6 +344,64 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
   if (!pending.empty()) emit_tile(u, pending.data(), pending.size(), T, out);
 }
 
+// Product-grid detection for the lattice kernel: positions n = i*ny + j with
+// x depending on the row i only, y on the column j only, and one z (the
+// reference's plane generators, cuda/init.h:89-158, x slow as in
+// cuda/macros.h:59).  nx and ny must be multiples of 8; the grid is cut into
+// 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).
+struct HostGrid {
+  size_t nx = 0, ny = 0;
+  std::vector<double4> centre;
+  std::vector<float4> goff;
+  std::vector<int32_t> perm;
+};
+
+bool build_grid(const double* u, size_t N, HostGrid& g) {
+  g = HostGrid();
+  if (N < 64) return false;
+  size_t ny = 1;
+  while (ny < N && u[3 * ny] == u[0]) ++ny;
+  if (ny == N || N % ny) return false;
+  const size_t nx = N / ny;
+  if (nx % 8 || ny % 8) return false;
+  const double z = u[2];
+  for (size_t n = 0; n < N; ++n) {
+    const size_t i = n / ny, j = n % ny;
+    if (u[3 * n] != u[3 * i * ny] || u[3 * n + 1] != u[3 * j + 1] || u[3 * n + 2] != z)
+      return false;
+  }
+  g.nx = nx;
+  g.ny = ny;
+  const size_t TI = nx / 8, TJ = ny / 8;
+  g.centre.reserve(TI * TJ);
+  g.goff.reserve(4 * TI * TJ);
+  g.perm.reserve(64 * TI * TJ);
+  for (size_t I = 0; I < TI; ++I)
+    for (size_t J = 0; J < TJ; ++J) {
+      double xs[8], ys[8];
+      for (int a = 0; a < 8; ++a) xs[a] = u[3 * (8 * I + a) * ny];
+      for (int b = 0; b < 8; ++b) ys[b] = u[3 * (8 * J + b) + 1];
+      const double cx = 0.5 * (*std::min_element(xs, xs + 8) + *std::max_element(xs, xs + 8));
+      const double cy = 0.5 * (*std::min_element(ys, ys + 8) + *std::max_element(ys, ys + 8));
+      float gx[8], gy[8];
+      double rx = 0.0, ry = 0.0;
+      for (int k = 0; k < 8; ++k) {
+        gx[k] = (float)(xs[k] - cx);
+        gy[k] = (float)(ys[k] - cy);
+        rx = std::max(rx, std::fabs(xs[k] - cx));
+        ry = std::max(ry, std::fabs(ys[k] - cy));
+      }
+      const float2 rho{std::nextafter((float)rx, INFINITY), std::nextafter((float)ry, INFINITY)};
+      g.centre.push_back(double4{cx, cy, z, __builtin_bit_cast(double, rho)});
+      for (int q = 0; q < 4; ++q)
+        g.goff.push_back(float4{gx[2 * q], gx[2 * q + 1], gy[2 * q], gy[2 * q + 1]});
+      for (int a = 0; a < 8; ++a)
+        for (int b = 0; b < 8; ++b)
+          g.perm.push_back((int32_t)((8 * I + a) * ny + 8 * J + b));
+    }
+  return true;
+}
+
 holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
   HostTiles t;
   build_tiles(u_host, p->N, p->tile, t);
@@ -360,6 +421,27 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
     HOLO_HIP(hipMemcpyAsync(p->geo.ptr, t.geo.data(), slots * sizeof(float4),
                             hipMemcpyHostToDevice, ctx->stream));
     HOLO_HIP(hipMemcpyAsync(p->perm.ptr, t.perm.data(), slots * sizeof(int32_t),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
+  HostGrid grid;
+  if (p->tile == 64 && build_grid(u_host, p->N, grid)) {
+    const size_t nt = grid.centre.size();
+    if (nt * 64 > kMaxCount) return fail(HOLO_EINVAL, "too many tile slots");
+    p->grid_nx = grid.nx;
+    p->grid_ny = grid.ny;
+    p->grid_tiles = nt;
+    HOLO_TRY(ensure(p->g_centre, nt * sizeof(double4), ctx->device));
+    HOLO_TRY(ensure(p->g_goff, 4 * nt * sizeof(float4), ctx->device));
+    HOLO_TRY(ensure(p->g_perm, 64 * nt * sizeof(int32_t), ctx->device));
+    HOLO_TRY(ensure(p->g_wc, 64 * nt * sizeof(float2), ctx->device));
+    HOLO_TRY(ensure(p->g_wa, 64 * nt * sizeof(float4), ctx->device));
+    HOLO_TRY(ensure(p->g_wa16, 64 * nt * sizeof(uint2), ctx->device));
+    HOLO_TRY(ensure(p->g_ts, nt * sizeof(float), ctx->device));
+    HOLO_HIP(hipMemcpyAsync(p->g_centre.ptr, grid.centre.data(), nt * sizeof(double4),
+                            hipMemcpyHostToDevice, ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(p->g_goff.ptr, grid.goff.data(), 4 * nt * sizeof(float4),
+                            hipMemcpyHostToDevice, ctx->stream));
+    HOLO_HIP(hipMemcpyAsync(p->g_perm.ptr, grid.perm.data(), 64 * nt * sizeof(int32_t),
                             hipMemcpyHostToDevice, ctx->stream));
   }
   // host vectors are freed on return: wait for the copies
@@ -409,6 +491,31 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
                                         p->N, v, M, K, reinterpret_cast<double2*>(y),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));
+  } else if (p->grid_tiles && holo::tuning().grid) {
+    const size_t nt = p->grid_tiles;
+    HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g_perm.ptr),
+                                       64 * nt, static_cast<float2*>(p->g_wc.ptr), ctx->stream));
+    HOLO_HIP(holo::launch_grid_weights(static_cast<const float2*>(p->g_wc.ptr), (uint32_t)nt,
+                                       static_cast<float4*>(p->g_wa.ptr),
+                                       static_cast<uint2*>(p->g_wa16.ptr),
+                                       static_cast<float*>(p->g_ts.ptr), ctx->stream));
+    holo::GridTiles gt;
+    gt.centre = static_cast<const double4*>(p->g_centre.ptr);
+    gt.goff = static_cast<const float4*>(p->g_goff.ptr);
+    gt.perm = static_cast<const int32_t*>(p->g_perm.ptr);
+    gt.u = static_cast<const double*>(p->u.ptr);
+    gt.n_tiles = (uint32_t)nt;
+    const int S = holo::grid_splits(M, nt);
+    const size_t pe = S > 1 ? (size_t)S * M : 0;
+    if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
+    HOLO_HIP(holo::launch_superpose_grid(sign, obliquity, gt,
+                                         static_cast<const float2*>(p->g_wc.ptr),
+                                         static_cast<const float4*>(p->g_wa.ptr),
+                                         static_cast<const uint2*>(p->g_wa16.ptr),
+                                         static_cast<const float*>(p->g_ts.ptr), v, M, lambda,
+                                         reinterpret_cast<double2*>(y),
+                                         static_cast<double2*>(ctx->partial.ptr), pe,
+                                         ctx->stream));
   } else {
     const size_t slots = p->n_tiles * p->tile;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->perm.ptr),
@@ -524,6 +631,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.target_waves = (int)value;
   } else if (n == "f64_libm") {
     t.f64_libm = value != 0;
+  } else if (n == "grid") {
+    t.grid = value != 0;
   } else {
     return fail(HOLO_EINVAL, "unknown option " + n);
   }
@@ -539,6 +648,7 @@ long holo_get_option(const char* name) {
   if (n == "splits") return t.splits;
   if (n == "target_waves") return t.target_waves;
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
+  if (n == "grid") return t.grid ? 1 : 0;
   return -1;
 }
 
@@ -678,6 +788,13 @@ holo_status holo_plan_create(holo_ctx* ctx, holo_plan** plan, const double* u, s
 }
 
 void holo_plan_destroy(holo_plan* plan) { delete plan; }
+
+holo_status holo_plan_lattice(const holo_plan* plan, size_t* nx, size_t* ny) {
+  if (!plan) return fail(HOLO_EINVAL, "null plan");
+  if (nx) *nx = plan->grid_nx;
+  if (ny) *ny = plan->grid_ny;
+  return HOLO_OK;
+}
 
 holo_status holo_plan_info(const holo_plan* plan, size_t* n_sources, size_t* n_tiles,
                            unsigned* tile, double* max_offset) {

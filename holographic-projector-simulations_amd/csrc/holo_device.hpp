@@ -130,4 +130,53 @@ __device__ __forceinline__ void sincos_rev_f32(float t, float& s, float& c) {
   c = __builtin_amdgcn_cosf(t);
 }
 
+// ---------------------------------------------------------------------------
+// FP32-split helpers shared by the tiled and lattice kernels.
+
+// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
+// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).  OBL:
+// amplitude |dz| / d^2 instead of 1 / d.
+template <int SIGN, bool OBL = false>
+__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
+                                             double inv_lambda, float& fr, float& fi) {
+  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+  double d, id;
+  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  const float f = (float)frac_cycle(d * inv_lambda);
+  float sn, cs;
+  sincos_rev_f32(f, sn, cs);
+  if (SIGN < 0) sn = -sn;
+  const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
+  const float2 w = __builtin_bit_cast(float2, p.w);
+  const float pr = idf * cs, pi = idf * sn;
+  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
+  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+}
+
+// Accuracy guard: per (target, tile) the bound q_max of |q| over the tile
+// (from the tile's lateral and axial radii) gives the FP32 rounding error of
+// t, ~eps32 q_max / (2 d lambda) cycles, and the truncation of the q-series,
+// q_max^3 / (16 d^5 lambda) cycles.  If either exceeds kGuardCycles (thick
+// 3-D tiles, near field) that target evaluates the tile exactly: FP64
+// distance from the source's FP64 position, FP64 fractional cycle, FP32
+// sin/cos and weights.
+constexpr float kGuardCycles = 2e-7f;
+
+template <int SIGN, bool OBL>
+__device__ __forceinline__ void exact_tile_pairs(
+    const double* __restrict__ u, const int32_t* __restrict__ perm,
+    const float2* __restrict__ w, int T, double vx, double vy, double vz,
+    double inv_lambda, double& ar, double& ai) {
+  float fr = 0.f, fi = 0.f;
+  for (int j = 0; j < T; ++j) {
+    const int32_t n = perm[j];
+    if (n < 0) continue;
+    const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
+                                   __builtin_bit_cast(double, w[j]));
+    mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
+  }
+  ar += fr;
+  ai += fi;
+}
+
 }  // namespace holo

@@ -1,0 +1,264 @@
+// Lattice kernel: FP32-split superposition for sources on a product grid
+// (every plane of the reference's generators, cuda/init.h:89-158), with the
+// sum over each 8 x 8 block of sources evaluated as a small complex GEMM on
+// the matrix cores.
+//
+// For target m and tile t (centre c, r = v_m - c, d_ref = |r|), the source at
+// offset (gx_a, gy_b, 0) has q = d^2 - d_ref^2 = P_a + Q_b with
+//   P_a = gx_a^2 - 2 r_x gx_a,    Q_b = gy_b^2 - 2 r_y gy_b.
+// The tiled FP32-split expansion (d - d_ref)/lambda = alpha q + c2 q^2 and
+// 1/d = (1/d_ref)(1 + kappa q) then factorises into a row factor, a column
+// factor and a small cross term:
+//   e^{i s 2 pi d/lambda} / d ~= e^{i s 2 pi f} X_a Y_b (1 + i s beta P_a Q_b)
+//   X_a = (1/d_ref)(1 + kappa P_a) e^{i s 2 pi (alpha P_a + c2 P_a^2)}
+//   Y_b = (1 + kappa Q_b) e^{i s 2 pi (alpha Q_b + c2 Q_b^2)},   beta = 4 pi c2
+// with f = frac(d_ref / lambda) from FP64.  Dropped: kappa^2 P Q in the
+// amplitude and (beta P Q)^2 / 2 in the phasor; the accuracy guard bounds
+// both, with the tiled kernel's rounding and truncation bounds, and sends a
+// (target, tile) that fails any of them through exact FP64-distance pairs.
+// A tile then contributes
+//   S = sum_a X_a (Z_a + i s beta P_a Z'_a),
+//   Z_a = sum_b W_ab Y_b,   Z'_a = sum_b W_ab Y_b Q_b,
+// and Z, Z' for 16 targets at once are 16 x 16 real products (the complex
+// 8 x 8 written out): Z on v_mfma_f32_16x16x4f32 in FP32, Z' -- below 1e-4 of
+// Z -- on v_mfma_f32_16x16x16f16.
+//
+// Work split (wave of 64 lanes, 64 targets): lane l = 16 g + j computes the
+// FP64 constants of target l for the tile and shares them through LDS; for
+// each block B of 16 targets it then supplies Y of columns 2g, 2g+1 of target
+// 16 B + j as the GEMM's B operand and receives Z of rows 2g, 2g+1, which it
+// multiplies by its X.  Operand layout (16x16x4: lane l holds A[l%16][l/16],
+// B[l/16][l%16], D[4(l/16)+r][l%16]; 16x16x16 holds four consecutive k):
+//   k = 4g + s  <->  column b = 2g + s/2, component s%2 (re, im)
+//   i = 2a + c  <->  row a, component c
+// so the weights enter as A[2a+c][2b+c'] = (Wr, -Wi; Wi, Wr) (k_grid_weights).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "holo_device.hpp"
+#include "holo_internal.hpp"
+
+namespace holo {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+
+constexpr int kGridWaves = kBlock / 64;
+constexpr int kGridFlush = 8;  // tiles summed in FP32 between FP64 flushes
+constexpr float kTwoPow20 = 1048576.f;
+constexpr float kFourPi = 12.566370614359172f;
+
+// MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
+// holds row i = 2a + c of W^T for the columns b = 2g, 2g+1.
+__global__ __launch_bounds__(kBlock) void k_grid_weights(
+    const float2* __restrict__ wc, uint32_t n_tiles, float4* __restrict__ wa,
+    uint2* __restrict__ wa16, float* __restrict__ tscale) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t t = blockIdx.x * kGridWaves + (threadIdx.x >> 6);
+  if (t >= n_tiles) return;  // uniform per wave
+  const uint32_t i = lane & 15, g = lane >> 4, a = i >> 1, co = i & 1;
+  const float2 w0 = wc[(size_t)t * 64 + 8 * a + 2 * g];
+  const float2 w1 = wc[(size_t)t * 64 + 8 * a + 2 * g + 1];
+  float4 A;
+  A.x = co ? w0.y : w0.x;   // (b0, re) -> (a, c)
+  A.y = co ? w0.x : -w0.y;  // (b0, im)
+  A.z = co ? w1.y : w1.x;   // (b1, re)
+  A.w = co ? w1.x : -w1.y;  // (b1, im)
+  float mx = fmaxf(fmaxf(fabsf(w0.x), fabsf(w0.y)), fmaxf(fabsf(w1.x), fabsf(w1.y)));
+  for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  // power-of-two scale s > max |w| of the tile: the f16 copy holds w / s
+  int e = 0;
+  if (mx > 0.f && __builtin_isfinite(mx)) frexpf(mx, &e);
+  e = e < -126 ? -126 : (e > 127 ? 127 : e);
+  const float s = ldexpf(1.f, e), is = ldexpf(1.f, -e);
+  const h4v h = {(_Float16)(A.x * is), (_Float16)(A.y * is), (_Float16)(A.z * is),
+                 (_Float16)(A.w * is)};
+  wa[(size_t)t * 64 + lane] = A;
+  wa16[(size_t)t * 64 + lane] = __builtin_bit_cast(uint2, h);
+  if (lane == 0) tscale[t] = s;
+}
+
+hipError_t launch_grid_weights(const float2* wc, uint32_t n_tiles, float4* wa, uint2* wa16,
+                               float* tscale, hipStream_t stream) {
+  if (!n_tiles) return hipSuccess;
+  const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
+  k_grid_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, wa, wa16, tscale);
+  return hipGetLastError();
+}
+
+template <int SIGN, bool OBL>
+__global__ __launch_bounds__(kBlock) void k_superpose_grid(
+    const double4* __restrict__ centre, const float4* __restrict__ goff,
+    const int32_t* __restrict__ perm, const double* __restrict__ u,
+    const float2* __restrict__ wc, const float4* __restrict__ wa,
+    const uint2* __restrict__ wa16, const float* __restrict__ tscale,
+    uint32_t n_tiles, uint32_t chunk, const double* __restrict__ v, uint32_t M,
+    double inv_lambda, double2* __restrict__ out) {
+  // per-(target, tile) constants: c0 = (-2 r_x, -2 r_y, alpha, c2),
+  // c1 = (kappa, f, amplitude (0: exact path), s beta)
+  __shared__ float4 s_cst[kGridWaves][64][2];
+  __shared__ double2 s_red[kGridWaves][64][4];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t g = lane >> 4, j = lane & 15;
+  const uint32_t base = blockIdx.x * kBlock + wv * 64;
+  const uint32_t t0 = blockIdx.y * chunk;
+  const uint32_t t1 = min(n_tiles, t0 + chunk);
+  const uint32_t mo = base + lane;  // the target whose constants this lane computes
+  const size_t mm = mo < M ? mo : M - 1;
+  const double vx = v[3 * mm], vy = v[3 * mm + 1], vz = v[3 * mm + 2];
+  double ex_r = 0.0, ex_i = 0.0;  // exact-path sum of target mo
+  double ar[4], ai[4];            // target 16 B + j, rows 2g, 2g+1
+  float fr[4], fi[4];
+#pragma unroll
+  for (int B = 0; B < 4; ++B) {
+    ar[B] = ai[B] = 0.0;
+    fr[B] = fi[B] = 0.f;
+  }
+  int since = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    const double4 c = centre[t];
+    const float2 rho = __builtin_bit_cast(float2, c.w);  // (rho_x, rho_y)
+    {
+      const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
+      double dref, id;
+      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      const double al = 0.5 * id * inv_lambda;
+      const double id2 = id * id;
+      const float c2 = (float)(-0.25 * al * id2);
+      float kappa, amp;
+      if constexpr (OBL) {  // |dz| / d^2 = (|r_z| / d_ref^2)(1 - q / d_ref^2 + ...)
+        kappa = (float)(-id2);
+        amp = (float)(__builtin_fabs(rz) * id2);
+      } else {
+        kappa = (float)(-0.5 * id2);
+        amp = (float)id;
+      }
+      const float Pm = 2.f * __builtin_fabsf((float)rx) * rho.x + rho.x * rho.x;
+      const float Qm = 2.f * __builtin_fabsf((float)ry) * rho.y + rho.y * rho.y;
+      const float qmax = Pm + Qm;
+      const float ia = (float)(id * inv_lambda), idf = (float)id;
+      const float round_err = 3e-8f * qmax * ia;
+      const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
+      const float cross = kFourPi * __builtin_fabsf(c2) * Pm * Qm;  // |beta| P Q, radians
+      const float ampsep = kappa * kappa * Pm * Qm;
+      // written so that NaN (a target at the tile centre) also counts as near
+      const bool near = !(round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles &&
+                          cross <= 1.5e-3f && ampsep <= 1e-6f);
+      const float f = (float)frac_cycle(dref * inv_lambda);
+      float4 c0 = make_float4((float)(-2.0 * rx), (float)(-2.0 * ry), (float)al, c2);
+      float4 c1 = make_float4(kappa, f, amp, (float)SIGN * kFourPi * c2);
+      if (near) {  // the GEMM part of this target gets exactly zero
+        c0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        c1 = c0;
+      }
+      s_cst[wv][lane][0] = c0;
+      s_cst[wv][lane][1] = c1;
+      if (near)  // rare: near field
+        exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 64, wc + (size_t)t * 64, 64, vx, vy,
+                                    vz, inv_lambda, ex_r, ex_i);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float4 go = goff[(size_t)t * 4 + g];
+    const float4 A = wa[(size_t)t * 64 + lane];
+    const h4v A16 = __builtin_bit_cast(h4v, wa16[(size_t)t * 64 + lane]);
+    const float ts = tscale[t] * (1.f / kTwoPow20);
+    const float gxs[2] = {go.x, go.y}, gys[2] = {go.z, go.w};
+#pragma unroll
+    for (int B = 0; B < 4; ++B) {
+      const float4 c0 = s_cst[wv][16 * B + j][0];
+      const float4 c1 = s_cst[wv][16 * B + j][1];
+      float yv[4];
+      h4v y16;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float gy = gys[h];
+        const float Q = __builtin_fmaf(c0.y, gy, gy * gy);
+        float sn, cs;
+        sincos_rev_f32(Q * __builtin_fmaf(c0.w, Q, c0.z), sn, cs);
+        const float am = __builtin_fmaf(c1.x, Q, 1.f);
+        const float yr = am * cs, yi = SIGN > 0 ? am * sn : -(am * sn);
+        yv[2 * h] = yr;
+        yv[2 * h + 1] = yi;
+        const float qs = Q * kTwoPow20;
+        y16[2 * h] = (_Float16)(yr * qs);
+        y16[2 * h + 1] = (_Float16)(yi * qs);
+      }
+      f4v D = {0.f, 0.f, 0.f, 0.f};
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, yv[0], D, 0, 0, 0);
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, yv[1], D, 0, 0, 0);
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, yv[2], D, 0, 0, 0);
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, yv[3], D, 0, 0, 0);
+      f4v Dq = {0.f, 0.f, 0.f, 0.f};
+      Dq = __builtin_amdgcn_mfma_f32_16x16x16f16(A16, y16, Dq, 0, 0, 0);
+      const float bs = c1.w * ts;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float gx = gxs[h];
+        const float P = __builtin_fmaf(c0.x, gx, gx * gx);
+        float sn, cs;
+        sincos_rev_f32(__builtin_fmaf(P, __builtin_fmaf(c0.w, P, c0.z), c1.y), sn, cs);
+        const float am = c1.z * __builtin_fmaf(c1.x, P, 1.f);
+        const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
+        const float bp = bs * P;
+        const float tr = __builtin_fmaf(-bp, Dq[2 * h + 1], D[2 * h]);
+        const float ti = __builtin_fmaf(bp, Dq[2 * h], D[2 * h + 1]);
+        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
+        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
+      }
+    }
+    if (++since == kGridFlush) {
+      since = 0;
+#pragma unroll
+      for (int B = 0; B < 4; ++B) {
+        ar[B] += fr[B];
+        ai[B] += fi[B];
+        fr[B] = fi[B] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int B = 0; B < 4; ++B) s_red[wv][16 * B + j][g] = make_double2(ar[B] + fr[B], ai[B] + fi[B]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double sr = ex_r, si = ex_i;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {  // fixed order: deterministic
+    const double2 p = s_red[wv][lane][q];
+    sr += p.x;
+    si += p.y;
+  }
+  if (mo < M) out[(size_t)blockIdx.y * M + mo] = make_double2(sr, si);
+}
+
+int grid_splits(size_t M, size_t n_tiles) {
+  return n_tiles ? choose_splits(M, 1, n_tiles, 16) : 1;
+}
+
+hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
+                                 const float2* wc, const float4* wa, const uint2* wa16,
+                                 const float* tscale, const double* v, size_t M,
+                                 double lambda, double2* y, double2* partial,
+                                 size_t partial_cap, hipStream_t stream) {
+  if (M == 0) return hipSuccess;
+  int S = grid_splits(M, gt.n_tiles);
+  while (S > 1 && (size_t)S * M > partial_cap) --S;
+  const uint32_t chunk = gt.n_tiles ? (gt.n_tiles + S - 1) / S : 0;
+  const dim3 grid((unsigned)((M + kBlock - 1) / kBlock), (unsigned)S);
+  double2* out = S > 1 ? partial : y;
+  const double il = 1.0 / lambda;
+  with_sign(sign, [&](auto sg) {
+    with_bool(obliquity, [&](auto obl) {
+      k_superpose_grid<sg.value, (bool)obl.value><<<grid, kBlock, 0, stream>>>(
+          gt.centre, gt.goff, gt.perm, gt.u, wc, wa, wa16, tscale, gt.n_tiles, chunk, v,
+          (uint32_t)M, il, out);
+    });
+  });
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_reduce_partials(S, M, partial, y, stream);
+}
+
+}  // namespace holo

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace holo {
 
 enum WeightForm { kPolar = 0, kCartesian = 1 };
@@ -24,15 +26,58 @@ struct TileSet {
   bool planar = false;  // every slot has delta_z == 0
 };
 
+// Source set of the lattice kernel: the sources form a product grid
+// x_i (i < nx) by y_j (j < ny) at one z, in the caller's order n = i*ny + j,
+// cut into 8 x 8 blocks.  Tile t = I*(ny/8) + J holds sources (8I+a, 8J+b);
+// slot 8a+b; centre[t] = (cx, cy, z, bit-cast float2 (rho_x, rho_y)) with
+// rho the largest |offset| along x and y; goff[4t+g] = (gx_2g, gx_2g+1,
+// gy_2g, gy_2g+1), the FP32 offsets of rows 2g, 2g+1 and columns 2g, 2g+1
+// from the centre; perm[64t + slot] = source index.
+struct GridTiles {
+  const double4* centre = nullptr;
+  const float4* goff = nullptr;
+  const int32_t* perm = nullptr;
+  const double* u = nullptr;
+  uint32_t n_tiles = 0;
+};
+
 struct Tuning {
   int r_f64 = 4;          // targets per lane, FP64 kernel (1, 2, 4)
   int r_f32 = 2;          // targets per lane, FP32-split kernel (1, 2, 4)
   int splits = 0;         // source splits (0 = automatic)
   int target_waves = 16384;
   bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
+  bool grid = true;       // FP32-split: lattice kernel for product-grid sources
 };
 
 Tuning& tuning();
+
+// Run-time -> compile-time dispatch: each helper calls f with a
+// std::integral_constant carrying the value.
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+inline void with_bool(bool b, F&& f) {
+  if (b) f(IC<1>{});
+  else f(IC<0>{});
+}
+
+template <typename F>
+inline void with_sign(int sign, F&& f) {
+  if (sign > 0) f(IC<+1>{});
+  else f(IC<-1>{});
+}
+
+template <typename F>
+inline void with_r(int R, F&& f) {
+  switch (R) {
+    case 1: f(IC<1>{}); break;
+    case 4: f(IC<4>{}); break;
+    default: f(IC<2>{}); break;
+  }
+}
+
 
 // Number of source splits S chosen for a launch over M targets with R
 // targets per lane and `units` source units (sources or tiles).
@@ -48,11 +93,29 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const d
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream);
 
+// y[m] = sum_s partial[s*M + m] in fixed order (nothing to do for S <= 1).
+hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, double2* y,
+                                  hipStream_t stream);
+
 // FP32-split: weights wc (float2 per slot), tiles as above.
 hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
                                  const float2* wc, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
                                  size_t partial_cap, hipStream_t stream);
+
+// Lattice kernel (FP32-split on product-grid tiles).  wc: per-slot FP32
+// weights (launch_tile_weights with the grid perm).  launch_grid_weights
+// lays them out as the MFMA operands: wa (float4 per tile and lane), wa16
+// (4 x f16 per tile and lane, scaled by 1 / tscale[t], a power of two).
+hipError_t launch_grid_weights(const float2* wc, uint32_t n_tiles, float4* wa, uint2* wa16,
+                               float* tscale, hipStream_t stream);
+hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
+                                 const float2* wc, const float4* wa, const uint2* wa16,
+                                 const float* tscale, const double* v, size_t M,
+                                 double lambda, double2* y, double2* partial,
+                                 size_t partial_cap, hipStream_t stream);
+// Source splits the lattice kernel uses for M targets over n_tiles tiles.
+int grid_splits(size_t M, size_t n_tiles);
 
 // wc[slot] = a e^{i phi} (polar) or w (cartesian) of source perm[slot].
 hipError_t launch_tile_weights(WeightForm wf, const double* w0,

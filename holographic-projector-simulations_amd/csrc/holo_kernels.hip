@@ -16,8 +16,6 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-#include <type_traits>
-
 #include "holo_device.hpp"
 #include "holo_internal.hpp"
 
@@ -37,32 +35,6 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units) {
   S = std::min(S, max_by_units);
   S = std::min<size_t>(S, 64);
   return (int)std::max<size_t>(S, 1);
-}
-
-// Run-time -> compile-time dispatch: each helper calls f with a
-// std::integral_constant carrying the value.
-template <int V>
-using IC = std::integral_constant<int, V>;
-
-template <typename F>
-static void with_bool(bool b, F&& f) {
-  if (b) f(IC<1>{});
-  else f(IC<0>{});
-}
-
-template <typename F>
-static void with_sign(int sign, F&& f) {
-  if (sign > 0) f(IC<+1>{});
-  else f(IC<-1>{});
-}
-
-template <typename F>
-static void with_r(int R, F&& f) {
-  switch (R) {
-    case 1: f(IC<1>{}); break;
-    case 4: f(IC<4>{}); break;
-    default: f(IC<2>{}); break;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -145,8 +117,8 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(
   y[m] = make_double2(re, im);
 }
 
-static hipError_t reduce_if_split(int S, size_t M, const double2* partial,
-                                  double2* y, hipStream_t stream) {
+hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, double2* y,
+                                  hipStream_t stream) {
   if (S <= 1) return hipSuccess;
   const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
   k_reduce_partials<<<blocks, kBlock, 0, stream>>>(partial, (uint32_t)S,
@@ -183,7 +155,7 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const d
   });
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return reduce_if_split(S, M, partial, y, stream);
+  return launch_reduce_partials(S, M, partial, y, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -203,52 +175,6 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const d
 // has delta_z = 0 (sources on planes z = const), so the z term of q drops.
 // Valid in the far field of each tile: |q| / d_ref^2 << 1 (tile radius much
 // smaller than the distance) and |t| of order one cycle.
-
-// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
-// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).  OBL:
-// amplitude |dz| / d^2 instead of 1 / d.
-template <int SIGN, bool OBL = false>
-__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
-                                             double inv_lambda, float& fr, float& fi) {
-  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
-  double d, id;
-  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
-  const float f = (float)frac_cycle(d * inv_lambda);
-  float sn, cs;
-  sincos_rev_f32(f, sn, cs);
-  if (SIGN < 0) sn = -sn;
-  const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
-  const float2 w = __builtin_bit_cast(float2, p.w);
-  const float pr = idf * cs, pi = idf * sn;
-  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
-  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
-}
-
-// Accuracy guard: per (target, tile) the bound q_max of |q| over the tile
-// (from the tile's lateral and axial radii) gives the FP32 rounding error of
-// t, ~eps32 q_max / (2 d lambda) cycles, and the truncation of the q-series,
-// q_max^3 / (16 d^5 lambda) cycles.  If either exceeds kGuardCycles (thick
-// 3-D tiles, near field) that target evaluates the tile exactly: FP64
-// distance from the source's FP64 position, FP64 fractional cycle, FP32
-// sin/cos and weights.
-constexpr float kGuardCycles = 2e-7f;
-
-template <int SIGN, bool OBL>
-__device__ __forceinline__ void exact_tile_pairs(
-    const double* __restrict__ u, const int32_t* __restrict__ perm,
-    const float2* __restrict__ w, int T, double vx, double vy, double vz,
-    double inv_lambda, double& ar, double& ai) {
-  float fr = 0.f, fi = 0.f;
-  for (int j = 0; j < T; ++j) {
-    const int32_t n = perm[j];
-    if (n < 0) continue;
-    const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
-                                   __builtin_bit_cast(double, w[j]));
-    mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
-  }
-  ar += fr;
-  ai += fi;
-}
 
 template <int SIGN, int R, int T, bool PLANAR, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
@@ -291,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
         const float idf = (float)id;
         const float round_err = 3e-8f * qmax * ia;
         const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
-        near[r] = round_err > kGuardCycles || trunc_err > 0.05f * kGuardCycles;
+        // NaN (a target at the tile centre) also counts as near
+        near[r] = !(round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles);
         any_near |= near[r];
       }
       const float f = (float)frac_cycle(dref * inv_lambda);
@@ -409,7 +336,7 @@ hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
   if (!ok) return hipErrorInvalidValue;
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return reduce_if_split(S, M, partial, y, stream);
+  return launch_reduce_partials(S, M, partial, y, stream);
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_weights(

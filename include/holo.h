@@ -86,8 +86,10 @@ holo_status holo_synchronize(holo_ctx *ctx);
 
 /* Tuning knobs (process-wide): "r_f64", "r_f32" targets per lane (1, 2, 4),
  * "splits" source splits (0 = automatic), "target_waves", "f64_libm" (0/1:
- * use the device libm sqrt/div/sincos in the FP64 kernel).  Results do not
- * depend on them beyond summation order.  Replaces the blockDim/gridDim/
+ * use the device libm sqrt/div/sincos in the FP64 kernel), "grid" (0/1:
+ * HOLO_FP32_SPLIT on lattice sources uses the matrix-core lattice kernel;
+ * default 1).  Results do not depend on them beyond summation order and, for
+ * "grid", the FP32-split rounding (both paths stay within ~1e-6).  Replaces the blockDim/gridDim/
  * thread_size/n_streams hints of cuda/input.h:116-122. */
 holo_status holo_set_option(const char *name, long value);
 long holo_get_option(const char *name);
@@ -150,6 +152,14 @@ void holo_plan_destroy(holo_plan *plan);
 holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
                            size_t *n_tiles, unsigned *tile,
                            double *max_offset);
+
+/* Lattice the plan detected: the sources are a product grid x_i * y_j at one
+ * z, in the order n = i*ny + j, with nx and ny multiples of 8 (the
+ * reference's plane generators, cuda/init.h:89-158).  HOLO_FP32_SPLIT then
+ * runs the lattice kernel (8 x 8 tiles as complex GEMMs on the matrix
+ * cores).  nx = ny = 0 when the plan is not a lattice.  No reference
+ * counterpart. */
+holo_status holo_plan_lattice(const holo_plan *plan, size_t *nx, size_t *ny);
 
 /* Host-only preview of the tiling a plan builds (no device needed): tile
  * count, slot -> source index map (n_tiles * tile entries, -1 = padding;

@@ -1,0 +1,157 @@
+"""The matrix-core lattice kernel (HOLO_FP32_SPLIT on product-grid sources).
+
+Bars: relative L2 <= 1e-6 against the FP64 oracle, as for every FP32-split
+path; the lattice and the tiled kernel agree to the same bar; results do not
+depend on the launch split or on the weights' scale.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM = 632.8e-9
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.nodeid}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def weights(seed, N):
+    g = np.random.default_rng(seed)
+    return g.random(N), g.random(N) * 2 * np.pi
+
+
+def targets(seed, M, z, half=3e-3):
+    g = np.random.default_rng(seed)
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * 2 * half
+    v[:, 2] = z
+    return v
+
+
+def test_lattice_detection(pkg, ctx, workloads):
+    assert ctx.plan(workloads.grid(64, 48, 10e-6, 0.0)).lattice() == (64, 48)
+    assert ctx.plan(workloads.grid(60, 48, 10e-6, 0.0)).lattice() == (0, 0)  # 60 % 8
+    u = workloads.grid(64, 48, 10e-6, 0.0)
+    assert ctx.plan(u[np.random.default_rng(0).permutation(len(u))]).lattice() == (0, 0)
+    ux = u.reshape(64, 48, 3).transpose(1, 0, 2).reshape(-1, 3)  # x fast
+    assert ctx.plan(ux).lattice() == (0, 0)
+    uz = u.copy()
+    uz[5, 2] = 1e-9
+    assert ctx.plan(uz).lattice() == (0, 0)
+    assert ctx.plan(u, 16).lattice() == (0, 0)  # the lattice kernel uses 8 x 8 tiles
+
+
+@pytest.mark.parametrize("sign", [+1, -1])
+@pytest.mark.parametrize("obliquity", [False, True])
+def test_lattice_matches_oracle(pkg, ctx, oracle, workloads, sign, obliquity):
+    u = workloads.grid(128, 96, 10e-6, 0.0)
+    amp, phase = weights(1, len(u))
+    v = targets(2, 777, 0.3)
+    plan = ctx.plan(u)
+    assert plan.lattice() == (128, 96)
+    y = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=obliquity)
+    assert rel_l2(y, ref) <= 1e-6
+
+
+def test_lattice_nonuniform_product_grid(pkg, ctx, oracle):
+    """Any x_i by y_j product (not only uniform pitch), off-centre, z != 0."""
+    g = np.random.default_rng(3)
+    xs = np.sort(g.random(64)) * 1.2e-3 + 2e-4
+    ys = np.sort(g.random(40)) * 0.8e-3 - 1e-3
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    u = np.stack([X.ravel(), Y.ravel(), np.full(X.size, 0.01)], 1)
+    amp, phase = weights(4, len(u))
+    v = targets(5, 500, 0.26)
+    plan = ctx.plan(u)
+    assert plan.lattice() == (64, 40)
+    y = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-6
+
+
+def test_lattice_and_tiled_kernels_agree(pkg, ctx, workloads):
+    u = workloads.grid(256, 256, 10e-6, 0.0)
+    amp, phase = weights(6, len(u))
+    v = workloads.grid(48, 48, 7.56e-6, 0.35)
+    plan = ctx.plan(u)
+    y64 = plan.transform(amp, phase, v, +1, LAM, pkg.FP64)
+    yg = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    pkg.set_option("grid", 0)
+    yt = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    pkg.set_option("grid", 1)
+    assert rel_l2(yg, y64) <= 1e-6 and rel_l2(yt, y64) <= 1e-6
+    assert rel_l2(yg, yt) <= 1e-6
+
+
+@pytest.mark.parametrize("splits", [1, 3, 7])
+def test_lattice_split_independent(pkg, ctx, workloads, splits):
+    u = workloads.grid(96, 96, 10e-6, 0.0)
+    amp, phase = weights(7, len(u))
+    v = targets(8, 300, 0.35)
+    plan = ctx.plan(u)
+    y0 = plan.transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    pkg.set_option("splits", splits)
+    y1 = plan.transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    # the FP32 partial sums (flushed to FP64 every 8 tiles) regroup with the
+    # split, so results move at the FP32 rounding level only
+    assert rel_l2(y1, y0) <= 1e-6
+
+
+@pytest.mark.parametrize("M", [1, 63, 64, 65, 257])
+def test_lattice_ragged_target_counts(pkg, ctx, oracle, workloads, M):
+    u = workloads.grid(64, 64, 10e-6, 0.0)
+    amp, phase = weights(9, len(u))
+    v = targets(10, M, 0.35)
+    y = ctx.plan(u).transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    assert y.shape == (M,)
+    assert rel_l2(y, oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-6
+
+
+@pytest.mark.parametrize("scale", [1e-30, 1e-8, 1e25])
+def test_lattice_weight_scale(pkg, ctx, workloads, scale):
+    """The f16 correction operand is scaled per tile: any weight scale works."""
+    u = workloads.grid(64, 64, 10e-6, 0.0)
+    amp, phase = weights(11, len(u))
+    v = targets(12, 200, 0.35)
+    plan = ctx.plan(u)
+    y1 = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    ys = plan.transform(amp * scale, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(ys / scale, y1) <= 1e-6
+
+
+@pytest.mark.parametrize("z", [1e-3, 5e-3, 2e-2])
+def test_lattice_near_field_guard(pkg, ctx, oracle, workloads, z):
+    """Near the source plane the separable expansion fails its bounds; those
+    (target, tile) pairs run exact FP64-distance pairs inside the kernel."""
+    u = workloads.grid(96, 96, 10e-6, 0.0)
+    amp, phase = weights(13, len(u))
+    v = workloads.grid(40, 40, 25e-6, z)
+    y = ctx.plan(u).transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
+
+
+def test_lattice_complex_weights_and_adjoint(pkg, ctx, oracle, workloads):
+    """Cartesian weights (the adjoint's sources) on a lattice plan, and the
+    adjoint of a lattice forward plane."""
+    v = workloads.grid(40, 32, 10e-6, 0.35)
+    g0 = np.random.default_rng(14)
+    gz = g0.standard_normal(len(v)) + 1j * g0.standard_normal(len(v))
+    amp, theta = weights(15, 2500)
+    u = targets(16, 2500, 0.0, 2e-3)
+    plan = ctx.plan(v)
+    assert plan.lattice() == (40, 32)
+    ga, gt = plan.adjoint(gz, amp, theta, u, +1, LAM, pkg.FP32_SPLIT)
+    ra, rt = oracle.adjoint(+1, LAM, amp, theta, u, gz, v)
+    assert rel_l2(ga, ra) <= 1e-6 and rel_l2(gt, rt) <= 1e-6
+    yc = plan.transform_complex(gz, u, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(yc, oracle.transform_cartesian(-1, LAM, gz, v, u)) <= 1e-6
