@@ -10,8 +10,8 @@
 // 1/d = (1/d_ref)(1 + kappa q) then factorises into a row factor, a column
 // factor and a small cross term:
 //   e^{i s 2 pi d/lambda} / d ~= e^{i s 2 pi f} X_a Y_b (1 + i s beta P_a Q_b)
-//   X_a = (1/d_ref)(1 + kappa P_a) e^{i s 2 pi (alpha P_a + c2 P_a^2)}
-//   Y_b = (1 + kappa Q_b) e^{i s 2 pi (alpha Q_b + c2 Q_b^2)},   beta = 4 pi c2
+//   X_a = (1 + kappa P_a) e^{i s 2 pi (alpha P_a + c2 P_a^2)}
+//   Y_b = (1/d_ref)(1 + kappa Q_b) e^{i s 2 pi (alpha Q_b + c2 Q_b^2)},  beta = 4 pi c2
 // with f = frac(d_ref / lambda) from FP64.  Dropped: kappa^2 P Q in the
 // amplitude and (beta P Q)^2 / 2 in the phasor; the accuracy guard bounds
 // both, with the tiled kernel's rounding and truncation bounds, and sends a
@@ -95,8 +95,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     uint32_t n_tiles, uint32_t chunk, const double* __restrict__ v, uint32_t M,
     double inv_lambda, double2* __restrict__ out) {
   // per-(target, tile) constants: c0 = (-2 r_x, -2 r_y, alpha, c2),
-  // c1 = (kappa, f, amplitude (0: exact path), s beta)
-  __shared__ float4 s_cst[kGridWaves][64][2];
+  // c1 = (kappa, f, A, A kappa) with A the amplitude 1/d_ref (all zero when
+  // the target takes the exact path); separate arrays keep the 16-lane
+  // reads of 16-byte rows free of bank conflicts
+  __shared__ float4 s_c0[kGridWaves][64];
+  __shared__ float4 s_c1[kGridWaves][64];
   __shared__ double2 s_red[kGridWaves][64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t g = lane >> 4, j = lane & 15;
@@ -146,13 +149,13 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
                           cross <= 1.5e-3f && ampsep <= 1e-6f);
       const float f = (float)frac_cycle(dref * inv_lambda);
       float4 c0 = make_float4((float)(-2.0 * rx), (float)(-2.0 * ry), (float)al, c2);
-      float4 c1 = make_float4(kappa, f, amp, (float)SIGN * kFourPi * c2);
+      float4 c1 = make_float4(kappa, f, amp, amp * kappa);
       if (near) {  // the GEMM part of this target gets exactly zero
         c0 = make_float4(0.f, 0.f, 0.f, 0.f);
         c1 = c0;
       }
-      s_cst[wv][lane][0] = c0;
-      s_cst[wv][lane][1] = c1;
+      s_c0[wv][lane] = c0;
+      s_c1[wv][lane] = c1;
       if (near)  // rare: near field
         exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 64, wc + (size_t)t * 64, 64, vx, vy,
                                     vz, inv_lambda, ex_r, ex_i);
@@ -163,12 +166,13 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const float4 go = goff[(size_t)t * 4 + g];
     const float4 A = wa[(size_t)t * 64 + lane];
     const h4v A16 = __builtin_bit_cast(h4v, wa16[(size_t)t * 64 + lane]);
-    const float ts = tscale[t] * (1.f / kTwoPow20);
+    // s beta = s 4 pi c2; the f16 product carries Q 2^20 / tscale
+    const float bt = (float)SIGN * kFourPi * tscale[t] * (1.f / kTwoPow20);
     const float gxs[2] = {go.x, go.y}, gys[2] = {go.z, go.w};
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
-      const float4 c0 = s_cst[wv][16 * B + j][0];
-      const float4 c1 = s_cst[wv][16 * B + j][1];
+      const float4 c0 = s_c0[wv][16 * B + j];
+      const float4 c1 = s_c1[wv][16 * B + j];
       float yv[4];
       h4v y16;
 #pragma unroll
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
         const float Q = __builtin_fmaf(c0.y, gy, gy * gy);
         float sn, cs;
         sincos_rev_f32(Q * __builtin_fmaf(c0.w, Q, c0.z), sn, cs);
-        const float am = __builtin_fmaf(c1.x, Q, 1.f);
+        const float am = __builtin_fmaf(c1.w, Q, c1.z);  // A (1 + kappa Q)
         const float yr = am * cs, yi = SIGN > 0 ? am * sn : -(am * sn);
         yv[2 * h] = yr;
         yv[2 * h + 1] = yi;
@@ -192,14 +196,14 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, yv[3], D, 0, 0, 0);
       f4v Dq = {0.f, 0.f, 0.f, 0.f};
       Dq = __builtin_amdgcn_mfma_f32_16x16x16f16(A16, y16, Dq, 0, 0, 0);
-      const float bs = c1.w * ts;
+      const float bs = c0.w * bt;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gx = gxs[h];
         const float P = __builtin_fmaf(c0.x, gx, gx * gx);
         float sn, cs;
         sincos_rev_f32(__builtin_fmaf(P, __builtin_fmaf(c0.w, P, c0.z), c1.y), sn, cs);
-        const float am = c1.z * __builtin_fmaf(c1.x, P, 1.f);
+        const float am = __builtin_fmaf(c1.x, P, 1.f);  // 1 + kappa P
         const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
         const float bp = bs * P;
         const float tr = __builtin_fmaf(-bp, Dq[2 * h + 1], D[2 * h]);
