@@ -94,6 +94,8 @@ struct holo_plan {
   DevBuf centre, geo, perm, wc;
   // lattice tiling (product-grid sources, 8 x 8 tiles) for the MFMA kernel
   size_t grid_nx = 0, grid_ny = 0, grid_tiles = 0;
+  double grid_cx0 = 0, grid_cx1 = 0, grid_cy0 = 0, grid_cy1 = 0, grid_z = 0;
+  float grid_rho_x = 0, grid_rho_y = 0;
   DevBuf g_centre, g_goff, g_perm, g_wc, g_wa, g_wa16, g_ts;
 };
 
@@ -351,6 +353,8 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
 // 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).
 struct HostGrid {
   size_t nx = 0, ny = 0;
+  double cx0 = INFINITY, cx1 = -INFINITY, cy0 = INFINITY, cy1 = -INFINITY, z = 0;
+  float rho_x = 0, rho_y = 0;
   std::vector<double4> centre;
   std::vector<float4> goff;
   std::vector<int32_t> perm;
@@ -372,6 +376,7 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   }
   g.nx = nx;
   g.ny = ny;
+  g.z = z;
   const size_t TI = nx / 8, TJ = ny / 8;
   g.centre.reserve(TI * TJ);
   g.goff.reserve(4 * TI * TJ);
@@ -393,8 +398,16 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
       }
       const float2 rho{std::nextafter((float)rx, INFINITY), std::nextafter((float)ry, INFINITY)};
       g.centre.push_back(double4{cx, cy, z, __builtin_bit_cast(double, rho)});
+      g.cx0 = std::min(g.cx0, cx);
+      g.cx1 = std::max(g.cx1, cx);
+      g.cy0 = std::min(g.cy0, cy);
+      g.cy1 = std::max(g.cy1, cy);
+      g.rho_x = std::max(g.rho_x, rho.x);
+      g.rho_y = std::max(g.rho_y, rho.y);
+      const float k = 1024.f;  // 2^10: exact
       for (int q = 0; q < 4; ++q)
-        g.goff.push_back(float4{gx[2 * q], gx[2 * q + 1], gy[2 * q], gy[2 * q + 1]});
+        g.goff.push_back(float4{k * gx[2 * q], k * gx[2 * q + 1], k * gy[2 * q],
+                                k * gy[2 * q + 1]});
       for (int a = 0; a < 8; ++a)
         for (int b = 0; b < 8; ++b)
           g.perm.push_back((int32_t)((8 * I + a) * ny + 8 * J + b));
@@ -429,6 +442,13 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
     if (nt * 64 > kMaxCount) return fail(HOLO_EINVAL, "too many tile slots");
     p->grid_nx = grid.nx;
     p->grid_ny = grid.ny;
+    p->grid_cx0 = grid.cx0;
+    p->grid_cx1 = grid.cx1;
+    p->grid_cy0 = grid.cy0;
+    p->grid_cy1 = grid.cy1;
+    p->grid_z = grid.z;
+    p->grid_rho_x = grid.rho_x;
+    p->grid_rho_y = grid.rho_y;
     p->grid_tiles = nt;
     HOLO_TRY(ensure(p->g_centre, nt * sizeof(double4), ctx->device));
     HOLO_TRY(ensure(p->g_goff, 4 * nt * sizeof(float4), ctx->device));
@@ -505,6 +525,13 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     gt.perm = static_cast<const int32_t*>(p->g_perm.ptr);
     gt.u = static_cast<const double*>(p->u.ptr);
     gt.n_tiles = (uint32_t)nt;
+    gt.cx0 = p->grid_cx0;
+    gt.cx1 = p->grid_cx1;
+    gt.cy0 = p->grid_cy0;
+    gt.cy1 = p->grid_cy1;
+    gt.z = p->grid_z;
+    gt.rho_x = p->grid_rho_x;
+    gt.rho_y = p->grid_rho_y;
     const int S = holo::grid_splits(M, nt);
     const size_t pe = S > 1 ? (size_t)S * M : 0;
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));

@@ -45,8 +45,29 @@ typedef _Float16 h4v __attribute__((ext_vector_type(4)));
 
 constexpr int kGridWaves = kBlock / 64;
 constexpr int kGridFlush = 8;  // tiles summed in FP32 between FP64 flushes
-constexpr float kTwoPow20 = 1048576.f;
 constexpr float kFourPi = 12.566370614359172f;
+
+// Lattice-wide bounds for the once-per-launch accuracy check.
+struct GridBounds {
+  double cx0, cx1, cy0, cy1, z;
+  float rho_x, rho_y;
+};
+
+// The accuracy bounds of one (target, tile), or of a target against every
+// tile when given the largest lateral offsets and the smallest distance: FP32
+// rounding of the phase, truncation of the q-series, the dropped second order
+// of the cross term and of the amplitude factorisation.  Written so that NaN
+// (a target at a tile centre) fails.
+__device__ __forceinline__ bool grid_pairs_ok(float Pm, float Qm, float ia, float idf,
+                                              float kappa) {
+  const float qmax = Pm + Qm;
+  const float round_err = 3e-8f * qmax * ia;
+  const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
+  const float cross = kFourPi * 0.125f * ia * idf * idf * Pm * Qm;  // |beta| P Q, radians
+  const float ampsep = kappa * kappa * Pm * Qm;
+  return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
+         ampsep <= 1e-6f;
+}
 
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
 // holds row i = 2a + c of W^T for the columns b = 2g, 2g+1.
@@ -93,11 +114,12 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const float2* __restrict__ wc, const float4* __restrict__ wa,
     const uint2* __restrict__ wa16, const float* __restrict__ tscale,
     uint32_t n_tiles, uint32_t chunk, const double* __restrict__ v, uint32_t M,
-    double inv_lambda, double2* __restrict__ out) {
-  // per-(target, tile) constants: c0 = (-2 r_x, -2 r_y, alpha, c2),
-  // c1 = (kappa, f, A, A kappa) with A the amplitude 1/d_ref (all zero when
-  // the target takes the exact path); separate arrays keep the 16-lane
-  // reads of 16-byte rows free of bank conflicts
+    double inv_lambda, GridBounds gb, double2* __restrict__ out) {
+  // per-(target, tile) constants, with the offsets scaled by 2^10 (so that
+  // P' = 2^20 P, Q' = 2^20 Q): c0 = (-2^11 r_x, -2^11 r_y, 2^-20 alpha,
+  // 2^-40 c2), c1 = (2^-20 kappa, f, A, 2^-20 A kappa) with A the amplitude
+  // 1/d_ref (all zero when the target takes the exact path); separate
+  // arrays keep the 16-lane reads of 16-byte rows free of bank conflicts
   __shared__ float4 s_c0[kGridWaves][64];
   __shared__ float4 s_c1[kGridWaves][64];
   __shared__ double2 s_red[kGridWaves][64][4];
@@ -116,6 +138,19 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
   for (int B = 0; B < 4; ++B) {
     ar[B] = ai[B] = 0.0;
     fr[B] = fi[B] = 0.f;
+  }
+  // Far-field check against every tile at once: if it holds for all 64
+  // targets of the wave, the per-tile check is skipped.
+  bool far_all;
+  {
+    const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
+    const float rym = (float)fmax(fabs(vy - gb.cy0), fabs(vy - gb.cy1));
+    const double dz = fabs(vz - gb.z);
+    const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
+    const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
+    const bool ok = grid_pairs_ok(2.f * rxm * gb.rho_x + gb.rho_x * gb.rho_x,
+                                  2.f * rym * gb.rho_y + gb.rho_y * gb.rho_y, ia, idf, kmax);
+    far_all = __all(ok);
   }
   int since = 0;
   for (uint32_t t = t0; t < t1; ++t) {
@@ -136,20 +171,16 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
         kappa = (float)(-0.5 * id2);
         amp = (float)id;
       }
-      const float Pm = 2.f * __builtin_fabsf((float)rx) * rho.x + rho.x * rho.x;
-      const float Qm = 2.f * __builtin_fabsf((float)ry) * rho.y + rho.y * rho.y;
-      const float qmax = Pm + Qm;
-      const float ia = (float)(id * inv_lambda), idf = (float)id;
-      const float round_err = 3e-8f * qmax * ia;
-      const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
-      const float cross = kFourPi * __builtin_fabsf(c2) * Pm * Qm;  // |beta| P Q, radians
-      const float ampsep = kappa * kappa * Pm * Qm;
-      // written so that NaN (a target at the tile centre) also counts as near
-      const bool near = !(round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles &&
-                          cross <= 1.5e-3f && ampsep <= 1e-6f);
+      bool near = false;
+      if (!far_all)
+        near = !grid_pairs_ok(2.f * __builtin_fabsf((float)rx) * rho.x + rho.x * rho.x,
+                              2.f * __builtin_fabsf((float)ry) * rho.y + rho.y * rho.y,
+                              (float)(id * inv_lambda), (float)id, kappa);
       const float f = (float)frac_cycle(dref * inv_lambda);
-      float4 c0 = make_float4((float)(-2.0 * rx), (float)(-2.0 * ry), (float)al, c2);
-      float4 c1 = make_float4(kappa, f, amp, amp * kappa);
+      const float ks = kappa * 0x1p-20f;
+      float4 c0 = make_float4((float)(-2048.0 * rx), (float)(-2048.0 * ry),
+                              (float)(al * 0x1p-20), c2 * 0x1p-40f);
+      float4 c1 = make_float4(ks, f, amp, amp * ks);
       if (near) {  // the GEMM part of this target gets exactly zero
         c0 = make_float4(0.f, 0.f, 0.f, 0.f);
         c1 = c0;
@@ -166,8 +197,9 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const float4 go = goff[(size_t)t * 4 + g];
     const float4 A = wa[(size_t)t * 64 + lane];
     const h4v A16 = __builtin_bit_cast(h4v, wa16[(size_t)t * 64 + lane]);
-    // s beta = s 4 pi c2; the f16 product carries Q 2^20 / tscale
-    const float bt = (float)SIGN * kFourPi * tscale[t] * (1.f / kTwoPow20);
+    // s beta P Z' = (2^-40 c2) (s 4 pi tscale) P' Z'_f16: the f16 product
+    // carries Q' = 2^20 Q and the weights / tscale
+    const float bt = (float)SIGN * kFourPi * tscale[t];
     const float gxs[2] = {go.x, go.y}, gys[2] = {go.z, go.w};
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
@@ -178,16 +210,15 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gy = gys[h];
-        const float Q = __builtin_fmaf(c0.y, gy, gy * gy);
+        const float Q = __builtin_fmaf(c0.y, gy, gy * gy);  // 2^20 Q
         float sn, cs;
         sincos_rev_f32(Q * __builtin_fmaf(c0.w, Q, c0.z), sn, cs);
         const float am = __builtin_fmaf(c1.w, Q, c1.z);  // A (1 + kappa Q)
         const float yr = am * cs, yi = SIGN > 0 ? am * sn : -(am * sn);
         yv[2 * h] = yr;
         yv[2 * h + 1] = yi;
-        const float qs = Q * kTwoPow20;
-        y16[2 * h] = (_Float16)(yr * qs);
-        y16[2 * h + 1] = (_Float16)(yi * qs);
+        y16[2 * h] = (_Float16)(yr * Q);
+        y16[2 * h + 1] = (_Float16)(yi * Q);
       }
       f4v D = {0.f, 0.f, 0.f, 0.f};
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, yv[0], D, 0, 0, 0);
@@ -200,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gx = gxs[h];
-        const float P = __builtin_fmaf(c0.x, gx, gx * gx);
+        const float P = __builtin_fmaf(c0.x, gx, gx * gx);  // 2^20 P
         float sn, cs;
         sincos_rev_f32(__builtin_fmaf(P, __builtin_fmaf(c0.w, P, c0.z), c1.y), sn, cs);
         const float am = __builtin_fmaf(c1.x, P, 1.f);  // 1 + kappa P
@@ -257,7 +288,8 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
     with_bool(obliquity, [&](auto obl) {
       k_superpose_grid<sg.value, (bool)obl.value><<<grid, kBlock, 0, stream>>>(
           gt.centre, gt.goff, gt.perm, gt.u, wc, wa, wa16, tscale, gt.n_tiles, chunk, v,
-          (uint32_t)M, il, out);
+          (uint32_t)M, il, GridBounds{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y},
+          out);
     });
   });
   hipError_t e = hipGetLastError();

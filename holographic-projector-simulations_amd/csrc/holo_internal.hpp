@@ -30,15 +30,19 @@ struct TileSet {
 // x_i (i < nx) by y_j (j < ny) at one z, in the caller's order n = i*ny + j,
 // cut into 8 x 8 blocks.  Tile t = I*(ny/8) + J holds sources (8I+a, 8J+b);
 // slot 8a+b; centre[t] = (cx, cy, z, bit-cast float2 (rho_x, rho_y)) with
-// rho the largest |offset| along x and y; goff[4t+g] = (gx_2g, gx_2g+1,
+// rho the largest |offset| along x and y; goff[4t+g] = 2^10 (gx_2g, gx_2g+1,
 // gy_2g, gy_2g+1), the FP32 offsets of rows 2g, 2g+1 and columns 2g, 2g+1
-// from the centre; perm[64t + slot] = source index.
+// from the centre, scaled by 2^10; perm[64t + slot] = source index.
+// Bounds over all tiles (for a per-target accuracy check once per launch):
+// centre x in [cx0, cx1], y in [cy0, cy1], radii <= (rho_x, rho_y), plane z.
 struct GridTiles {
   const double4* centre = nullptr;
   const float4* goff = nullptr;
   const int32_t* perm = nullptr;
   const double* u = nullptr;
   uint32_t n_tiles = 0;
+  double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, z = 0;
+  float rho_x = 0, rho_y = 0;
 };
 
 struct Tuning {
