@@ -515,7 +515,8 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     const size_t nt = p->grid_tiles;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g_perm.ptr),
                                        64 * nt, static_cast<float2*>(p->g_wc.ptr), ctx->stream));
-    HOLO_HIP(holo::launch_grid_weights(static_cast<const float2*>(p->g_wc.ptr), (uint32_t)nt,
+    HOLO_HIP(holo::launch_grid_weights(static_cast<const float2*>(p->g_wc.ptr),
+                                       static_cast<const float4*>(p->g_goff.ptr), (uint32_t)nt,
                                        static_cast<float4*>(p->g_wa.ptr),
                                        static_cast<uint2*>(p->g_wa16.ptr),
                                        static_cast<float*>(p->g_ts.ptr), ctx->stream));

@@ -16,12 +16,15 @@
 // amplitude and (beta P Q)^2 / 2 in the phasor; the accuracy guard bounds
 // both, with the tiled kernel's rounding and truncation bounds, and sends a
 // (target, tile) that fails any of them through exact FP64-distance pairs.
-// A tile then contributes
-//   S = sum_a X_a (Z_a + i s beta P_a Z'_a),
-//   Z_a = sum_b W_ab Y_b,   Z'_a = sum_b W_ab Y_b Q_b,
-// and Z, Z' for 16 targets at once are 16 x 16 real products (the complex
-// 8 x 8 written out): Z on v_mfma_f32_16x16x4f32 in FP32, Z' -- below 1e-4 of
-// Z -- on v_mfma_f32_16x16x16f16.
+// In the cross term, P_a = gx_a^2 - 2 r_x gx_a is dominated by its linear
+// part (the gx^2 share of beta P Q is below 1e-7 rad), so with the linear
+// part only a tile contributes
+//   S = sum_a X_a T_a,   T_a = sum_b W_ab Y_b + sum_b (i gx_a W_ab) (-2 s beta r_x Q_b Y_b)
+// and T for 16 targets at once is two 16 x 16 real products (complex 8 x 8
+// written out) into one accumulator: the main term on v_mfma_f32_16x16x4f32
+// in FP32 with the tile's weights divided by a power of two tscale, the
+// correction -- below 1e-4 of it -- on v_mfma_f32_16x16x16f16 accumulating
+// onto the FP32 result; X carries the factor tscale back.
 //
 // Work split (wave of 64 lanes, 64 targets): lane l = 16 g + j computes the
 // FP64 constants of target l for the tile and shares them through LDS; for
@@ -56,54 +59,63 @@ struct GridBounds {
 // The accuracy bounds of one (target, tile), or of a target against every
 // tile when given the largest lateral offsets and the smallest distance: FP32
 // rounding of the phase, truncation of the q-series, the dropped second order
-// of the cross term and of the amplitude factorisation.  Written so that NaN
-// (a target at a tile centre) fails.
-__device__ __forceinline__ bool grid_pairs_ok(float Pm, float Qm, float ia, float idf,
-                                              float kappa) {
+// of the cross term, the dropped gx^2 share of the cross term (rx2 = largest
+// gx^2) and the dropped second order of the amplitude factorisation.  Written
+// so that NaN (a target at a tile centre) fails.
+__device__ __forceinline__ bool grid_pairs_ok(float Pm, float Qm, float rx2, float ia,
+                                              float idf, float kappa) {
   const float qmax = Pm + Qm;
   const float round_err = 3e-8f * qmax * ia;
   const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
-  const float cross = kFourPi * 0.125f * ia * idf * idf * Pm * Qm;  // |beta| P Q, radians
+  const float beta = kFourPi * 0.125f * ia * idf * idf;  // |beta| = 4 pi |c2|
+  const float cross = beta * Pm * Qm;                    // radians
   const float ampsep = kappa * kappa * Pm * Qm;
   return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
-         ampsep <= 1e-6f;
+         beta * rx2 * Qm <= 1e-6f && ampsep <= 1e-6f;
 }
 
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
-// holds row i = 2a + c of W^T for the columns b = 2g, 2g+1.
+// holds row i = 2a + c, columns b = 2g, 2g+1 (re, im) of the real form of
+// W / tscale (FP32) and of i gx'_a W / tscale (f16, gx' = 2^10 gx), with
+// tscale[t] the power of two just above the tile's largest |w| component.
 __global__ __launch_bounds__(kBlock) void k_grid_weights(
-    const float2* __restrict__ wc, uint32_t n_tiles, float4* __restrict__ wa,
-    uint2* __restrict__ wa16, float* __restrict__ tscale) {
+    const float2* __restrict__ wc, const float4* __restrict__ goff, uint32_t n_tiles,
+    float4* __restrict__ wa, uint2* __restrict__ wa16, float* __restrict__ tscale) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t t = blockIdx.x * kGridWaves + (threadIdx.x >> 6);
   if (t >= n_tiles) return;  // uniform per wave
   const uint32_t i = lane & 15, g = lane >> 4, a = i >> 1, co = i & 1;
   const float2 w0 = wc[(size_t)t * 64 + 8 * a + 2 * g];
   const float2 w1 = wc[(size_t)t * 64 + 8 * a + 2 * g + 1];
-  float4 A;
-  A.x = co ? w0.y : w0.x;   // (b0, re) -> (a, c)
-  A.y = co ? w0.x : -w0.y;  // (b0, im)
-  A.z = co ? w1.y : w1.x;   // (b1, re)
-  A.w = co ? w1.x : -w1.y;  // (b1, im)
+  const float4 ga = goff[(size_t)t * 4 + (a >> 1)];
+  const float gx = (a & 1) ? ga.y : ga.x;  // 2^10 gx_a
   float mx = fmaxf(fmaxf(fabsf(w0.x), fabsf(w0.y)), fmaxf(fabsf(w1.x), fabsf(w1.y)));
   for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-  // power-of-two scale s > max |w| of the tile: the f16 copy holds w / s
   int e = 0;
   if (mx > 0.f && __builtin_isfinite(mx)) frexpf(mx, &e);
   e = e < -126 ? -126 : (e > 127 ? 127 : e);
-  const float s = ldexpf(1.f, e), is = ldexpf(1.f, -e);
-  const h4v h = {(_Float16)(A.x * is), (_Float16)(A.y * is), (_Float16)(A.z * is),
-                 (_Float16)(A.w * is)};
+  const float is = ldexpf(1.f, -e);
+  const float2 v0 = make_float2(w0.x * is, w0.y * is), v1 = make_float2(w1.x * is, w1.y * is);
+  // real form of a complex weight V: rows (a, re) = (Vr, -Vi), (a, im) = (Vi, Vr)
+  float4 A;
+  A.x = co ? v0.y : v0.x;
+  A.y = co ? v0.x : -v0.y;
+  A.z = co ? v1.y : v1.x;
+  A.w = co ? v1.x : -v1.y;
+  // V = i gx W: Vr = -gx Wi, Vi = gx Wr
+  const float4 C = make_float4(co ? gx * v0.x : -gx * v0.y, co ? -gx * v0.y : -gx * v0.x,
+                               co ? gx * v1.x : -gx * v1.y, co ? -gx * v1.y : -gx * v1.x);
+  const h4v h = {(_Float16)C.x, (_Float16)C.y, (_Float16)C.z, (_Float16)C.w};
   wa[(size_t)t * 64 + lane] = A;
   wa16[(size_t)t * 64 + lane] = __builtin_bit_cast(uint2, h);
-  if (lane == 0) tscale[t] = s;
+  if (lane == 0) tscale[t] = ldexpf(1.f, e);
 }
 
-hipError_t launch_grid_weights(const float2* wc, uint32_t n_tiles, float4* wa, uint2* wa16,
-                               float* tscale, hipStream_t stream) {
+hipError_t launch_grid_weights(const float2* wc, const float4* goff, uint32_t n_tiles,
+                               float4* wa, uint2* wa16, float* tscale, hipStream_t stream) {
   if (!n_tiles) return hipSuccess;
   const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
-  k_grid_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, wa, wa16, tscale);
+  k_grid_weights<<<blocks, kBlock, 0, stream>>>(wc, goff, n_tiles, wa, wa16, tscale);
   return hipGetLastError();
 }
 
@@ -149,7 +161,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
     const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
     const bool ok = grid_pairs_ok(2.f * rxm * gb.rho_x + gb.rho_x * gb.rho_x,
-                                  2.f * rym * gb.rho_y + gb.rho_y * gb.rho_y, ia, idf, kmax);
+                                  2.f * rym * gb.rho_y + gb.rho_y * gb.rho_y,
+                                  gb.rho_x * gb.rho_x, ia, idf, kmax);
     far_all = __all(ok);
   }
   int since = 0;
@@ -159,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     {
       const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
       double dref, id;
-      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
       const double al = 0.5 * id * inv_lambda;
       const double id2 = id * id;
       const float c2 = (float)(-0.25 * al * id2);
@@ -175,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
       if (!far_all)
         near = !grid_pairs_ok(2.f * __builtin_fabsf((float)rx) * rho.x + rho.x * rho.x,
                               2.f * __builtin_fabsf((float)ry) * rho.y + rho.y * rho.y,
-                              (float)(id * inv_lambda), (float)id, kappa);
+                              rho.x * rho.x, (float)(id * inv_lambda), (float)id, kappa);
       const float f = (float)frac_cycle(dref * inv_lambda);
       const float ks = kappa * 0x1p-20f;
       float4 c0 = make_float4((float)(-2048.0 * rx), (float)(-2048.0 * ry),
@@ -197,14 +210,15 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const float4 go = goff[(size_t)t * 4 + g];
     const float4 A = wa[(size_t)t * 64 + lane];
     const h4v A16 = __builtin_bit_cast(h4v, wa16[(size_t)t * 64 + lane]);
-    // s beta P Z' = (2^-40 c2) (s 4 pi tscale) P' Z'_f16: the f16 product
-    // carries Q' = 2^20 Q and the weights / tscale
-    const float bt = (float)SIGN * kFourPi * tscale[t];
+    const float ts = tscale[t];
     const float gxs[2] = {go.x, go.y}, gys[2] = {go.z, go.w};
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
       const float4 c0 = s_c0[wv][16 * B + j];
       const float4 c1 = s_c1[wv][16 * B + j];
+      // f16 operand -2 s beta r_x Q Y 2^-10 (the weights carry gx' = 2^10 gx)
+      // = s 4 pi (2^-40 c2) (-2^11 r_x) (2^20 Q) Y
+      const float kb = (float)SIGN * kFourPi * c0.w * c0.x;
       float yv[4];
       h4v y16;
 #pragma unroll
@@ -217,30 +231,27 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
         const float yr = am * cs, yi = SIGN > 0 ? am * sn : -(am * sn);
         yv[2 * h] = yr;
         yv[2 * h + 1] = yi;
-        y16[2 * h] = (_Float16)(yr * Q);
-        y16[2 * h + 1] = (_Float16)(yi * Q);
+        const float qk = Q * kb;
+        y16[2 * h] = (_Float16)(yr * qk);
+        y16[2 * h + 1] = (_Float16)(yi * qk);
       }
       f4v D = {0.f, 0.f, 0.f, 0.f};
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, yv[0], D, 0, 0, 0);
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, yv[1], D, 0, 0, 0);
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, yv[2], D, 0, 0, 0);
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, yv[3], D, 0, 0, 0);
-      f4v Dq = {0.f, 0.f, 0.f, 0.f};
-      Dq = __builtin_amdgcn_mfma_f32_16x16x16f16(A16, y16, Dq, 0, 0, 0);
-      const float bs = c0.w * bt;
+      D = __builtin_amdgcn_mfma_f32_16x16x16f16(A16, y16, D, 0, 0, 0);  // T / tscale
+      const float ks = c1.x * ts;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gx = gxs[h];
         const float P = __builtin_fmaf(c0.x, gx, gx * gx);  // 2^20 P
         float sn, cs;
         sincos_rev_f32(__builtin_fmaf(P, __builtin_fmaf(c0.w, P, c0.z), c1.y), sn, cs);
-        const float am = __builtin_fmaf(c1.x, P, 1.f);  // 1 + kappa P
+        const float am = __builtin_fmaf(ks, P, ts);  // tscale (1 + kappa P)
         const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
-        const float bp = bs * P;
-        const float tr = __builtin_fmaf(-bp, Dq[2 * h + 1], D[2 * h]);
-        const float ti = __builtin_fmaf(bp, Dq[2 * h], D[2 * h + 1]);
-        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
-        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
+        fr[B] = __builtin_fmaf(xr, D[2 * h], __builtin_fmaf(-xi, D[2 * h + 1], fr[B]));
+        fi[B] = __builtin_fmaf(xr, D[2 * h + 1], __builtin_fmaf(xi, D[2 * h], fi[B]));
       }
     }
     if (++since == kGridFlush) {

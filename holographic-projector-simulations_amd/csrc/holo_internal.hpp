@@ -109,10 +109,11 @@ hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
 
 // Lattice kernel (FP32-split on product-grid tiles).  wc: per-slot FP32
 // weights (launch_tile_weights with the grid perm).  launch_grid_weights
-// lays them out as the MFMA operands: wa (float4 per tile and lane), wa16
-// (4 x f16 per tile and lane, scaled by 1 / tscale[t], a power of two).
-hipError_t launch_grid_weights(const float2* wc, uint32_t n_tiles, float4* wa, uint2* wa16,
-                               float* tscale, hipStream_t stream);
+// lays them out as the MFMA operands, divided by tscale[t] (a power of
+// two): wa (float4 per tile and lane) and wa16 (4 x f16 per tile and lane,
+// the cross-term weights i gx W).
+hipError_t launch_grid_weights(const float2* wc, const float4* goff, uint32_t n_tiles,
+                               float4* wa, uint2* wa16, float* tscale, hipStream_t stream);
 hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
                                  const float2* wc, const float4* wa, const uint2* wa16,
                                  const float* tscale, const double* v, size_t M,

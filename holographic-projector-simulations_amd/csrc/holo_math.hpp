@@ -95,6 +95,28 @@ HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
   inv = __builtin_fma(h2, e, h2);
 }
 
+// d = sqrt(s) to within an ulp and inv = 1/d to ~1e-13 relative (the
+// Goldschmidt step and one Newton correction of hm_sqrt_inv, without the
+// final correction and the reciprocal refinement).  For the FP32-split
+// kernels, which need d only for the fractional cycle d / lambda (an ulp of
+// d is ~1e-11 cycles at 0.35 m) and 1/d only in FP32.
+HM_HD void hm_sqrt_inv_fast(double s, double& d, double& inv) {
+#pragma clang fp contract(off)
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(s);
+#else
+  const double y = 1.0 / __builtin_sqrt(s);
+#endif
+  double g = s * y;
+  double h = 0.5 * y;
+  const double e = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  const double rr = __builtin_fma(-g, g, s);
+  d = __builtin_fma(rr, h, g);
+  inv = h + h;
+}
+
 #undef HM_HD
 
 }  // namespace holo
