@@ -74,6 +74,21 @@ def cpu_baseline(stage, n_targets: int):
     res = {"value": N * n_targets / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
            "sample": f"x->y stage, {N} sources x {n_targets} stride-sampled targets "
                      f"({N * n_targets:.3g} pairs, {dt:.1f} s wall on {cores} processes)"}
+    # the C FP64 restatement, single-threaded and with OpenMP, on the same subset
+    import oracle
+    c_sub = sub[:: max(1, n_targets // 64)]
+    t0 = time.perf_counter()
+    oracle.transform_full(stage.sign, stage.lam, stage.amp, stage.phase, stage.u, stage.v,
+                          targets=c_sub, nthreads=1)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.transform_full(stage.sign, stage.lam, stage.amp, stage.phase, stage.u, stage.v,
+                          targets=sub, nthreads=cores)
+    tm = time.perf_counter() - t0
+    res["c_oracle"] = {"single_thread_pairs_per_s": N * c_sub.size / t1,
+                       "openmp_pairs_per_s": N * n_targets / tm, "openmp_threads": cores,
+                       "sample": f"{c_sub.size} (1 thread) and {n_targets} ({cores} threads) "
+                                 "of the same stride-sampled targets"}
     return res, sub, y[:, 0] * np.exp(1j * y[:, 1])
 
 
