@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -91,7 +92,11 @@ struct holo_plan {
   size_t n_tiles = 0;
   double max_offset = 0.0;
   bool planar = true;
-  DevBuf centre, geo, perm, wc;
+  // The spatial tiling for k_superpose_f32s.  A lattice plan defers it to
+  // the first call that needs it (the lattice kernel does not).
+  mutable bool tiles_built = false;
+  mutable std::mutex tiles_mu;
+  mutable DevBuf centre, geo, perm, wc;
   // lattice tiling (product-grid sources, 8 x 8 tiles) for the MFMA kernel
   size_t grid_nx = 0, grid_ny = 0, grid_tiles = 0;
   double grid_cx0 = 0, grid_cx1 = 0, grid_cy0 = 0, grid_cy1 = 0, grid_z = 0;
@@ -415,27 +420,45 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   return true;
 }
 
-holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
+// Builds and uploads the spatial tiling for k_superpose_f32s (u_host: the
+// plan's positions on the host, or NULL to read them back from the device).
+// Waits for the copies before returning.
+holo_status plan_build_tiles(holo_ctx* ctx, const holo_plan* p, const double* u_host) {
+  std::vector<double> host;
+  if (!u_host && p->N) {
+    host.resize(3 * p->N);
+    HOLO_HIP(hipMemcpyAsync(host.data(), p->u.ptr, 3 * p->N * sizeof(double),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    HOLO_HIP(hipStreamSynchronize(ctx->stream));
+    u_host = host.data();
+  }
   HostTiles t;
   build_tiles(u_host, p->N, p->tile, t);
-  p->n_tiles = t.centre.size();
-  p->max_offset = t.max_offset;
-  p->planar = t.planar;
-  if (p->n_tiles * (size_t)p->tile > kMaxCount)
-    return fail(HOLO_EINVAL, "too many tile slots");
-  const size_t slots = p->n_tiles * p->tile;
-  HOLO_TRY(ensure(p->centre, std::max<size_t>(p->n_tiles, 1) * sizeof(double4), ctx->device));
+  const size_t n_tiles = t.centre.size();
+  if (n_tiles * (size_t)p->tile > kMaxCount) return fail(HOLO_EINVAL, "too many tile slots");
+  const size_t slots = n_tiles * p->tile;
+  HOLO_TRY(ensure(p->centre, std::max<size_t>(n_tiles, 1) * sizeof(double4), ctx->device));
   HOLO_TRY(ensure(p->geo, std::max<size_t>(slots, 1) * sizeof(float4), ctx->device));
   HOLO_TRY(ensure(p->perm, std::max<size_t>(slots, 1) * sizeof(int32_t), ctx->device));
   HOLO_TRY(ensure(p->wc, std::max<size_t>(slots, 1) * sizeof(float2), ctx->device));
-  if (p->n_tiles) {
-    HOLO_HIP(hipMemcpyAsync(p->centre.ptr, t.centre.data(), p->n_tiles * sizeof(double4),
+  if (n_tiles) {
+    HOLO_HIP(hipMemcpyAsync(p->centre.ptr, t.centre.data(), n_tiles * sizeof(double4),
                             hipMemcpyHostToDevice, ctx->stream));
     HOLO_HIP(hipMemcpyAsync(p->geo.ptr, t.geo.data(), slots * sizeof(float4),
                             hipMemcpyHostToDevice, ctx->stream));
     HOLO_HIP(hipMemcpyAsync(p->perm.ptr, t.perm.data(), slots * sizeof(int32_t),
                             hipMemcpyHostToDevice, ctx->stream));
   }
+  HOLO_HIP(hipStreamSynchronize(ctx->stream));  // host vectors are freed on return
+  holo_plan* q = const_cast<holo_plan*>(p);
+  q->n_tiles = n_tiles;
+  q->max_offset = t.max_offset;
+  q->planar = t.planar;
+  p->tiles_built = true;
+  return HOLO_OK;
+}
+
+holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
   HostGrid grid;
   if (p->tile == 64 && build_grid(u_host, p->N, grid)) {
     const size_t nt = grid.centre.size();
@@ -466,6 +489,11 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
   }
   // host vectors are freed on return: wait for the copies
   HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  if (!p->grid_tiles) return plan_build_tiles(ctx, p, u_host);
+  // a lattice: the same 8 x 8 tiles the spatial tiler would cut (built on demand)
+  p->n_tiles = p->grid_tiles;
+  p->max_offset = std::hypot((double)p->grid_rho_x, (double)p->grid_rho_y);
+  p->planar = true;
   return HOLO_OK;
 }
 
@@ -545,6 +573,10 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
                                          static_cast<double2*>(ctx->partial.ptr), pe,
                                          ctx->stream));
   } else {
+    {
+      std::lock_guard<std::mutex> lock(p->tiles_mu);
+      if (!p->tiles_built) HOLO_TRY(plan_build_tiles(ctx, p, nullptr));
+    }
     const size_t slots = p->n_tiles * p->tile;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->perm.ptr),
                                        slots, static_cast<float2*>(p->wc.ptr), ctx->stream));
