@@ -101,7 +101,14 @@ struct holo_plan {
   size_t grid_nx = 0, grid_ny = 0, grid_tiles = 0;
   double grid_cx0 = 0, grid_cx1 = 0, grid_cy0 = 0, grid_cy1 = 0, grid_z = 0;
   float grid_rho_x = 0, grid_rho_y = 0;
-  DevBuf g_centre, g_goff, g_perm, g_wc, g_wa, g_wa16, g_ts;
+  DevBuf g_centre, g_goff, g_perm, g_wc, g_wh, g_wl, g_ts;
+  // 16 x 16 tiles of the same lattice when its pitches are uniform and nx, ny
+  // are multiples of 16 (g16_tiles = 0 otherwise)
+  size_t g16_tiles = 0;
+  double g16_px = 0, g16_py = 0;
+  double g16_cx0 = 0, g16_cx1 = 0, g16_cy0 = 0, g16_cy1 = 0;
+  float g16_rho_x = 0, g16_rho_y = 0;
+  DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_ts;
 };
 
 namespace {
@@ -355,7 +362,11 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
 // x depending on the row i only, y on the column j only, and one z (the
 // reference's plane generators, cuda/init.h:89-158, x slow as in
 // cuda/macros.h:59).  nx and ny must be multiples of 8; the grid is cut into
-// 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).
+// 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).  When
+// both are multiples of 16 and the positions are x_0 + i p_x, y_0 + j p_y
+// (to 1e-9 of a pitch), it is also cut into 16 x 16 tiles t = I*(ny/16) + J,
+// slot 16a + b = source (16I+a, 16J+b), centred at the lattice points
+// x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.
 struct HostGrid {
   size_t nx = 0, ny = 0;
   double cx0 = INFINITY, cx1 = -INFINITY, cy0 = INFINITY, cy1 = -INFINITY, z = 0;
@@ -363,7 +374,46 @@ struct HostGrid {
   std::vector<double4> centre;
   std::vector<float4> goff;
   std::vector<int32_t> perm;
+  double px = 0, py = 0;  // nonzero: 16 x 16 tiles below
+  double cx0_16 = INFINITY, cx1_16 = -INFINITY, cy0_16 = INFINITY, cy1_16 = -INFINITY;
+  float rho16_x = 0, rho16_y = 0;
+  std::vector<double4> centre16;
+  std::vector<int32_t> perm16;
 };
+
+void build_grid16(const double* u, HostGrid& g) {
+  const size_t nx = g.nx, ny = g.ny;
+  if (nx % 16 || ny % 16) return;
+  const double x0 = u[0], y0 = u[1];
+  const double px = (u[3 * (nx - 1) * ny] - x0) / (double)(nx - 1);
+  const double py = (u[3 * (ny - 1) + 1] - y0) / (double)(ny - 1);
+  if (!(px != 0.0 && py != 0.0 && std::isfinite(px) && std::isfinite(py))) return;
+  for (size_t i = 0; i < nx; ++i)
+    if (!(std::fabs(u[3 * i * ny] - (x0 + (double)i * px)) <= 1e-9 * std::fabs(px))) return;
+  for (size_t j = 0; j < ny; ++j)
+    if (!(std::fabs(u[3 * j + 1] - (y0 + (double)j * py)) <= 1e-9 * std::fabs(py))) return;
+  g.px = px;
+  g.py = py;
+  g.rho16_x = std::nextafter((float)(7.5 * std::fabs(px)), INFINITY);
+  g.rho16_y = std::nextafter((float)(7.5 * std::fabs(py)), INFINITY);
+  const float2 rho{g.rho16_x, g.rho16_y};
+  const size_t TI = nx / 16, TJ = ny / 16;
+  g.centre16.reserve(TI * TJ);
+  g.perm16.reserve(256 * TI * TJ);
+  for (size_t I = 0; I < TI; ++I)
+    for (size_t J = 0; J < TJ; ++J) {
+      const double cx = x0 + ((double)(16 * I) + 7.5) * px;
+      const double cy = y0 + ((double)(16 * J) + 7.5) * py;
+      g.centre16.push_back(double4{cx, cy, g.z, __builtin_bit_cast(double, rho)});
+      g.cx0_16 = std::min(g.cx0_16, cx);
+      g.cx1_16 = std::max(g.cx1_16, cx);
+      g.cy0_16 = std::min(g.cy0_16, cy);
+      g.cy1_16 = std::max(g.cy1_16, cy);
+      for (int a = 0; a < 16; ++a)
+        for (int b = 0; b < 16; ++b)
+          g.perm16.push_back((int32_t)((16 * I + a) * ny + 16 * J + b));
+    }
+}
 
 bool build_grid(const double* u, size_t N, HostGrid& g) {
   g = HostGrid();
@@ -417,6 +467,7 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
         for (int b = 0; b < 8; ++b)
           g.perm.push_back((int32_t)((8 * I + a) * ny + 8 * J + b));
     }
+  build_grid16(u, g);
   return true;
 }
 
@@ -477,8 +528,8 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
     HOLO_TRY(ensure(p->g_goff, 4 * nt * sizeof(float4), ctx->device));
     HOLO_TRY(ensure(p->g_perm, 64 * nt * sizeof(int32_t), ctx->device));
     HOLO_TRY(ensure(p->g_wc, 64 * nt * sizeof(float2), ctx->device));
-    HOLO_TRY(ensure(p->g_wa, 64 * nt * sizeof(float4), ctx->device));
-    HOLO_TRY(ensure(p->g_wa16, 64 * nt * sizeof(uint2), ctx->device));
+    HOLO_TRY(ensure(p->g_wh, 64 * nt * sizeof(uint4), ctx->device));
+    HOLO_TRY(ensure(p->g_wl, 64 * nt * sizeof(uint4), ctx->device));
     HOLO_TRY(ensure(p->g_ts, nt * sizeof(float), ctx->device));
     HOLO_HIP(hipMemcpyAsync(p->g_centre.ptr, grid.centre.data(), nt * sizeof(double4),
                             hipMemcpyHostToDevice, ctx->stream));
@@ -486,6 +537,27 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
                             hipMemcpyHostToDevice, ctx->stream));
     HOLO_HIP(hipMemcpyAsync(p->g_perm.ptr, grid.perm.data(), 64 * nt * sizeof(int32_t),
                             hipMemcpyHostToDevice, ctx->stream));
+    const size_t n16 = grid.centre16.size();
+    if (n16) {
+      p->g16_tiles = n16;
+      p->g16_px = grid.px;
+      p->g16_py = grid.py;
+      p->g16_cx0 = grid.cx0_16;
+      p->g16_cx1 = grid.cx1_16;
+      p->g16_cy0 = grid.cy0_16;
+      p->g16_cy1 = grid.cy1_16;
+      p->g16_rho_x = grid.rho16_x;
+      p->g16_rho_y = grid.rho16_y;
+      HOLO_TRY(ensure(p->g16_centre, n16 * sizeof(double4), ctx->device));
+      HOLO_TRY(ensure(p->g16_perm, 256 * n16 * sizeof(int32_t), ctx->device));
+      HOLO_TRY(ensure(p->g16_wc, 256 * n16 * sizeof(float2), ctx->device));
+      HOLO_TRY(ensure(p->g16_wq, 6 * 64 * n16 * sizeof(uint4), ctx->device));
+      HOLO_TRY(ensure(p->g16_ts, n16 * sizeof(float), ctx->device));
+      HOLO_HIP(hipMemcpyAsync(p->g16_centre.ptr, grid.centre16.data(), n16 * sizeof(double4),
+                              hipMemcpyHostToDevice, ctx->stream));
+      HOLO_HIP(hipMemcpyAsync(p->g16_perm.ptr, grid.perm16.data(),
+                              256 * n16 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    }
   }
   // host vectors are freed on return: wait for the copies
   HOLO_HIP(hipStreamSynchronize(ctx->stream));
@@ -539,14 +611,46 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
                                         p->N, v, M, K, reinterpret_cast<double2*>(y),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));
+  } else if (p->g16_tiles && holo::tuning().grid && holo::tuning().grid16) {
+    const size_t nt = p->g16_tiles;
+    HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g16_perm.ptr),
+                                       256 * nt, static_cast<float2*>(p->g16_wc.ptr),
+                                       ctx->stream));
+    HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
+                                         static_cast<uint4*>(p->g16_wq.ptr),
+                                         static_cast<float*>(p->g16_ts.ptr), ctx->stream));
+    holo::GridTiles gt;
+    gt.centre = static_cast<const double4*>(p->g16_centre.ptr);
+    gt.perm = static_cast<const int32_t*>(p->g16_perm.ptr);
+    gt.u = static_cast<const double*>(p->u.ptr);
+    gt.n_tiles = (uint32_t)nt;
+    gt.cx0 = p->g16_cx0;
+    gt.cx1 = p->g16_cx1;
+    gt.cy0 = p->g16_cy0;
+    gt.cy1 = p->g16_cy1;
+    gt.z = p->grid_z;
+    gt.rho_x = p->g16_rho_x;
+    gt.rho_y = p->g16_rho_y;
+    gt.px = p->g16_px;
+    gt.py = p->g16_py;
+    const int S = holo::grid_splits(M, nt);
+    const size_t pe = S > 1 ? (size_t)S * M : 0;
+    if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
+    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, gt,
+                                           static_cast<const float2*>(p->g16_wc.ptr),
+                                           static_cast<const uint4*>(p->g16_wq.ptr),
+                                           static_cast<const float*>(p->g16_ts.ptr), v, M,
+                                           lambda, reinterpret_cast<double2*>(y),
+                                           static_cast<double2*>(ctx->partial.ptr), pe,
+                                           ctx->stream));
   } else if (p->grid_tiles && holo::tuning().grid) {
     const size_t nt = p->grid_tiles;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g_perm.ptr),
                                        64 * nt, static_cast<float2*>(p->g_wc.ptr), ctx->stream));
     HOLO_HIP(holo::launch_grid_weights(static_cast<const float2*>(p->g_wc.ptr),
                                        static_cast<const float4*>(p->g_goff.ptr), (uint32_t)nt,
-                                       static_cast<float4*>(p->g_wa.ptr),
-                                       static_cast<uint2*>(p->g_wa16.ptr),
+                                       static_cast<uint4*>(p->g_wh.ptr),
+                                       static_cast<uint4*>(p->g_wl.ptr),
                                        static_cast<float*>(p->g_ts.ptr), ctx->stream));
     holo::GridTiles gt;
     gt.centre = static_cast<const double4*>(p->g_centre.ptr);
@@ -566,8 +670,8 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     HOLO_HIP(holo::launch_superpose_grid(sign, obliquity, gt,
                                          static_cast<const float2*>(p->g_wc.ptr),
-                                         static_cast<const float4*>(p->g_wa.ptr),
-                                         static_cast<const uint2*>(p->g_wa16.ptr),
+                                         static_cast<const uint4*>(p->g_wh.ptr),
+                                         static_cast<const uint4*>(p->g_wl.ptr),
                                          static_cast<const float*>(p->g_ts.ptr), v, M, lambda,
                                          reinterpret_cast<double2*>(y),
                                          static_cast<double2*>(ctx->partial.ptr), pe,
@@ -693,6 +797,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.f64_libm = value != 0;
   } else if (n == "grid") {
     t.grid = value != 0;
+  } else if (n == "grid16") {
+    t.grid16 = value != 0;
   } else {
     return fail(HOLO_EINVAL, "unknown option " + n);
   }
@@ -709,6 +815,7 @@ long holo_get_option(const char* name) {
   if (n == "target_waves") return t.target_waves;
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
   if (n == "grid") return t.grid ? 1 : 0;
+  if (n == "grid16") return t.grid16 ? 1 : 0;
   return -1;
 }
 

@@ -20,19 +20,25 @@
 // part (the gx^2 share of beta P Q is below 1e-7 rad), so with the linear
 // part only a tile contributes
 //   S = sum_a X_a T_a,   T_a = sum_b W_ab Y_b + sum_b (i gx_a W_ab) (-2 s beta r_x Q_b Y_b)
-// and T for 16 targets at once is two 16 x 16 real products (complex 8 x 8
-// written out) into one accumulator: the main term on v_mfma_f32_16x16x4f32
-// in FP32 with the tile's weights divided by a power of two tscale, the
-// correction -- below 1e-4 of it -- on v_mfma_f32_16x16x16f16 accumulating
-// onto the FP32 result; X carries the factor tscale back.
+// and T for 16 targets at once is a 16 x 16 real product (complex 8 x 8
+// written out) on the matrix cores.  Both operands are split into f16 pairs,
+// x = x_hi + x_lo with x_hi the value rounded to 11 bits, and two
+// v_mfma_f32_16x16x32_f16 accumulate, in FP32,
+//   W_hi Y_hi + W_hi Y_lo   and   W_lo Y_hi + (i gx W) (correction)
+// leaving out only W_lo Y_lo (below 2^-22 of each term); the correction is
+// below 1e-4 of T, so f16 suffices for it.  The weights are divided by a power
+// of two tscale per tile and Y is kept near unit size (Y' = Y / A), so the
+// f16 range holds; X carries tscale and the amplitude A back.
 //
 // Work split (wave of 64 lanes, 64 targets): lane l = 16 g + j computes the
 // FP64 constants of target l for the tile and shares them through LDS; for
 // each block B of 16 targets it then supplies Y of columns 2g, 2g+1 of target
 // 16 B + j as the GEMM's B operand and receives Z of rows 2g, 2g+1, which it
-// multiplies by its X.  Operand layout (16x16x4: lane l holds A[l%16][l/16],
-// B[l/16][l%16], D[4(l/16)+r][l%16]; 16x16x16 holds four consecutive k):
-//   k = 4g + s  <->  column b = 2g + s/2, component s%2 (re, im)
+// multiplies by its X.  Operand layout of 16x16x32 f16: lane l holds eight
+// consecutive k of row l%16 of A and of column l%16 of B, and D[4(l/16)+r][l%16];
+// A and B use the same k for the same (lane group g, slot e), and the slots are
+//   e < 4: column b = 2g + e/2, component e%2 (re, im), hi part
+//   e >= 4: the same column and component, lo part or correction
 //   i = 2a + c  <->  row a, component c
 // so the weights enter as A[2a+c][2b+c'] = (Wr, -Wi; Wi, Wr) (k_grid_weights).
 #include <hip/hip_runtime.h>
@@ -44,7 +50,7 @@
 namespace holo {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 constexpr int kGridWaves = kBlock / 64;
 constexpr int kGridFlush = 8;  // tiles summed in FP32 between FP64 flushes
@@ -74,13 +80,29 @@ __device__ __forceinline__ bool grid_pairs_ok(float Pm, float Qm, float rx2, flo
          beta * rx2 * Qm <= 1e-6f && ampsep <= 1e-6f;
 }
 
+// x = hi + lo in f16: hi is x rounded to 11 significant bits (ties away from
+// zero) by integer arithmetic, so hi is exact in f16 for |x| >= 2^-14 and
+// every conversion instruction yields the same hi; lo = x - hi is exact in
+// FP32 and |lo| <= 2^-11 |x|.  (Splitting as lo = x - (float)(half)x is not
+// safe: the compiler may fold the product that made x into the f16
+// conversion -- one rounding instead of two -- for one use of hi and not for
+// another, and at exact ties hi + lo is then off by one f16 ulp.)
+__device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo) {
+  const float h =
+      __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, x) + 0x1000u) & 0xffffe000u);
+  hi = (_Float16)h;
+  lo = (_Float16)(x - h);
+}
+
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
 // holds row i = 2a + c, columns b = 2g, 2g+1 (re, im) of the real form of
-// W / tscale (FP32) and of i gx'_a W / tscale (f16, gx' = 2^10 gx), with
-// tscale[t] the power of two just above the tile's largest |w| component.
+// W / tscale, split into f16 hi and lo, and of i gx'_a W / tscale (gx' =
+// 2^10 gx), with tscale[t] the power of two just above the tile's largest |w|
+// component: wh = (hi, hi), wl = (lo, i gx' W / tscale) in the slot order of
+// the 16x16x32 operand.
 __global__ __launch_bounds__(kBlock) void k_grid_weights(
     const float2* __restrict__ wc, const float4* __restrict__ goff, uint32_t n_tiles,
-    float4* __restrict__ wa, uint2* __restrict__ wa16, float* __restrict__ tscale) {
+    uint4* __restrict__ wh, uint4* __restrict__ wl, float* __restrict__ tscale) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t t = blockIdx.x * kGridWaves + (threadIdx.x >> 6);
   if (t >= n_tiles) return;  // uniform per wave
@@ -97,25 +119,31 @@ __global__ __launch_bounds__(kBlock) void k_grid_weights(
   const float is = ldexpf(1.f, -e);
   const float2 v0 = make_float2(w0.x * is, w0.y * is), v1 = make_float2(w1.x * is, w1.y * is);
   // real form of a complex weight V: rows (a, re) = (Vr, -Vi), (a, im) = (Vi, Vr)
-  float4 A;
-  A.x = co ? v0.y : v0.x;
-  A.y = co ? v0.x : -v0.y;
-  A.z = co ? v1.y : v1.x;
-  A.w = co ? v1.x : -v1.y;
+  const float A[4] = {co ? v0.y : v0.x, co ? v0.x : -v0.y, co ? v1.y : v1.x,
+                      co ? v1.x : -v1.y};
   // V = i gx W: Vr = -gx Wi, Vi = gx Wr
-  const float4 C = make_float4(co ? gx * v0.x : -gx * v0.y, co ? -gx * v0.y : -gx * v0.x,
-                               co ? gx * v1.x : -gx * v1.y, co ? -gx * v1.y : -gx * v1.x);
-  const h4v h = {(_Float16)C.x, (_Float16)C.y, (_Float16)C.z, (_Float16)C.w};
-  wa[(size_t)t * 64 + lane] = A;
-  wa16[(size_t)t * 64 + lane] = __builtin_bit_cast(uint2, h);
+  const float C[4] = {co ? gx * v0.x : -gx * v0.y, co ? -gx * v0.y : -gx * v0.x,
+                      co ? gx * v1.x : -gx * v1.y, co ? -gx * v1.y : -gx * v1.x};
+  h8v hi, lo;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    _Float16 h, l;
+    split_f16(A[s], h, l);
+    hi[s] = h;
+    hi[s + 4] = h;
+    lo[s] = l;
+    lo[s + 4] = (_Float16)C[s];
+  }
+  wh[(size_t)t * 64 + lane] = __builtin_bit_cast(uint4, hi);
+  wl[(size_t)t * 64 + lane] = __builtin_bit_cast(uint4, lo);
   if (lane == 0) tscale[t] = ldexpf(1.f, e);
 }
 
 hipError_t launch_grid_weights(const float2* wc, const float4* goff, uint32_t n_tiles,
-                               float4* wa, uint2* wa16, float* tscale, hipStream_t stream) {
+                               uint4* wh, uint4* wl, float* tscale, hipStream_t stream) {
   if (!n_tiles) return hipSuccess;
   const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
-  k_grid_weights<<<blocks, kBlock, 0, stream>>>(wc, goff, n_tiles, wa, wa16, tscale);
+  k_grid_weights<<<blocks, kBlock, 0, stream>>>(wc, goff, n_tiles, wh, wl, tscale);
   return hipGetLastError();
 }
 
@@ -123,14 +151,15 @@ template <int SIGN, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     const double4* __restrict__ centre, const float4* __restrict__ goff,
     const int32_t* __restrict__ perm, const double* __restrict__ u,
-    const float2* __restrict__ wc, const float4* __restrict__ wa,
-    const uint2* __restrict__ wa16, const float* __restrict__ tscale,
+    const float2* __restrict__ wc, const uint4* __restrict__ wh,
+    const uint4* __restrict__ wl, const float* __restrict__ tscale,
     uint32_t n_tiles, uint32_t chunk, const double* __restrict__ v, uint32_t M,
     double inv_lambda, GridBounds gb, double2* __restrict__ out) {
   // per-(target, tile) constants, with the offsets scaled by 2^10 (so that
   // P' = 2^20 P, Q' = 2^20 Q): c0 = (-2^11 r_x, -2^11 r_y, 2^-20 alpha,
   // 2^-40 c2), c1 = (2^-20 kappa, f, A, 2^-20 A kappa) with A the amplitude
-  // 1/d_ref (all zero when the target takes the exact path); separate
+  // 1/d_ref (all zero when the target takes the exact path, which makes its
+  // X, and so its share of the GEMM, exactly zero); separate
   // arrays keep the 16-lane reads of 16-byte rows free of bank conflicts
   __shared__ float4 s_c0[kGridWaves][64];
   __shared__ float4 s_c1[kGridWaves][64];
@@ -194,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
       float4 c0 = make_float4((float)(-2048.0 * rx), (float)(-2048.0 * ry),
                               (float)(al * 0x1p-20), c2 * 0x1p-40f);
       float4 c1 = make_float4(ks, f, amp, amp * ks);
-      if (near) {  // the GEMM part of this target gets exactly zero
+      if (near) {  // X = 0: the GEMM part of this target adds exactly zero
         c0 = make_float4(0.f, 0.f, 0.f, 0.f);
         c1 = c0;
       }
@@ -208,47 +237,48 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float4 go = goff[(size_t)t * 4 + g];
-    const float4 A = wa[(size_t)t * 64 + lane];
-    const h4v A16 = __builtin_bit_cast(h4v, wa16[(size_t)t * 64 + lane]);
+    const h8v Wh = __builtin_bit_cast(h8v, wh[(size_t)t * 64 + lane]);
+    const h8v Wl = __builtin_bit_cast(h8v, wl[(size_t)t * 64 + lane]);
     const float ts = tscale[t];
     const float gxs[2] = {go.x, go.y}, gys[2] = {go.z, go.w};
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
       const float4 c0 = s_c0[wv][16 * B + j];
       const float4 c1 = s_c1[wv][16 * B + j];
-      // f16 operand -2 s beta r_x Q Y 2^-10 (the weights carry gx' = 2^10 gx)
-      // = s 4 pi (2^-40 c2) (-2^11 r_x) (2^20 Q) Y
+      // correction operand -2 s beta r_x Q Y' 2^-10 (the weights carry
+      // gx' = 2^10 gx) = s 4 pi (2^-40 c2) (-2^11 r_x) (2^20 Q) Y'
       const float kb = (float)SIGN * kFourPi * c0.w * c0.x;
-      float yv[4];
-      h4v y16;
+      h8v y1, y2;  // (Y'_hi, Y'_lo), (Y'_hi, correction)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gy = gys[h];
         const float Q = __builtin_fmaf(c0.y, gy, gy * gy);  // 2^20 Q
         float sn, cs;
         sincos_rev_f32(Q * __builtin_fmaf(c0.w, Q, c0.z), sn, cs);
-        const float am = __builtin_fmaf(c1.w, Q, c1.z);  // A (1 + kappa Q)
-        const float yr = am * cs, yi = SIGN > 0 ? am * sn : -(am * sn);
-        yv[2 * h] = yr;
-        yv[2 * h + 1] = yi;
+        const float am = __builtin_fmaf(c1.x, Q, 1.f);  // 1 + kappa Q
+        const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
         const float qk = Q * kb;
-        y16[2 * h] = (_Float16)(yr * qk);
-        y16[2 * h + 1] = (_Float16)(yi * qk);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          _Float16 hi, lo;
+          split_f16(yv[c], hi, lo);
+          y1[2 * h + c] = hi;
+          y1[4 + 2 * h + c] = lo;
+          y2[2 * h + c] = hi;
+          y2[4 + 2 * h + c] = (_Float16)(yv[c] * qk);
+        }
       }
       f4v D = {0.f, 0.f, 0.f, 0.f};
-      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.x, yv[0], D, 0, 0, 0);
-      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.y, yv[1], D, 0, 0, 0);
-      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.z, yv[2], D, 0, 0, 0);
-      D = __builtin_amdgcn_mfma_f32_16x16x4f32(A.w, yv[3], D, 0, 0, 0);
-      D = __builtin_amdgcn_mfma_f32_16x16x16f16(A16, y16, D, 0, 0, 0);  // T / tscale
-      const float ks = c1.x * ts;
+      D = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wh, y1, D, 0, 0, 0);
+      D = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wl, y2, D, 0, 0, 0);  // T / (tscale A)
+      const float as = c1.z * ts, ks = c1.w * ts;
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
         const float gx = gxs[h];
         const float P = __builtin_fmaf(c0.x, gx, gx * gx);  // 2^20 P
         float sn, cs;
         sincos_rev_f32(__builtin_fmaf(P, __builtin_fmaf(c0.w, P, c0.z), c1.y), sn, cs);
-        const float am = __builtin_fmaf(ks, P, ts);  // tscale (1 + kappa P)
+        const float am = __builtin_fmaf(ks, P, as);  // tscale A (1 + kappa P)
         const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
         fr[B] = __builtin_fmaf(xr, D[2 * h], __builtin_fmaf(-xi, D[2 * h + 1], fr[B]));
         fi[B] = __builtin_fmaf(xr, D[2 * h + 1], __builtin_fmaf(xi, D[2 * h], fi[B]));
@@ -284,7 +314,7 @@ int grid_splits(size_t M, size_t n_tiles) {
 }
 
 hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
-                                 const float2* wc, const float4* wa, const uint2* wa16,
+                                 const float2* wc, const uint4* wh, const uint4* wl,
                                  const float* tscale, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
                                  size_t partial_cap, hipStream_t stream) {
@@ -298,8 +328,330 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
       k_superpose_grid<sg.value, (bool)obl.value><<<grid, kBlock, 0, stream>>>(
-          gt.centre, gt.goff, gt.perm, gt.u, wc, wa, wa16, tscale, gt.n_tiles, chunk, v,
+          gt.centre, gt.goff, gt.perm, gt.u, wc, wh, wl, tscale, gt.n_tiles, chunk, v,
           (uint32_t)M, il, GridBounds{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y},
+          out);
+    });
+  });
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_reduce_partials(S, M, partial, y, stream);
+}
+
+// ---- 16 x 16 tiles on uniform lattices -------------------------------------
+//
+// On a lattice with constant pitches (p_x, p_y) the sources of a 16 x 16 tile
+// sit at gx_a = p_x k_a, gy_b = p_y k_b from its centre, k = i - 7.5 (i < 16).
+// In units of the pitch, P = p_x^2 Pt with Pt = k_a^2 + qx k_a,
+// qx = -2 r_x / p_x (likewise Q, Qt, qy), and the row phase is
+//   alpha P + c2 P^2 = sx k_a + ax k_a^2 + cx Pt^2,
+//   sx = -2 alpha r_x p_x (cycles per pitch), ax = alpha p_x^2, cx = c2 p_x^4.
+// The linear part spans several cycles over a tile, so it is evaluated
+// without rounding: sx = sh + sl with sh keeping 20 significant bits, which
+// makes sh k_a exact in FP32 (2 k_a is an odd integer below 16); its fraction
+// is exact too (v_fract_f32), and only then are sl k_a, the quadratic terms
+// and f added, so the FP32 argument of sin/cos stays below 1.5 cycles however
+// large the tile's phase span.  Per tile and target that leaves the same
+// rounding as the 8 x 8 kernel with a quarter of the per-target constants and
+// half the phasors per pair.
+//
+// T for 16 targets is a 32 x 32 real product (complex 16 x 16), computed as two
+// row blocks rb (source rows 8 rb .. 8 rb + 7) of 16 x 16 x 32 f16 products:
+// lane l = 16 g + i holds row i = 2a' + c (a = 8 rb + a') of the A operand
+// and k-slots e < 8 of column 4g + e/2, component e%2 -- the same slots as
+// the Y operand of target j = l%16.  Per row block four MFMAs accumulate
+// W_hi Y_hi, W_hi Y_lo, W_lo Y_hi and the correction (i k_a 2^-13 W / tscale)
+// (Y Qt kb), kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13; the lane receives T of
+// rows 8 rb + 2g, 8 rb + 2g + 1 for target j.
+
+constexpr int kGrid16Flush = 2;  // tiles summed in FP32 between FP64 flushes
+
+// Accuracy bounds of the 16 x 16 kernel: those of grid_pairs_ok, with the
+// FP32 rounding of the phase argument bounded by its reduced size (below one
+// cycle after the exact fraction, plus f and the quadratic terms).  rx2, ry2:
+// largest gx^2, gy^2.  NaN fails.
+__device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, float ry2,
+                                                float ia, float idf, float kappa) {
+  const float qmax = Pm + Qm;
+  const float round_err = 6e-8f * (2.f + 0.5f * ia * (rx2 + ry2));
+  const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
+  const float beta = kFourPi * 0.125f * ia * idf * idf;
+  const float cross = beta * Pm * Qm;
+  const float ampsep = kappa * kappa * Pm * Qm;
+  return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
+         beta * rx2 * Qm <= 1e-6f && ampsep <= 1e-6f;
+}
+
+// x rounded to 20 significant bits (low 4 bits of the FP32 significand cleared)
+__device__ __forceinline__ float hi20(double x) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, (float)x) & 0xfffffff0u);
+}
+
+// MFMA operands of the 16 x 16 tiles' weights, one wave per tile: for row
+// block rb, wq[(6t + 3rb + q) * 64 + lane] holds, for lane l = 16 g + i,
+// row i = 2a' + c (a = 8 rb + a') and columns 4g .. 4g + 3 (re, im) of the real
+// form of W / tscale split into f16 hi (q = 0) and lo (q = 1), and of
+// i k_a 2^-13 W / tscale (q = 2).  wc: 256 slots per tile, slot 16 a + b.
+__global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restrict__ wc,
+                                                           uint32_t n_tiles,
+                                                           uint4* __restrict__ wq,
+                                                           float* __restrict__ tscale) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t t = blockIdx.x * kGridWaves + (threadIdx.x >> 6);
+  if (t >= n_tiles) return;  // uniform per wave
+  const uint32_t i = lane & 15, g = lane >> 4, ap = i >> 1, co = i & 1;
+  float2 w[2][4];
+  float mx = 0.f;
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      w[rb][q] = wc[(size_t)t * 256 + 16 * (8 * rb + ap) + 4 * g + q];
+      mx = fmaxf(mx, fmaxf(fabsf(w[rb][q].x), fabsf(w[rb][q].y)));
+    }
+  for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  int e = 0;
+  if (mx > 0.f && __builtin_isfinite(mx)) frexpf(mx, &e);
+  e = e < -126 ? -126 : (e > 127 ? 127 : e);
+  const float is = ldexpf(1.f, -e);
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    const float ka = ((float)(8 * rb + ap) - 7.5f) * 0x1p-13f;
+    h8v hi, lo, cr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const float vr = w[rb][q].x * is, vi = w[rb][q].y * is;
+      // real form: rows (a, re) = (Vr, -Vi), (a, im) = (Vi, Vr)
+      const float A0 = co ? vi : vr, A1 = co ? vr : -vi;
+      // i k W: real part -k Wi, imaginary part k Wr
+      const float C0 = co ? ka * vr : -ka * vi, C1 = co ? -ka * vi : -ka * vr;
+      _Float16 h, l;
+      split_f16(A0, h, l);
+      hi[2 * q] = h;
+      lo[2 * q] = l;
+      split_f16(A1, h, l);
+      hi[2 * q + 1] = h;
+      lo[2 * q + 1] = l;
+      cr[2 * q] = (_Float16)C0;
+      cr[2 * q + 1] = (_Float16)C1;
+    }
+    wq[((size_t)t * 6 + 3 * rb + 0) * 64 + lane] = __builtin_bit_cast(uint4, hi);
+    wq[((size_t)t * 6 + 3 * rb + 1) * 64 + lane] = __builtin_bit_cast(uint4, lo);
+    wq[((size_t)t * 6 + 3 * rb + 2) * 64 + lane] = __builtin_bit_cast(uint4, cr);
+  }
+  if (lane == 0) tscale[t] = ldexpf(1.f, e);
+}
+
+hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, uint4* wq, float* tscale,
+                                 hipStream_t stream) {
+  if (!n_tiles) return hipSuccess;
+  const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
+  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, wq, tscale);
+  return hipGetLastError();
+}
+
+template <int SIGN, bool OBL>
+__global__ __launch_bounds__(kBlock) void k_superpose_grid16(
+    const double4* __restrict__ centre, const int32_t* __restrict__ perm,
+    const double* __restrict__ u, const float2* __restrict__ wc,
+    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
+    uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
+    double py, GridBounds gb, double2* __restrict__ out) {
+  // per-(target, tile) constants (all zero when the target takes the exact
+  // path, which makes its X, and so its share of the GEMM, exactly zero):
+  //   k0 = (sx_hi, sx_lo, sy_hi, sy_lo)   linear phase, cycles per pitch
+  //   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
+  //   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
+  //   k3 = (f, A, kb, 0)
+  __shared__ float4 s_k[4][kGridWaves][64];
+  __shared__ double2 s_red[kGridWaves][64][4];
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t g = lane >> 4, j = lane & 15;
+  const uint32_t base = blockIdx.x * kBlock + wv * 64;
+  const uint32_t t0 = blockIdx.y * chunk;
+  const uint32_t t1 = min(n_tiles, t0 + chunk);
+  const uint32_t mo = base + lane;
+  const size_t mm = mo < M ? mo : M - 1;
+  const double vx = v[3 * mm], vy = v[3 * mm + 1], vz = v[3 * mm + 2];
+  // this lane's rows 8 rb + 2g + c and columns 4g + e, as k = index - 7.5
+  float kr[4], kc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    kr[q] = (float)(8 * (q >> 1) + 2 * (int)g + (q & 1)) - 7.5f;
+    kc[q] = (float)(4 * (int)g + q) - 7.5f;
+  }
+  const double px2 = px * px, py2 = py * py;
+  const double ipx = 1.0 / px, ipy = 1.0 / py;
+  double ex_r = 0.0, ex_i = 0.0;
+  double ar[4], ai[4];
+  float fr[4], fi[4];
+#pragma unroll
+  for (int B = 0; B < 4; ++B) {
+    ar[B] = ai[B] = 0.0;
+    fr[B] = fi[B] = 0.f;
+  }
+  const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
+  bool far_all;
+  {
+    const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
+    const float rym = (float)fmax(fabs(vy - gb.cy0), fabs(vy - gb.cy1));
+    const double dz = fabs(vz - gb.z);
+    const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
+    const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
+    const bool ok = grid16_pairs_ok(2.f * rxm * gb.rho_x + rx2, 2.f * rym * gb.rho_y + ry2, rx2,
+                                    ry2, ia, idf, kmax);
+    far_all = __all(ok);
+  }
+  int since = 0;
+  for (uint32_t t = t0; t < t1; ++t) {
+    {
+      const double4 c = centre[t];
+      const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
+      double dref, id;
+      hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      const double al = 0.5 * id * inv_lambda;
+      const double id2 = id * id;
+      const double c2 = -0.25 * al * id2;
+      double kappa;
+      float amp;
+      if constexpr (OBL) {
+        kappa = -id2;
+        amp = (float)(__builtin_fabs(rz) * id2);
+      } else {
+        kappa = -0.5 * id2;
+        amp = (float)id;
+      }
+      bool near = false;
+      if (!far_all)
+        near = !grid16_pairs_ok(2.f * __builtin_fabsf((float)rx) * gb.rho_x + rx2,
+                                2.f * __builtin_fabsf((float)ry) * gb.rho_y + ry2, rx2, ry2,
+                                (float)(id * inv_lambda), (float)id, (float)kappa);
+      const double sx = -2.0 * al * rx * px, sy = -2.0 * al * ry * py;
+      const float sxh = hi20(sx), syh = hi20(sy);
+      float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
+      float4 k1 = make_float4((float)(al * px2), (float)(al * py2), (float)(c2 * px2 * px2),
+                              (float)(c2 * py2 * py2));
+      float4 k2 = make_float4((float)(-2.0 * rx * ipx), (float)(-2.0 * ry * ipy),
+                              (float)(kappa * px2), (float)(kappa * py2));
+      float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp,
+                              (float)((double)SIGN * (4.0 * M_PI) * c2 * (-2.0 * rx) * px * py2 *
+                                      8192.0),
+                              0.f);
+      if (near) {
+        k0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        k1 = k2 = k3 = k0;
+      }
+      s_k[0][wv][lane] = k0;
+      s_k[1][wv][lane] = k1;
+      s_k[2][wv][lane] = k2;
+      s_k[3][wv][lane] = k3;
+      if (near)
+        exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256, wc + (size_t)t * 256, 256, vx,
+                                    vy, vz, inv_lambda, ex_r, ex_i);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    h8v W[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 + q) * 64 + lane]);
+    const float ts = tscale[t];
+#pragma unroll
+    for (int B = 0; B < 4; ++B) {
+      const float4 k0 = s_k[0][wv][16 * B + j];
+      const float4 k1 = s_k[1][wv][16 * B + j];
+      const float4 k2 = s_k[2][wv][16 * B + j];
+      const float4 k3 = s_k[3][wv][16 * B + j];
+      h8v yh, yl, yq;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float k = kc[q], kk = k * k;
+        const float Qt = __builtin_fmaf(k2.y, k, kk);
+        const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
+        float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
+        tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
+        float sn, cs;
+        sincos_rev_f32(r + tq, sn, cs);
+        const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
+        const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
+        const float qk = Qt * k3.z;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          _Float16 hi, lo;
+          split_f16(yv[c], hi, lo);
+          yh[2 * q + c] = hi;
+          yl[2 * q + c] = lo;
+          yq[2 * q + c] = (_Float16)(yv[c] * qk);
+        }
+      }
+      f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
+      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yh, D0, 0, 0, 0);
+      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yh, D1, 0, 0, 0);
+      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yl, D0, 0, 0, 0);
+      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yl, D1, 0, 0, 0);
+      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[1], yh, D0, 0, 0, 0);
+      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[4], yh, D1, 0, 0, 0);
+      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[2], yq, D0, 0, 0, 0);
+      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[5], yq, D1, 0, 0, 0);  // T / (tscale A)
+      const float at = k3.y * ts, akt = at * k2.z;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float k = kr[q], kk = k * k;
+        const float Pt = __builtin_fmaf(k2.x, k, kk);
+        const float r = __builtin_amdgcn_fractf(k0.x * k);  // exact
+        float tp = __builtin_fmaf(k0.y, k, __builtin_fmaf(k1.x, kk, k3.x));
+        tp = __builtin_fmaf(k1.z * Pt, Pt, tp);
+        float sn, cs;
+        sincos_rev_f32(r + tp, sn, cs);
+        const float am = __builtin_fmaf(akt, Pt, at);  // tscale A (1 + kappa P)
+        const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
+        const f4v& D = (q >> 1) ? D1 : D0;
+        const float tr = D[2 * (q & 1)], ti = D[2 * (q & 1) + 1];
+        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
+        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
+      }
+    }
+    if (++since == kGrid16Flush) {
+      since = 0;
+#pragma unroll
+      for (int B = 0; B < 4; ++B) {
+        ar[B] += fr[B];
+        ai[B] += fi[B];
+        fr[B] = fi[B] = 0.f;
+      }
+    }
+  }
+#pragma unroll
+  for (int B = 0; B < 4; ++B) s_red[wv][16 * B + j][g] = make_double2(ar[B] + fr[B], ai[B] + fi[B]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  double sr = ex_r, si = ex_i;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {  // fixed order: deterministic
+    const double2 p = s_red[wv][lane][q];
+    sr += p.x;
+    si += p.y;
+  }
+  if (mo < M) out[(size_t)blockIdx.y * M + mo] = make_double2(sr, si);
+}
+
+hipError_t launch_superpose_grid16(int sign, bool obliquity, const GridTiles& gt,
+                                   const float2* wc, const uint4* wq, const float* tscale,
+                                   const double* v, size_t M, double lambda, double2* y,
+                                   double2* partial, size_t partial_cap, hipStream_t stream) {
+  if (M == 0) return hipSuccess;
+  int S = grid_splits(M, gt.n_tiles);
+  while (S > 1 && (size_t)S * M > partial_cap) --S;
+  const uint32_t chunk = gt.n_tiles ? (gt.n_tiles + S - 1) / S : 0;
+  const dim3 grid((unsigned)((M + kBlock - 1) / kBlock), (unsigned)S);
+  double2* out = S > 1 ? partial : y;
+  const double il = 1.0 / lambda;
+  with_sign(sign, [&](auto sg) {
+    with_bool(obliquity, [&](auto obl) {
+      k_superpose_grid16<sg.value, (bool)obl.value><<<grid, kBlock, 0, stream>>>(
+          gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
+          gt.px, gt.py, GridBounds{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y},
           out);
     });
   });

@@ -43,6 +43,7 @@ struct GridTiles {
   uint32_t n_tiles = 0;
   double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, z = 0;
   float rho_x = 0, rho_y = 0;
+  double px = 0, py = 0;  // pitches of a uniform lattice (16 x 16 tiles)
 };
 
 struct Tuning {
@@ -52,6 +53,7 @@ struct Tuning {
   int target_waves = 16384;
   bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
+  bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
 };
 
 Tuning& tuning();
@@ -110,15 +112,24 @@ hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
 // Lattice kernel (FP32-split on product-grid tiles).  wc: per-slot FP32
 // weights (launch_tile_weights with the grid perm).  launch_grid_weights
 // lays them out as the MFMA operands, divided by tscale[t] (a power of
-// two): wa (float4 per tile and lane) and wa16 (4 x f16 per tile and lane,
-// the cross-term weights i gx W).
+// two): wh and wl, 8 x f16 per tile and lane each (the f16 hi and lo parts
+// of the weights and the cross-term weights i gx W).
 hipError_t launch_grid_weights(const float2* wc, const float4* goff, uint32_t n_tiles,
-                               float4* wa, uint2* wa16, float* tscale, hipStream_t stream);
+                               uint4* wh, uint4* wl, float* tscale, hipStream_t stream);
 hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
-                                 const float2* wc, const float4* wa, const uint2* wa16,
+                                 const float2* wc, const uint4* wh, const uint4* wl,
                                  const float* tscale, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
                                  size_t partial_cap, hipStream_t stream);
+// 16 x 16 tiles of a uniform lattice (pitches gt.px, gt.py; gt.goff unused):
+// wc holds 256 slots per tile (slot 16a + b), launch_grid16_weights lays them
+// out as wq (6 x 8 f16 per tile and lane) and tscale.
+hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, uint4* wq, float* tscale,
+                                 hipStream_t stream);
+hipError_t launch_superpose_grid16(int sign, bool obliquity, const GridTiles& gt,
+                                   const float2* wc, const uint4* wq, const float* tscale,
+                                   const double* v, size_t M, double lambda, double2* y,
+                                   double2* partial, size_t partial_cap, hipStream_t stream);
 // Source splits the lattice kernel uses for M targets over n_tiles tiles.
 int grid_splits(size_t M, size_t n_tiles);
 

@@ -1,5 +1,7 @@
-"""The matrix-core lattice kernel (HOLO_FP32_SPLIT on product-grid sources).
+"""The matrix-core lattice kernels (HOLO_FP32_SPLIT on product-grid sources).
 
+Every test runs twice: with option "grid16" on (uniform lattices whose sides
+are multiples of 16 use 16 x 16 tiles) and off (8 x 8 tiles everywhere).
 Bars: relative L2 <= 1e-6 against the FP64 oracle, as for every FP32-split
 path; the lattice and the tiled kernel agree to the same bar; results do not
 depend on the launch split or on the weights' scale.
@@ -19,6 +21,13 @@ def _announce(request):
     print(f"\n[gpu] {request.node.nodeid}", flush=True)
     sys.stdout.flush()
     yield
+
+
+@pytest.fixture(autouse=True, params=[16, 8], ids=["tiles16", "tiles8"])
+def tiles(request, pkg):
+    pkg.set_option("grid16", 1 if request.param == 16 else 0)
+    yield request.param
+    pkg.set_option("grid16", 1)
 
 
 def rel_l2(a, b):
@@ -155,3 +164,50 @@ def test_lattice_complex_weights_and_adjoint(pkg, ctx, oracle, workloads):
     assert rel_l2(ga, ra) <= 1e-6 and rel_l2(gt, rt) <= 1e-6
     yc = plan.transform_complex(gz, u, -1, LAM, pkg.FP32_SPLIT)
     assert rel_l2(yc, oracle.transform_cartesian(-1, LAM, gz, v, u)) <= 1e-6
+
+
+def test_lattice_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloads, tiles):
+    """A uniform 128 x 96 lattice runs the 16 x 16 kernel when "grid16" is on:
+    both kernels meet the bar and round differently."""
+    u = workloads.grid(128, 96, 10e-6, 0.0)
+    amp, phase = weights(17, len(u))
+    v = targets(18, 500, 0.35, 4e-3)
+    plan = ctx.plan(u)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
+    pkg.set_option("grid16", 1)
+    y16 = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    pkg.set_option("grid16", 0)
+    y8 = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y16, ref) <= 1e-6 and rel_l2(y8, ref) <= 1e-6
+    assert 0 < rel_l2(y16, y8) <= 1e-6
+
+
+@pytest.mark.parametrize("half, z", [(2e-2, 0.1), (5e-2, 0.2), (8e-3, 0.05)])
+def test_lattice_steep_angles(pkg, ctx, oracle, workloads, half, z, tiles):
+    """Targets far off axis: the phase advances by up to several cycles per
+    source pitch, so it spans tens of cycles across a tile (the 16 x 16
+    kernel reduces that linear part exactly)."""
+    u = workloads.grid(64, 48, 10e-6, 0.0)
+    amp, phase = weights(19, len(u))
+    v = targets(20, 400, z, half)
+    y = ctx.plan(u).transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
+
+
+@pytest.mark.parametrize("flip", ["x", "y", "xy"])
+def test_lattice_descending_pitch(pkg, ctx, oracle, workloads, flip, tiles):
+    """Uniform lattices traversed with decreasing x and/or y (negative pitch),
+    off-centre and at z != 0."""
+    u = workloads.grid(64, 32, 8e-6, 0.02).reshape(64, 32, 3)
+    if "x" in flip:
+        u = u[::-1]
+    if "y" in flip:
+        u = u[:, ::-1]
+    u = np.ascontiguousarray(u.reshape(-1, 3)) + np.array([3e-4, -2e-4, 0.0])
+    amp, phase = weights(21, len(u))
+    v = targets(22, 300, 0.3)
+    plan = ctx.plan(u)
+    assert plan.lattice() == (64, 32)
+    y = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT, obliquity=True)
+    ref = oracle.transform_full(+1, LAM, amp, phase, u, v, obliquity=True)
+    assert rel_l2(y, ref) <= 1e-6
