@@ -22,10 +22,11 @@
 //   S = sum_a X_a T_a,   T_a = sum_b W_ab Y_b + sum_b (i gx_a W_ab) (-2 s beta r_x Q_b Y_b)
 // and T for 16 targets at once is a 16 x 16 real product (complex 8 x 8
 // written out) on the matrix cores.  Both operands are split into f16 pairs,
-// x = x_hi + x_lo with x_hi the value rounded to 11 bits, and two
-// v_mfma_f32_16x16x32_f16 accumulate, in FP32,
+// x = x_hi + x_lo (W rounded to nearest, |W_lo| <= 2^-11 |W|; Y rounded
+// toward zero, |Y_lo| < 2^-10 |Y|), and two v_mfma_f32_16x16x32_f16
+// accumulate, in FP32,
 //   W_hi Y_hi + W_hi Y_lo   and   W_lo Y_hi + (i gx W) (correction)
-// leaving out only W_lo Y_lo (below 2^-22 of each term); the correction is
+// leaving out only W_lo Y_lo (below 2^-21 of each term, of either sign); the correction is
 // below 1e-4 of T, so f16 suffices for it.  The weights are divided by a power
 // of two tscale per tile and Y is kept near unit size (Y' = Y / A), so the
 // f16 range holds; X carries tscale and the amplitude A back.
@@ -50,6 +51,7 @@
 namespace holo {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 constexpr int kGridWaves = kBlock / 64;
@@ -92,6 +94,16 @@ __device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo) {
       __builtin_bit_cast(float, (__builtin_bit_cast(uint32_t, x) + 0x1000u) & 0xffffe000u);
   hi = (_Float16)h;
   lo = (_Float16)(x - h);
+}
+
+// The GEMM's Y operand, two values at a time: hi = (a, b) rounded toward zero
+// to f16 by one v_cvt_pkrtz_f16_f32, so the MFMA operand and the lo parts use
+// the same hi; lo = x - hi rounded to f16 (|lo| < 2^-10 |x|, so hi + lo holds
+// x to ~2^-21).  Three instructions for two values.
+__device__ __forceinline__ void split2_f16(float a, float b, h2v& hi, h2v& lo) {
+  hi = __builtin_bit_cast(h2v, __builtin_amdgcn_cvt_pkrtz(a, b));
+  lo[0] = (_Float16)(a - (float)hi[0]);
+  lo[1] = (_Float16)(b - (float)hi[1]);
 }
 
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
@@ -258,15 +270,14 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
         const float am = __builtin_fmaf(c1.x, Q, 1.f);  // 1 + kappa Q
         const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
         const float qk = Q * kb;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          _Float16 hi, lo;
-          split_f16(yv[c], hi, lo);
-          y1[2 * h + c] = hi;
-          y1[4 + 2 * h + c] = lo;
-          y2[2 * h + c] = hi;
-          y2[4 + 2 * h + c] = (_Float16)(yv[c] * qk);
-        }
+        h2v hi, lo;
+        split2_f16(yv[0], yv[1], hi, lo);
+        y1[2 * h] = y2[2 * h] = hi[0];
+        y1[2 * h + 1] = y2[2 * h + 1] = hi[1];
+        y1[4 + 2 * h] = lo[0];
+        y1[5 + 2 * h] = lo[1];
+        y2[4 + 2 * h] = (_Float16)(yv[0] * qk);
+        y2[5 + 2 * h] = (_Float16)(yv[1] * qk);
       }
       f4v D = {0.f, 0.f, 0.f, 0.f};
       D = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wh, y1, D, 0, 0, 0);
@@ -575,14 +586,14 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
         const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
         const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
         const float qk = Qt * k3.z;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          _Float16 hi, lo;
-          split_f16(yv[c], hi, lo);
-          yh[2 * q + c] = hi;
-          yl[2 * q + c] = lo;
-          yq[2 * q + c] = (_Float16)(yv[c] * qk);
-        }
+        h2v hi, lo;
+        split2_f16(yv[0], yv[1], hi, lo);
+        yh[2 * q] = hi[0];
+        yh[2 * q + 1] = hi[1];
+        yl[2 * q] = lo[0];
+        yl[2 * q + 1] = lo[1];
+        yq[2 * q] = (_Float16)(yv[0] * qk);
+        yq[2 * q + 1] = (_Float16)(yv[1] * qk);
       }
       f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
       D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yh, D0, 0, 0, 0);
