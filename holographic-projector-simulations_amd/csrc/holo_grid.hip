@@ -96,14 +96,20 @@ __device__ __forceinline__ void split_f16(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - h);
 }
 
-// The GEMM's Y operand, two values at a time: hi = (a, b) rounded toward zero
-// to f16 by one v_cvt_pkrtz_f16_f32, so the MFMA operand and the lo parts use
-// the same hi; lo = x - hi rounded to f16 (|lo| < 2^-10 |x|, so hi + lo holds
-// x to ~2^-21).  Three instructions for two values.
-__device__ __forceinline__ void split2_f16(float a, float b, h2v& hi, h2v& lo) {
-  hi = __builtin_bit_cast(h2v, __builtin_amdgcn_cvt_pkrtz(a, b));
-  lo[0] = (_Float16)(a - (float)hi[0]);
-  lo[1] = (_Float16)(b - (float)hi[1]);
+// The GEMM's Y operand for one column, Y' = am (c, s) with am = 1 + kappa Q
+// within ~1e-5 of 1: hi = (c, s) rounded toward zero to f16 by one
+// v_cvt_pkrtz_f16_f32, so the MFMA operand and the lo parts use the same hi;
+// lo = am x - hi rounded to f16 (the product fused into v_fma_mix), |lo| <
+// 2^-10 |x| + |am - 1|, so hi + lo holds Y' to ~2^-21.  The correction operand
+// (c, s) (am qk) is one fused conversion each.
+__device__ __forceinline__ void y_operand(float c, float s, float am, float qk, h2v& hi,
+                                          h2v& lo, h2v& q) {
+  hi = __builtin_bit_cast(h2v, __builtin_amdgcn_cvt_pkrtz(c, s));
+  lo[0] = (_Float16)(am * c - (float)hi[0]);
+  lo[1] = (_Float16)(am * s - (float)hi[1]);
+  const float aq = am * qk;
+  q[0] = (_Float16)(c * aq);
+  q[1] = (_Float16)(s * aq);
 }
 
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
@@ -268,16 +274,14 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
         float sn, cs;
         sincos_rev_f32(Q * __builtin_fmaf(c0.w, Q, c0.z), sn, cs);
         const float am = __builtin_fmaf(c1.x, Q, 1.f);  // 1 + kappa Q
-        const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
-        const float qk = Q * kb;
-        h2v hi, lo;
-        split2_f16(yv[0], yv[1], hi, lo);
+        h2v hi, lo, yq;
+        y_operand(cs, SIGN > 0 ? sn : -sn, am, Q * kb, hi, lo, yq);
         y1[2 * h] = y2[2 * h] = hi[0];
         y1[2 * h + 1] = y2[2 * h + 1] = hi[1];
         y1[4 + 2 * h] = lo[0];
         y1[5 + 2 * h] = lo[1];
-        y2[4 + 2 * h] = (_Float16)(yv[0] * qk);
-        y2[5 + 2 * h] = (_Float16)(yv[1] * qk);
+        y2[4 + 2 * h] = yq[0];
+        y2[5 + 2 * h] = yq[1];
       }
       f4v D = {0.f, 0.f, 0.f, 0.f};
       D = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wh, y1, D, 0, 0, 0);
@@ -491,8 +495,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     kr[q] = (float)(8 * (q >> 1) + 2 * (int)g + (q & 1)) - 7.5f;
     kc[q] = (float)(4 * (int)g + q) - 7.5f;
   }
-  const double px2 = px * px, py2 = py * py;
-  const double ipx = 1.0 / px, ipy = 1.0 / py;
+  const float px2f = (float)(px * px), py2f = (float)(py * py);
+  const float px4f = px2f * px2f, py4f = py2f * py2f;
+  const float ipxf = (float)(1.0 / px), ipyf = (float)(1.0 / py);
+  // kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13 = c2 r_x kbf
+  const float kbf = (float)((double)SIGN * -8.0 * M_PI * px * (py * py) * 8192.0);
   double ex_r = 0.0, ex_i = 0.0;
   double ar[4], ai[4];
   float fr[4], fi[4];
@@ -520,34 +527,26 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
       double dref, id;
       hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      // the linear phase needs FP64 (it spans several cycles and is split
+      // exactly); the small coefficients are FP32 products
       const double al = 0.5 * id * inv_lambda;
-      const double id2 = id * id;
-      const double c2 = -0.25 * al * id2;
-      double kappa;
-      float amp;
-      if constexpr (OBL) {
-        kappa = -id2;
-        amp = (float)(__builtin_fabs(rz) * id2);
-      } else {
-        kappa = -0.5 * id2;
-        amp = (float)id;
-      }
-      bool near = false;
-      if (!far_all)
-        near = !grid16_pairs_ok(2.f * __builtin_fabsf((float)rx) * gb.rho_x + rx2,
-                                2.f * __builtin_fabsf((float)ry) * gb.rho_y + ry2, rx2, ry2,
-                                (float)(id * inv_lambda), (float)id, (float)kappa);
       const double sx = -2.0 * al * rx * px, sy = -2.0 * al * ry * py;
       const float sxh = hi20(sx), syh = hi20(sy);
       float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
-      float4 k1 = make_float4((float)(al * px2), (float)(al * py2), (float)(c2 * px2 * px2),
-                              (float)(c2 * py2 * py2));
-      float4 k2 = make_float4((float)(-2.0 * rx * ipx), (float)(-2.0 * ry * ipy),
-                              (float)(kappa * px2), (float)(kappa * py2));
-      float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp,
-                              (float)((double)SIGN * (4.0 * M_PI) * c2 * (-2.0 * rx) * px * py2 *
-                                      8192.0),
-                              0.f);
+      const float idf = (float)id, alf = (float)al;
+      const float rxf = (float)rx, ryf = (float)ry;
+      const float id2f = idf * idf;
+      const float c2f = -0.25f * alf * id2f;
+      const float kf = OBL ? -id2f : -0.5f * id2f;
+      const float amp = OBL ? __builtin_fabsf((float)rz) * id2f : idf;
+      bool near = false;
+      if (!far_all)
+        near = !grid16_pairs_ok(2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
+                                2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2, rx2, ry2, 2.f * alf,
+                                idf, kf);
+      float4 k1 = make_float4(alf * px2f, alf * py2f, c2f * px4f, c2f * py4f);
+      float4 k2 = make_float4(-2.f * rxf * ipxf, -2.f * ryf * ipyf, kf * px2f, kf * py2f);
+      float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp, c2f * rxf * kbf, 0.f);
       if (near) {
         k0 = make_float4(0.f, 0.f, 0.f, 0.f);
         k1 = k2 = k3 = k0;
@@ -584,16 +583,14 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
         float sn, cs;
         sincos_rev_f32(r + tq, sn, cs);
         const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
-        const float yv[2] = {am * cs, SIGN > 0 ? am * sn : -(am * sn)};
-        const float qk = Qt * k3.z;
-        h2v hi, lo;
-        split2_f16(yv[0], yv[1], hi, lo);
+        h2v hi, lo, cq;
+        y_operand(cs, SIGN > 0 ? sn : -sn, am, Qt * k3.z, hi, lo, cq);
         yh[2 * q] = hi[0];
         yh[2 * q + 1] = hi[1];
         yl[2 * q] = lo[0];
         yl[2 * q + 1] = lo[1];
-        yq[2 * q] = (_Float16)(yv[0] * qk);
-        yq[2 * q + 1] = (_Float16)(yv[1] * qk);
+        yq[2 * q] = cq[0];
+        yq[2 * q + 1] = cq[1];
       }
       f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
       D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yh, D0, 0, 0, 0);
