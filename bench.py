@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 METRIC = "phasor pairs/s (FP64 vs FP32-split @rel err≤1e-6) + ms per adjoint grad step"
 FLOP_PER_PAIR = 29            # cuda/macros.h:71-80
 PEAK_FP64 = 78.6e12           # MI355X FP64 vector (and matrix) peak, FLOP/s
+# FP64 FLOPs per pair that k_superpose_f64 executes (SQ_INSTS_VALU_FLOPS_FP64 x 64
+# per pair, profiles/k_superpose_f64_counters.md): the hand-written sqrt and
+# sin/cos cost more than the reference's nominal 29.
+COUNTED_FLOP_PER_PAIR = 69
 
 
 def log(*a):
@@ -281,6 +285,8 @@ def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
                 "traffic": read_traffic("k_superpose_f64"),
                 "kernel": "k_superpose_f64 (+ partial-sum reduce)",
                 "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_launch,
+                "counted_flop_per_pair": COUNTED_FLOP_PER_PAIR,
+                "counted_frac": achieved / FLOP_PER_PAIR * COUNTED_FLOP_PER_PAIR / PEAK_FP64,
                 "avg_launch_ms": 1e3 * avg_launch_s,
                 "median_launch_ms": float(np.median(launch_ms)),
                 "std_launch_ms": float(np.std(launch_ms))}
