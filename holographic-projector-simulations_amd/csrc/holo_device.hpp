@@ -54,16 +54,14 @@ __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
   return p;
 }
 
-// r * (cos x, sin x) accumulated into (ar, ai): hm_sincos_unsigned with the
-// sign folded into r (MATH = 0) or the device libm sincos (MATH = 1).
+// r * (cos x, sin x) accumulated into (ar, ai): hm_sincos with the table at
+// tab (MATH = 0) or the device libm sincos (MATH = 1).
 template <int MATH = 0>
-__device__ __forceinline__ void accumulate_phasor(double r, double x,
-                                                  double& ar, double& ai) {
+__device__ __forceinline__ void accumulate_phasor(double r, double x, double& ar, double& ai,
+                                                  const double* tab = kSinCosTab) {
   double s, c;
   if constexpr (MATH == 0) {
-    unsigned long long m;
-    hm_sincos_unsigned(x, s, c, m);
-    r = hm_apply_sign(r, m);
+    hm_sincos(x, tab, s, c);
   } else {
     sincos(x, &s, &c);
   }

@@ -46,6 +46,12 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     const double* __restrict__ u, uint32_t N, uint32_t chunk,
     const double* __restrict__ v, uint32_t M, double K,
     double2* __restrict__ out) {
+  // the sin/cos table, 4 KB, in LDS (kBlock = 256 threads copy 2 doubles each)
+  static_assert(2 * kBlock == sizeof(kSinCosTab) / sizeof(double), "table copy");
+  __shared__ double s_tab[512];
+  s_tab[threadIdx.x] = kSinCosTab[threadIdx.x];
+  s_tab[threadIdx.x + kBlock] = kSinCosTab[threadIdx.x + kBlock];
+  __syncthreads();
   const uint32_t n0 = blockIdx.y * chunk;
   const uint32_t n1 = min(N, n0 + chunk);
   const uint32_t base = blockIdx.x * (kBlock * R) + threadIdx.x;
@@ -70,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const PairF64 p = pair_f64<SIGN, MATH, OBL>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
-        accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r]);
+        accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r], s_tab);
       }
     } else {
       const double wr = w0[2 * (size_t)n];
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
         const double arg = (SIGN > 0 ? d : -d) * K;
         double s, c;
         if constexpr (MATH == 0) {
-          hm_sincos(arg, s, c);
+          hm_sincos(arg, s_tab, s, c);
         } else {
           sincos(arg, &s, &c);
         }

@@ -1,21 +1,22 @@
 // FP64 sin/cos and sqrt/reciprocal-sqrt for the superposition inner loop.
 //
 // hm_sincos: Cody & Waite argument reduction (W. J. Cody, W. Waite,
-// "Software Manual for the Elementary Functions", 1980) modulo pi with fused
-// multiply-adds and a two-term pi, followed by near-minimax polynomials on
-// [-pi/2, pi/2]:
-//   k = rint(x / pi);  r = fma(-k, PI1, x);  r = fma(-k, PI2, r)
-//   sin r = r + r^3 P(r^2),  cos r = 1 + r^2 Q(r^2)
-//   sin x = (-1)^k sin r,    cos x = (-1)^k cos r
-// PI1 = RN(pi), PI2 = RN(pi - PI1); the neglected third term is 3e-33, so for
-// |x| < 2^50 the reduced argument is within ~1.2e-16 of x - k pi.  P and Q
-// (degree 6 in r^2) are least-squares fits on Chebyshev nodes computed by
-// gen_sincos_coeffs.py; their absolute error is 1.2e-16 (sin) and 2.2e-15
-// (cos) on [-pi/2, pi/2].  Reducing by pi instead of pi/2 leaves a common
-// sign (-1)^k and no sin/cos swap: hm_sincos_unsigned returns that sign as a
-// bit mask for the caller to fold into an amplitude with one xor.  Only
-// multiplies, FMAs, rint and integer ops are used, so host and device builds
-// give identical bits.  Superposition arguments stay below 2^27 rad.
+// "Software Manual for the Elementary Functions", 1980) modulo pi/128 with
+// fused multiply-adds and a two-term pi/128, a table of (sin, cos)(j pi/128)
+// and the angle-addition formulas (the table-driven scheme of P. T. P. Tang,
+// "Table-driven implementation of the exponential function", 1989, applied
+// to sin/cos):
+//   k = rint(x 128/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r);  j = k mod 256
+//   sin x = S_j cos r + C_j sin r,   cos x = C_j cos r - S_j sin r
+// P1 = RN(pi/128), P2 = RN(pi/128 - P1); the neglected third term is below 1e-34,
+// so for |x| < 2^44 the reduced argument is within ~2e-18 of x - k pi/128.
+// |r| <= pi/256, where sin r = r - r^3/6 + r^5/120 and cos r = 1 - r^2/2 +
+// r^4/24 - r^6/720 are within 8.3e-18 and 1.3e-20.  The table
+// (holo_sincos_table.inc, written by gen_sincos_table.py) is correctly
+// rounded, so S_{256-j} = -S_j and C_{256-j} = C_j exactly and sin stays odd
+// and cos even bit for bit.  Only multiplies, FMAs, rint and integer ops are
+// used, so host and device builds give identical bits.  Superposition
+// arguments stay below 2^27 rad.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -23,48 +24,33 @@ namespace holo {
 
 #define HM_HD __host__ __device__ __forceinline__
 
-HM_HD void hm_sincos_unsigned(double x, double& s, double& c,
-                              unsigned long long& sign_mask) {
+constexpr double kSinCosTab[512] = {
+#include "holo_sincos_table.inc"
+};
+
+// sin and cos of x; tab is kSinCosTab or a copy of it (in LDS in the FP64
+// kernel).
+HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
 #pragma clang fp contract(off)
-  constexpr double kOneOverPi = 0x1.45f306dc9c883p-2;
-  constexpr double kPi1 = 0x1.921fb54442d18p+1;
-  constexpr double kPi2 = 0x1.1a62633145c07p-53;
-  constexpr double kMagic = 0x1.8p52;  // 1.5 * 2^52: low word = integer
-  const double k = __builtin_rint(x * kOneOverPi);
-  double r = __builtin_fma(-k, kPi1, x);
-  r = __builtin_fma(-k, kPi2, r);
-  const unsigned lo = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic);
-  sign_mask = (unsigned long long)(lo << 31) << 32;  // bit 63 = parity of k
+  constexpr double k128OverPi = 0x1.45f306dc9c883p+5;
+  constexpr double kP1 = 0x1.921fb54442d18p-6;
+  constexpr double kP2 = 0x1.1a62633145c07p-60;
+  constexpr double kMagic = 0x1.8p52;  // 1.5 * 2^52: low word = k mod 2^32
+  const double k = __builtin_rint(x * k128OverPi);
+  double r = __builtin_fma(-k, kP1, x);
+  r = __builtin_fma(-k, kP2, r);
+  const unsigned j = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 255u;
+  const double S = tab[2 * j], C = tab[2 * j + 1];
   const double u = r * r;
-  double p = -0x1.9e8d5c0fb24cdp-41;
-  p = __builtin_fma(p, u, 0x1.60e1e537cfcadp-33);
-  p = __builtin_fma(p, u, -0x1.ae63326f52860p-26);
-  p = __builtin_fma(p, u, 0x1.71de378613a35p-19);
-  p = __builtin_fma(p, u, -0x1.a01a01987bec7p-13);
-  p = __builtin_fma(p, u, 0x1.1111111106d2cp-7);
-  p = __builtin_fma(p, u, -0x1.5555555555412p-3);
-  s = __builtin_fma(r * u, p, r);
-  double q = -0x1.837194dec695cp-37;
-  q = __builtin_fma(q, u, 0x1.1eadefa1f8550p-29);
-  q = __builtin_fma(q, u, -0x1.27e3f6aae8f33p-22);
-  q = __builtin_fma(q, u, 0x1.a019fd06a0196p-16);
-  q = __builtin_fma(q, u, -0x1.6c16c160f5ed5p-10);
-  q = __builtin_fma(q, u, 0x1.555555553bbacp-5);
-  q = __builtin_fma(q, u, -0x1.ffffffffffafdp-2);
-  c = __builtin_fma(q, u, 1.0);
+  const double sr = __builtin_fma(r * u, __builtin_fma(u, 0x1.1111111111111p-7, -0x1.5555555555555p-3), r);
+  const double cr = __builtin_fma(
+      u, __builtin_fma(u, __builtin_fma(u, -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5), -0.5),
+      1.0);
+  s = __builtin_fma(S, cr, C * sr);
+  c = __builtin_fma(C, cr, -(S * sr));
 }
 
-// Flip the sign of x where bit 63 of mask is set.
-HM_HD double hm_apply_sign(double x, unsigned long long mask) {
-  return __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, x) ^ mask);
-}
-
-HM_HD void hm_sincos(double x, double& s, double& c) {
-  unsigned long long m;
-  hm_sincos_unsigned(x, s, c, m);
-  s = hm_apply_sign(s, m);
-  c = hm_apply_sign(c, m);
-}
+HM_HD void hm_sincos(double x, double& s, double& c) { hm_sincos(x, kSinCosTab, s, c); }
 
 // d = sqrt(s), correctly rounded, and inv = 1/d to about an ulp, for normal
 // s.  Goldschmidt iteration from the hardware reciprocal square root, then

@@ -1,7 +1,9 @@
-"""The FP64 kernel's own sin/cos (Cody-Waite reduction modulo pi + fitted
-polynomials, holo_math.hpp) through its host build: accuracy against mpmath
-and glibc over the superposition's argument range and the reduction's edges.
-The GPU test suite checks that the device produces the same bits."""
+"""The FP64 kernels' own sin/cos (Cody-Waite reduction modulo pi/128, a
+correctly rounded table of (sin, cos)(j pi/128) and short Taylor polynomials,
+holo_math.hpp) through its host build: accuracy against mpmath and glibc over
+the superposition's argument range and the reduction's edges, and exact
+odd/even symmetry.  tests/test_gpu_parity.py::test_fp64_device_sincos_bits
+checks that the device produces the same bits."""
 import math
 
 import mpmath as mp
@@ -16,20 +18,31 @@ def test_sincos_accuracy_random(pkg):
     for x in xs:
         s, c = pkg.host_sincos(x)
         worst = max(worst, abs(s - math.sin(x)), abs(c - math.cos(x)))
-    assert worst <= 2.5e-15  # cos polynomial: 2.2e-15; sin: 1.2e-16
+    assert worst <= 2.5e-16  # measured: 1.1e-16 (one ulp below 1)
 
 
 def test_sincos_reduction_edges_against_mpmath(pkg):
     """Arguments next to multiples of pi/2 (where the reduced value or the
-    result is tiny) and of pi (sign flips), checked at 40 digits."""
+    result is tiny), of pi (sign flips) and of pi/128 (table steps), checked
+    at 40 digits."""
     mp.mp.dps = 40
     for k in (1, 2, 3, 7, 1000, 123457, 2 ** 21 + 3):
-        for base in (k * math.pi / 2, k * math.pi):
+        for base in (k * math.pi / 2, k * math.pi, k * math.pi / 128, (k + 0.5) * math.pi / 128):
             for ulps in (-3, -1, 0, 1, 3):
                 x = base + ulps * math.ulp(base)
                 s, c = pkg.host_sincos(x)
-                assert abs(s - float(mp.sin(mp.mpf(x)))) <= 2.5e-15
-                assert abs(c - float(mp.cos(mp.mpf(x)))) <= 2.5e-15
+                assert abs(s - float(mp.sin(mp.mpf(x)))) <= 2.5e-16
+                assert abs(c - float(mp.cos(mp.mpf(x)))) <= 2.5e-16
+
+
+def test_sincos_odd_even_bitwise(pkg):
+    """sin(-x) = -sin(x) and cos(-x) = cos(x) bit for bit: the FP64 field of
+    Backwards at phase -phi is then exactly the conjugate of Forwards."""
+    g = np.random.default_rng(2)
+    for x in np.concatenate([g.uniform(0, 8, 3000), g.uniform(0, 3e7, 3000)]):
+        s, c = pkg.host_sincos(x)
+        sm, cm = pkg.host_sincos(-x)
+        assert sm == -s and cm == c
 
 
 def test_sincos_known_values(pkg):

@@ -110,6 +110,43 @@ def test_fp64_full_size_subset(pkg, ctx, oracle, workloads, name):
     assert np.all(np.isfinite(y))
 
 
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_fp64_device_sincos_bits(pkg, ctx, sign):
+    """The FP64 kernel's sin/cos on the device gives the same bits as its host
+    build (holo_host_sincos).  One source at the origin and targets at
+    power-of-two distances make d and 1/d exact, so the field is
+    2^-e (cos x, sin x) with x = fma(s d, K, phi) (or s d K for complex
+    weights), computed here exactly with rationals."""
+    from fractions import Fraction
+    dist = [2.0 ** e for e in range(-12, 13)]
+    v = []
+    for d in dist:
+        for ax in range(3):
+            for sg in (1.0, -1.0):
+                p = [0.0, 0.0, 0.0]
+                p[ax] = sg * d
+                v.append(p)
+    v = np.array(v)
+    dv = np.abs(v).sum(axis=1)
+    u = np.zeros((1, 3))
+    n = 0
+    for lam in (632.8e-9, 1.0e-6, 0.4e-6):
+        K = (2 * math.pi) / lam
+        for phi in np.linspace(-7.0, 7.0, 9):
+            y = ctx.transform_full(np.ones(1), np.array([phi]), u, v, sign, lam, pkg.FP64)
+            for m, d in enumerate(dv):
+                x = float(Fraction(sign * d) * Fraction(K) + Fraction(float(phi)))
+                s, c = pkg.host_sincos(x)
+                assert y[m].real == c / d and y[m].imag == s / d, (lam, phi, d)
+                n += 1
+        plan = ctx.plan(u)
+        yc = plan.transform_complex(np.ones(1, dtype=complex), v, sign, lam, pkg.FP64)
+        for m, d in enumerate(dv):
+            s, c = pkg.host_sincos(float(Fraction(sign * d) * Fraction(K)))
+            assert yc[m].real == c / d and yc[m].imag == s / d
+    assert n == 3 * 9 * len(v)
+
+
 def test_fp64_conjugation_symmetry_exact(pkg, ctx):
     amp, phase, u, v = rand_geom(4, 3000, 1000)
     yb = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64)
