@@ -81,6 +81,7 @@ struct holo_ctx {
   bool own_stream = false;
   // staging buffers for host-pointer calls and scratch
   DevBuf amp, phase, u, v, y, w, partial, small, b, pos, ph;
+  DevBuf mc_wc, mc_xy, mc_info;  // product-grid Monte Carlo (McLattice)
 };
 
 struct holo_plan {
@@ -799,6 +800,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid = value != 0;
   } else if (n == "grid16") {
     t.grid16 = value != 0;
+  } else if (n == "mc_lattice") {
+    t.mc_lattice = value != 0;
   } else {
     return fail(HOLO_EINVAL, "unknown option " + n);
   }
@@ -816,6 +819,7 @@ long holo_get_option(const char* name) {
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
+  if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;
   return -1;
 }
 
@@ -880,13 +884,30 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
   const bool fp64 = precision == HOLO_FP64;
   HOLO_TRY(ensure(ctx->pos, N * sizeof(double4), ctx->device));
   HOLO_TRY(ensure(ctx->ph, N * sizeof(double), ctx->device));
+  // per-target FP32-split draws on product-grid sources gather only the
+  // weight; the grid is recognised on the device, without a host round trip
+  const bool lattice = !fp64 && !(flags & HOLO_MC_SHARED) && holo::tuning().mc_lattice;
+  holo::McLattice lat;
+  if (lattice) {
+    HOLO_TRY(ensure(ctx->mc_wc, N * sizeof(float2), ctx->device));
+    HOLO_TRY(ensure(ctx->mc_xy, (holo::kLatticeMaxRow + 1 + N) * sizeof(double), ctx->device));
+    HOLO_TRY(ensure(ctx->mc_info, 2 * sizeof(uint32_t), ctx->device));
+    lat.info = static_cast<const uint32_t*>(ctx->mc_info.ptr);
+    lat.wc = static_cast<const float2*>(ctx->mc_wc.ptr);
+    lat.xy = static_cast<const double*>(ctx->mc_xy.ptr);
+  }
   HOLO_HIP(holo::launch_mc_pack(fp64, da, dp, du, N, static_cast<double4*>(ctx->pos.ptr),
-                                static_cast<double*>(ctx->ph.ptr), ctx->stream));
+                                static_cast<double*>(ctx->ph.ptr),
+                                lattice ? static_cast<float2*>(ctx->mc_wc.ptr) : nullptr,
+                                ctx->stream));
+  if (lattice)
+    HOLO_HIP(holo::launch_mc_lattice_probe(du, N, static_cast<uint32_t*>(ctx->mc_info.ptr),
+                                           static_cast<double*>(ctx->mc_xy.ptr), ctx->stream));
   const double Kw = (2 * M_PI) / lambda;
   HOLO_HIP(holo::launch_mc(sign, fp64, flags & HOLO_MC_SHARED, flags & HOLO_OBLIQUITY,
                            static_cast<const double4*>(ctx->pos.ptr),
                            static_cast<const double*>(ctx->ph.ptr), N, dv, M, K, seed, Kw,
-                           lambda, reinterpret_cast<double2*>(dy), ctx->stream));
+                           lambda, lat, reinterpret_cast<double2*>(dy), ctx->stream));
   return finish_out(ctx, y, dy, 2 * M, dev);
 }
 

@@ -54,6 +54,7 @@ struct Tuning {
   bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
   bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
+  bool mc_lattice = true; // per-target FP32-split MC: table gather on product grids
 };
 
 Tuning& tuning();
@@ -143,14 +144,33 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 // FP32-split: pos = (x,y,z, bitcast float2 weight).
 constexpr unsigned kMcGroup = 1024;  // targets sharing one subset (batched MC)
 
+// Sources on an x-slow product grid (x depends on n / ny only, y on n % ny
+// only, one z), found on the device by launch_mc_lattice_probe:
+//   info[0] = ny (0: not found), info[1] != 0: some source is off the grid;
+//   xy[j] = y of column j (j < ny <= kLatticeMaxRow), xy[kLatticeMaxRow] = z,
+//   xy[kLatticeMaxRow + 1 + i] = x of row i;  wc[n] = the FP32 weight.
+// The per-target FP32-split kernel then gathers 8 bytes per sample instead
+// of a 32-byte record, with bit-identical results.
+constexpr uint32_t kLatticeMaxRow = 65536;
+struct McLattice {
+  const uint32_t* info = nullptr;
+  const float2* wc = nullptr;
+  const double* xy = nullptr;
+};
+
 hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
-                     double2* y, hipStream_t stream);
+                     const McLattice& lat, double2* y, hipStream_t stream);
 
+// wc may be NULL; otherwise it also receives the FP32 weights (FP32-split).
 hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
-                          const double* u, size_t N, double4* pos, double* ph,
+                          const double* u, size_t N, double4* pos, double* ph, float2* wc,
                           hipStream_t stream);
+
+// info: 2 words, xy: kLatticeMaxRow + 1 + N doubles (device memory).
+hipError_t launch_mc_lattice_probe(const double* u, size_t N, uint32_t* info, double* xy,
+                                   hipStream_t stream);
 
 // Per-target indices of targets m0..m0+count-1, or (shared) the subsets of
 // groups m0..m0+count-1.

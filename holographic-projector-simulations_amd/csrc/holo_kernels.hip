@@ -392,11 +392,27 @@ __global__ __launch_bounds__(kBlock) void k_mc(
     const double4* __restrict__ pos, const double* __restrict__ ph,
     uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
     uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
-    double scale, double2* __restrict__ y) {
+    double scale, McLattice lat, double2* __restrict__ y) {
   const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
   if (m >= M) return;
   const double vx = v[3 * (size_t)m], vy = v[3 * (size_t)m + 1],
                vz = v[3 * (size_t)m + 2];
+  // product-grid sources: (x, y, z, w) from the row/column tables and the
+  // weight array, the same values as the packed record
+  uint32_t ny = 0;
+  if constexpr (!FP64) {
+    if (lat.info && !lat.info[1]) ny = lat.info[0];
+  }
+  const double inv_ny = ny ? 1.0 / (double)ny : 0.0;
+  const auto source = [&](uint32_t n) -> double4 {
+    if (!ny) return pos[n];
+    uint32_t i = (uint32_t)((double)n * inv_ny);
+    if (i * ny > n) --i;
+    else if ((i + 1) * ny <= n) ++i;
+    const uint32_t j = n - i * ny;
+    return make_double4(lat.xy[kLatticeMaxRow + 1 + i], lat.xy[j], lat.xy[kLatticeMaxRow],
+                        __builtin_bit_cast(double, lat.wc[n]));
+  };
   double ar = 0.0, ai = 0.0;
   float fr = 0.f, fi = 0.f;
   // 8 samples per step: 4 Philox blocks, then 8 independent gathers in flight
@@ -413,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
     double phv[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
-      p[h] = pos[idx[h]];
+      p[h] = source(idx[h]);
       if constexpr (FP64) phv[h] = ph[idx[h]];
     }
 #pragma unroll
@@ -439,7 +455,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       if (h == 1 && k + 1 >= K) break;
-      const double4 p = pos[idx[h]];
+      const double4 p = source(idx[h]);
       if constexpr (FP64) {
         const PairF64 q = pair_f64<SIGN, 0, OBL>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
         accumulate_phasor<0>(q.r, q.arg, ar, ai);
@@ -517,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_mc_shared(
 hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
-                     double2* y, hipStream_t stream) {
+                     const McLattice& lat, double2* y, hipStream_t stream) {
   if (!M) return hipSuccess;
   const double scale = (double)N / (double)K;
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
@@ -532,7 +548,7 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
               pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
         else
           k_mc<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
-              pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
+              pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, lat, y);
       });
     });
   });
@@ -542,7 +558,7 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
 __global__ __launch_bounds__(kBlock) void k_mc_pack(
     int fp64, const double* __restrict__ amp, const double* __restrict__ phase,
     const double* __restrict__ u, uint32_t N, double4* __restrict__ pos,
-    double* __restrict__ ph) {
+    double* __restrict__ ph, float2* __restrict__ wc) {
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= N) return;
   double4 p;
@@ -557,17 +573,71 @@ __global__ __launch_bounds__(kBlock) void k_mc_pack(
     sincos(phase[n], &sn, &cs);
     const float2 w = make_float2((float)(amp[n] * cs), (float)(amp[n] * sn));
     p.w = __builtin_bit_cast(double, w);
+    if (wc) wc[n] = w;
   }
   pos[n] = p;
 }
 
 hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
-                          const double* u, size_t N, double4* pos, double* ph,
+                          const double* u, size_t N, double4* pos, double* ph, float2* wc,
                           hipStream_t stream) {
   if (!N) return hipSuccess;
   const unsigned blocks = (unsigned)((N + kBlock - 1) / kBlock);
   k_mc_pack<<<blocks, kBlock, 0, stream>>>(fp64 ? 1 : 0, amp, phase, u,
-                                          (uint32_t)N, pos, ph);
+                                          (uint32_t)N, pos, ph, wc);
+  return hipGetLastError();
+}
+
+// Row length of an x-slow product grid: the first n with x_n != x_0, among
+// n < min(N, kLatticeMaxRow); N when all x are equal and N <= kLatticeMaxRow;
+// 0 (not a grid) otherwise or when it does not divide N.  Clears info[1].
+__global__ __launch_bounds__(kBlock) void k_lattice_row(const double* __restrict__ u,
+                                                        uint32_t N, uint32_t* __restrict__ info) {
+  __shared__ uint32_t s_min[kBlock / 64];
+  const double x0 = u[0];
+  const uint32_t lim = min(N, kLatticeMaxRow);
+  uint32_t first = lim;
+  for (uint32_t n = 1 + threadIdx.x; n < lim; n += kBlock)
+    if (u[3 * (size_t)n] != x0) {  // this thread's first hit is its smallest
+      first = n;
+      break;
+    }
+  for (int off = 32; off; off >>= 1) first = min(first, (uint32_t)__shfl_xor((int)first, off, 64));
+  if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = first;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t f = lim;
+    for (int w = 0; w < kBlock / 64; ++w) f = min(f, s_min[w]);
+    uint32_t ny = f < lim ? f : (N <= kLatticeMaxRow ? N : 0u);
+    if (ny && N % ny) ny = 0;
+    info[0] = ny;
+    info[1] = 0;
+  }
+}
+
+// Checks u[n] == (x of row n / ny, y of column n % ny, z_0) exactly, writing
+// 1 to info[1] where it fails (plain stores of the same value), and fills the
+// tables described with McLattice.
+__global__ __launch_bounds__(kBlock) void k_lattice_check(const double* __restrict__ u,
+                                                          uint32_t N, uint32_t* __restrict__ info,
+                                                          double* __restrict__ xy) {
+  const uint32_t ny = info[0];
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  if (!ny || n >= N) return;
+  const uint32_t i = n / ny, j = n - i * ny;
+  const double x = u[3 * (size_t)n], y = u[3 * (size_t)n + 1], z = u[3 * (size_t)n + 2];
+  if (!(x == u[3 * (size_t)i * ny] && y == u[3 * (size_t)j + 1] && z == u[2])) info[1] = 1;
+  if (j == 0) xy[kLatticeMaxRow + 1 + i] = x;
+  if (i == 0) xy[j] = y;
+  if (n == 0) xy[kLatticeMaxRow] = z;
+}
+
+hipError_t launch_mc_lattice_probe(const double* u, size_t N, uint32_t* info, double* xy,
+                                   hipStream_t stream) {
+  if (!N) return hipSuccess;
+  k_lattice_row<<<1, kBlock, 0, stream>>>(u, (uint32_t)N, info);
+  const unsigned blocks = (unsigned)((N + kBlock - 1) / kBlock);
+  k_lattice_check<<<blocks, kBlock, 0, stream>>>(u, (uint32_t)N, info, xy);
   return hipGetLastError();
 }
 

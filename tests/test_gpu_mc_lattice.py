@@ -1,0 +1,105 @@
+"""Per-target FP32-split Monte Carlo on product-grid sources.
+
+When the sources form an x-slow product grid (the reference's plane
+generators, cuda/init.h:89-158), the kernel takes x and y of a drawn source
+from row/column tables and gathers only its 8-byte weight, instead of a
+32-byte record.  The grid is recognised on the device; the pair inputs are
+the same values, so the estimate must equal the generic gather's bit for bit
+(option "mc_lattice" = 0), on and off the grid.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+LAM = 632.8e-9
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print(f"\n[gpu] {request.node.nodeid}", flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def weights(seed, N):
+    g = np.random.default_rng(seed)
+    return g.random(N), g.random(N) * 2 * np.pi
+
+
+def targets(seed, M, z=0.3, half=3e-3):
+    g = np.random.default_rng(seed)
+    v = np.zeros((M, 3))
+    v[:, :2] = (g.random((M, 2)) - 0.5) * 2 * half
+    v[:, 2] = z
+    return v
+
+
+def both(pkg, ctx, amp, phase, u, v, K, sign=+1, obliquity=False):
+    pkg.set_option("mc_lattice", 1)
+    y1 = ctx.transform_mc(amp, phase, u, v, K, 7, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+    pkg.set_option("mc_lattice", 0)
+    y0 = ctx.transform_mc(amp, phase, u, v, K, 7, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+    pkg.set_option("mc_lattice", 1)
+    return y1, y0
+
+
+def product_grid(xs, ys, z):
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], 1)
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (1, 300), (300, 1), (37, 29), (1024, 70)])
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_mc_lattice_bitwise(pkg, ctx, workloads, shape, sign):
+    u = workloads.grid(shape[0], shape[1], 10e-6, 0.0)
+    amp, phase = weights(1, len(u))
+    v = targets(2, 700)
+    y1, y0 = both(pkg, ctx, amp, phase, u, v, 333, sign)
+    assert np.array_equal(y1, y0)
+
+
+def test_mc_lattice_nonuniform_offcentre_obliquity(pkg, ctx):
+    g = np.random.default_rng(3)
+    u = product_grid(np.sort(g.random(40)) * 1e-3 + 2e-4, np.sort(g.random(24)) * 1e-3 - 1e-3, 0.01)
+    amp, phase = weights(4, len(u))
+    v = targets(5, 500, 0.25)
+    y1, y0 = both(pkg, ctx, amp, phase, u, v, 257, obliquity=True)
+    assert np.array_equal(y1, y0)
+
+
+@pytest.mark.parametrize("case", ["random", "one_moved", "one_z", "row_too_long"])
+def test_mc_off_grid_sources(pkg, ctx, workloads, case):
+    """Point sets that are not product grids take the generic gather."""
+    if case == "random":
+        g = np.random.default_rng(6)
+        u = np.zeros((3000, 3))
+        u[:, :2] = (g.random((3000, 2)) - 0.5) * 2e-3
+    elif case == "row_too_long":
+        u = workloads.grid(2, 70000, 1e-6, 0.0)  # rows longer than the device scan
+    else:
+        u = workloads.grid(50, 40, 10e-6, 0.0)
+        if case == "one_moved":
+            u[1234, 1] += 1e-9
+        else:
+            u[999, 2] = 1e-7
+    amp, phase = weights(7, len(u))
+    v = targets(8, 300)
+    y1, y0 = both(pkg, ctx, amp, phase, u, v, 129)
+    assert np.array_equal(y1, y0)
+
+
+def test_mc_lattice_matches_oracle(pkg, ctx, oracle, workloads):
+    u = workloads.grid(96, 80, 10e-6, 0.0)
+    amp, phase = weights(9, len(u))
+    v = targets(10, 2000)
+    y = ctx.transform_mc(amp, phase, u, v, 513, 11, +1, LAM, pkg.FP32_SPLIT)
+    sub = np.arange(0, 2000, 9)
+    ref = oracle.transform_mc(+1, LAM, amp, phase, u, v, 513, 11, targets=sub)
+    assert rel_l2(y[sub], ref) <= 1e-6
