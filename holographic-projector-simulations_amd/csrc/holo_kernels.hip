@@ -46,11 +46,12 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     const double* __restrict__ u, uint32_t N, uint32_t chunk,
     const double* __restrict__ v, uint32_t M, double K,
     double2* __restrict__ out) {
-  // the sin/cos table, 4 KB, in LDS (kBlock = 256 threads copy 2 doubles each)
-  static_assert(2 * kBlock == sizeof(kSinCosTab) / sizeof(double), "table copy");
-  __shared__ double s_tab[512];
-  s_tab[threadIdx.x] = kSinCosTab[threadIdx.x];
-  s_tab[threadIdx.x + kBlock] = kSinCosTab[threadIdx.x + kBlock];
+  // the sin/cos table, 32 KB, in LDS (5 workgroups per CU, as the VGPRs allow)
+  constexpr int kTab = sizeof(kSinCosTab) / sizeof(double);
+  static_assert(kTab % kBlock == 0, "table copy");
+  __shared__ double s_tab[kTab];
+#pragma unroll
+  for (int q = 0; q < kTab / kBlock; ++q) s_tab[threadIdx.x + q * kBlock] = kSinCosTab[threadIdx.x + q * kBlock];
   __syncthreads();
   const uint32_t n0 = blockIdx.y * chunk;
   const uint32_t n1 = min(N, n0 + chunk);

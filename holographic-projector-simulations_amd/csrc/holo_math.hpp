@@ -1,20 +1,20 @@
 // FP64 sin/cos and sqrt/reciprocal-sqrt for the superposition inner loop.
 //
 // hm_sincos: Cody & Waite argument reduction (W. J. Cody, W. Waite,
-// "Software Manual for the Elementary Functions", 1980) modulo pi/128 with
-// fused multiply-adds and a two-term pi/128, a table of (sin, cos)(j pi/128)
+// "Software Manual for the Elementary Functions", 1980) modulo pi/1024 with
+// fused multiply-adds and a two-term pi/1024, a table of (sin, cos)(j pi/1024)
 // and the angle-addition formulas (the table-driven scheme of P. T. P. Tang,
 // "Table-driven implementation of the exponential function", 1989, applied
 // to sin/cos):
-//   k = rint(x 128/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r);  j = k mod 256
+//   k = rint(x 1024/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r);  j = k mod 2048
 //   sin x = S_j cos r + C_j sin r,   cos x = C_j cos r - S_j sin r
-// P1 = RN(pi/128), P2 = RN(pi/128 - P1); the neglected third term is below 1e-34,
-// so for |x| < 2^44 the reduced argument is within ~2e-18 of x - k pi/128.
-// |r| <= pi/256, where sin r = r - r^3/6 + r^5/120 and cos r = 1 - r^2/2 +
-// r^4/24 - r^6/720 are within 8.3e-18 and 1.3e-20.  The table
-// (holo_sincos_table.inc, written by gen_sincos_table.py) is correctly
-// rounded, so S_{256-j} = -S_j and C_{256-j} = C_j exactly and sin stays odd
-// and cos even bit for bit.  Only multiplies, FMAs, rint and integer ops are
+// P1 = RN(pi/1024), P2 = RN(pi/1024 - P1); the neglected third term is below
+// 2e-35, so for |x| < 2^40 the reduced argument is within ~2e-19 of
+// x - k pi/1024.  |r| <= pi/2048, where sin r = r - r^3/6 and cos r = 1 -
+// r^2/2 + r^4/24 are within 7.1e-17 and 1.8e-20.  The table
+// (holo_sincos_table.inc, 32 KB, written by gen_sincos_table.py) is correctly
+// rounded, so S_{2048-j} = -S_j and C_{2048-j} = C_j exactly and sin stays
+// odd and cos even bit for bit.  Only multiplies, FMAs, rint and integer ops are
 // used, so host and device builds give identical bits.  Superposition
 // arguments stay below 2^27 rad.
 #pragma once
@@ -24,7 +24,7 @@ namespace holo {
 
 #define HM_HD __host__ __device__ __forceinline__
 
-constexpr double kSinCosTab[512] = {
+constexpr double kSinCosTab[4096] = {
 #include "holo_sincos_table.inc"
 };
 
@@ -32,20 +32,18 @@ constexpr double kSinCosTab[512] = {
 // kernel).
 HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
 #pragma clang fp contract(off)
-  constexpr double k128OverPi = 0x1.45f306dc9c883p+5;
-  constexpr double kP1 = 0x1.921fb54442d18p-6;
-  constexpr double kP2 = 0x1.1a62633145c07p-60;
+  constexpr double kStepsOverPi = 0x1.45f306dc9c883p+8;  // 1024 / pi
+  constexpr double kP1 = 0x1.921fb54442d18p-9;
+  constexpr double kP2 = 0x1.1a62633145c07p-63;
   constexpr double kMagic = 0x1.8p52;  // 1.5 * 2^52: low word = k mod 2^32
-  const double k = __builtin_rint(x * k128OverPi);
+  const double k = __builtin_rint(x * kStepsOverPi);
   double r = __builtin_fma(-k, kP1, x);
   r = __builtin_fma(-k, kP2, r);
-  const unsigned j = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 255u;
+  const unsigned j = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 2047u;
   const double S = tab[2 * j], C = tab[2 * j + 1];
   const double u = r * r;
-  const double sr = __builtin_fma(r * u, __builtin_fma(u, 0x1.1111111111111p-7, -0x1.5555555555555p-3), r);
-  const double cr = __builtin_fma(
-      u, __builtin_fma(u, __builtin_fma(u, -0x1.6c16c16c16c17p-10, 0x1.5555555555555p-5), -0.5),
-      1.0);
+  const double sr = __builtin_fma(r * u, -0x1.5555555555555p-3, r);
+  const double cr = __builtin_fma(u, __builtin_fma(u, 0x1.5555555555555p-5, -0.5), 1.0);
   s = __builtin_fma(S, cr, C * sr);
   c = __builtin_fma(C, cr, -(S * sr));
 }

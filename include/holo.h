@@ -211,8 +211,8 @@ holo_status holo_read_dat(const char *path, double *buf, size_t capacity,
 holo_status holo_write_dat(const char *path, const double *buf, size_t count);
 
 /* ---- host builds of the device math (verification only) -----------------
- * The FP64 kernels' sin/cos (Cody-Waite reduction modulo pi/128, a table of
- * (sin, cos)(j pi/128) and short polynomials)
+ * The FP64 kernels' sin/cos (Cody-Waite reduction modulo pi/1024, a table of
+ * (sin, cos)(j pi/1024) and short polynomials)
  * and sqrt / reciprocal, compiled for the host from the same source; the
  * sin/cos bits match the device's.  No reference counterpart. */
 void holo_host_sincos(double x, double *s, double *c);
