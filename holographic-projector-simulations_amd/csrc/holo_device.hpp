@@ -139,7 +139,7 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
                                              double inv_lambda, float& fr, float& fi) {
   const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
   double d, id;
-  hm_sqrt_inv_fast(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
   const float f = (float)frac_cycle(d * inv_lambda);
   float sn, cs;
   sincos_rev_f32(f, sn, cs);

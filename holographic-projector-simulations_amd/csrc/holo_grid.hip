@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid(
     {
       const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
       double dref, id;
-      hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
       const double al = 0.5 * id * inv_lambda;
       const double id2 = id * id;
       const float c2 = (float)(-0.25 * al * id2);
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       const double4 c = centre[t];
       const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
       double dref, id;
-      hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
       // the linear phase needs FP64 (it spans several cycles and is split
       // exactly); the small coefficients are FP32 products
       const double al = 0.5 * id * inv_lambda;

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     for (int r = 0; r < R; ++r) {
       const double rx = vx[r] - c.x, ry = vy[r] - c.y, rz = vz[r] - c.z;
       double dref, id;
-      hm_sqrt_inv_fast(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
+      hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
       {
         const float rxy = __builtin_sqrtf((float)__builtin_fma(ry, ry, rx * rx));
         const float qmax = 2.f * (rxy * rho.x + __builtin_fabsf((float)rz) * rho.y) +

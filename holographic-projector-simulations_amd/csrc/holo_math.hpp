@@ -50,41 +50,17 @@ HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
 
 HM_HD void hm_sincos(double x, double& s, double& c) { hm_sincos(x, kSinCosTab, s, c); }
 
-// d = sqrt(s), correctly rounded, and inv = 1/d to about an ulp, for normal
-// s.  Goldschmidt iteration from the hardware reciprocal square root, then
-// one Newton correction of d using h ~ 1/(2 sqrt s) (Markstein's final
-// step), then one Newton step of the reciprocal 1/d from 2h:
-// inv = 2h + 2h (1 - 2 d h).  On gfx950 a second correction changed d on
-// none of 4.3e9 random s (in [0.1, 0.2], [1e-6, 1e-2], [0.5, 4] and
-// [1e-12, 1e3]), and the two-correction d matched the host's IEEE sqrt on
-// 1.7e7 random s in [1e-8, 20]; inv is used only for amplitudes.  On the
-// host the seed comes from 1/sqrt(s).
+// d = sqrt(s), correctly rounded, and inv ~ 1/d, for normal s.  One
+// Goldschmidt iteration from the hardware reciprocal square root gives
+// g ~ sqrt(s) and h ~ 1/(2 sqrt s); one Newton correction of d with h
+// (Markstein's final step) rounds d correctly, and inv = 2h.  On gfx950 a
+// second correction changed d on none of 4.3e9 random s (in [0.1, 0.2],
+// [1e-6, 1e-2], [0.5, 4] and [1e-12, 1e3]), and the two-correction d
+// matched the host's IEEE sqrt on 1.7e7 random s in [1e-8, 20].  inv scales
+// amplitudes only; its error against 1/d is bounded on the device by
+// tests/test_gpu_parity.py::test_fp64_inverse_distance_accuracy.  On the host
+// the seed comes from 1/sqrt(s).
 HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
-#pragma clang fp contract(off)
-#if defined(__HIP_DEVICE_COMPILE__)
-  const double y = __builtin_amdgcn_rsq(s);
-#else
-  const double y = 1.0 / __builtin_sqrt(s);
-#endif
-  double g = s * y;
-  double h = 0.5 * y;
-  double e = __builtin_fma(-g, h, 0.5);
-  g = __builtin_fma(g, e, g);
-  h = __builtin_fma(h, e, h);
-  const double rr = __builtin_fma(-g, g, s);
-  g = __builtin_fma(rr, h, g);
-  const double h2 = h + h;
-  e = __builtin_fma(-g, h2, 1.0);
-  d = g;
-  inv = __builtin_fma(h2, e, h2);
-}
-
-// d = sqrt(s) to within an ulp and inv = 1/d to ~1e-13 relative (the
-// Goldschmidt step and one Newton correction of hm_sqrt_inv, without the
-// final correction and the reciprocal refinement).  For the FP32-split
-// kernels, which need d only for the fractional cycle d / lambda (an ulp of
-// d is ~1e-11 cycles at 0.35 m) and 1/d only in FP32.
-HM_HD void hm_sqrt_inv_fast(double s, double& d, double& inv) {
 #pragma clang fp contract(off)
 #if defined(__HIP_DEVICE_COMPILE__)
   const double y = __builtin_amdgcn_rsq(s);
