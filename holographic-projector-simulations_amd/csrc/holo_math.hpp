@@ -35,12 +35,15 @@ HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
   constexpr double kStepsOverPi = 0x1.45f306dc9c883p+8;  // 1024 / pi
   constexpr double kP1 = 0x1.921fb54442d18p-9;
   constexpr double kP2 = 0x1.1a62633145c07p-63;
-  constexpr double kMagic = 0x1.8p52;  // 1.5 * 2^52: low word = k mod 2^32
+  // k + 1.5 * 2^48 has ulp 1/16, so its low word is 16 k mod 2^32: the byte
+  // offset of entry k mod 2048 is one AND away (|k| < 2^47)
+  constexpr double kMagic = 0x1.8p48;
   const double k = __builtin_rint(x * kStepsOverPi);
   double r = __builtin_fma(-k, kP1, x);
   r = __builtin_fma(-k, kP2, r);
-  const unsigned j = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 2047u;
-  const double S = tab[2 * j], C = tab[2 * j + 1];
+  const unsigned off = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 0x7ff0u;
+  const double* e = reinterpret_cast<const double*>(reinterpret_cast<const char*>(tab) + off);
+  const double S = e[0], C = e[1];
   const double u = r * r;
   const double sr = __builtin_fma(r * u, -0x1.5555555555555p-3, r);
   const double cr = __builtin_fma(u, __builtin_fma(u, 0x1.5555555555555p-5, -0.5), 1.0);
