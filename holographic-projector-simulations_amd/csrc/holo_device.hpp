@@ -25,7 +25,7 @@ struct PairF64 {
   double arg;  // phase argument
 };
 
-// MATH = 0: hm_sqrt_inv (d correctly rounded, 1/d to ~1 ulp); MATH = 1: the
+// MATH = 0: hm_sqrt_hinv (d correctly rounded, 1/d within 3.7e-15); MATH = 1: the
 // compiler's sqrt and IEEE division.  Both give the same d and arg bits.
 // OBL: amplitude times the obliquity factor |dz| / d
 // (matlab/holofunctions.m:226-227), r = (a / d) (|dz| / d).
@@ -40,11 +40,11 @@ __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
   const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
   PairF64 p;
   if constexpr (MATH == 0) {
-    double d, inv;
-    hm_sqrt_inv(s, d, inv);
+    double d, h;  // h = (1/d) / 2
+    hm_sqrt_hinv(s, d, h);
     p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);
-    p.r = a * inv;
-    if constexpr (OBL) p.r *= __builtin_fabs(dz) * inv;
+    p.r = (a + a) * h;  // = a (h + h) exactly; a + a is shared by a lane's targets
+    if constexpr (OBL) p.r *= __builtin_fabs(dz) * (h + h);
   } else {
     const double d = __builtin_sqrt(s);
     p.arg = __builtin_fma(SIGN > 0 ? d : -d, K, phi);

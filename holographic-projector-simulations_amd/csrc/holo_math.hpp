@@ -62,8 +62,9 @@ HM_HD void hm_sincos(double x, double& s, double& c) { hm_sincos(x, kSinCosTab, 
 // matched the host's IEEE sqrt on 1.7e7 random s in [1e-8, 20].  inv scales
 // amplitudes only; its error against 1/d is bounded on the device by
 // tests/test_gpu_parity.py::test_fp64_inverse_distance_accuracy.  On the host
-// the seed comes from 1/sqrt(s).
-HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
+// the seed comes from 1/sqrt(s).  hm_sqrt_hinv returns h = inv / 2 (exactly)
+// for callers that fold the factor 2 into a value they compute once.
+HM_HD void hm_sqrt_hinv(double s, double& d, double& h) {
 #pragma clang fp contract(off)
 #if defined(__HIP_DEVICE_COMPILE__)
   const double y = __builtin_amdgcn_rsq(s);
@@ -71,12 +72,17 @@ HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
   const double y = 1.0 / __builtin_sqrt(s);
 #endif
   double g = s * y;
-  double h = 0.5 * y;
+  h = 0.5 * y;
   const double e = __builtin_fma(-g, h, 0.5);
   g = __builtin_fma(g, e, g);
   h = __builtin_fma(h, e, h);
   const double rr = __builtin_fma(-g, g, s);
   d = __builtin_fma(rr, h, g);
+}
+
+HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
+  double h;
+  hm_sqrt_hinv(s, d, h);
   inv = h + h;
 }
 
