@@ -684,7 +684,8 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     }
     const size_t slots = p->n_tiles * p->tile;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->perm.ptr),
-                                       slots, static_cast<float2*>(p->wc.ptr), ctx->stream));
+                                       slots, static_cast<float2*>(p->wc.ptr), ctx->stream,
+                                       true));
     holo::TileSet ts;
     ts.centre = static_cast<const double4*>(p->centre.ptr);
     ts.geo = static_cast<const float4*>(p->geo.ptr);

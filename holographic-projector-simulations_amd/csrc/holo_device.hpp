@@ -160,7 +160,8 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
 // sin/cos and weights.
 constexpr float kGuardCycles = 2e-7f;
 
-template <int SIGN, bool OBL>
+// POLAR_W: w holds (|w|, arg w in cycles) rather than (re, im).
+template <int SIGN, bool OBL, bool POLAR_W = false>
 __device__ __forceinline__ void exact_tile_pairs(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
     const float2* __restrict__ w, int T, double vx, double vy, double vz,
@@ -169,8 +170,14 @@ __device__ __forceinline__ void exact_tile_pairs(
   for (int j = 0; j < T; ++j) {
     const int32_t n = perm[j];
     if (n < 0) continue;
+    float2 wj = w[j];
+    if constexpr (POLAR_W) {
+      float sn, cs;
+      sincos_rev_f32(wj.y, sn, cs);
+      wj = make_float2(wj.x * cs, wj.x * sn);
+    }
     const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
-                                   __builtin_bit_cast(double, w[j]));
+                                   __builtin_bit_cast(double, wj));
     mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
   }
   ar += fr;

@@ -104,7 +104,8 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const d
 hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, double2* y,
                                   hipStream_t stream);
 
-// FP32-split: weights wc (float2 per slot), tiles as above.
+// FP32-split: weights wc (float2 per slot, polar: launch_tile_weights with
+// polar_out), tiles as above.
 hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
                                  const float2* wc, const double* v, size_t M,
                                  double lambda, double2* y, double2* partial,
@@ -134,10 +135,12 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, const GridTiles& gt
 // Source splits the lattice kernel uses for M targets over n_tiles tiles.
 int grid_splits(size_t M, size_t n_tiles);
 
-// wc[slot] = a e^{i phi} (polar) or w (cartesian) of source perm[slot].
+// wc[slot] = a e^{i phi} (polar input) or w (cartesian input) of source
+// perm[slot] as (re, im); with polar_out as (|w|, arg w / 2 pi in [-1/2, 1/2)).
 hipError_t launch_tile_weights(WeightForm wf, const double* w0,
                                const double* w1, const int32_t* perm,
-                               size_t n_slots, float2* wc, hipStream_t stream);
+                               size_t n_slots, float2* wc, hipStream_t stream,
+                               bool polar_out = false);
 
 // Monte Carlo: y[m] = (N/K) sum_k pair(n_{m,k}, m).
 // FP64: pos = (x,y,z,amp) per source, ph = phase per source.

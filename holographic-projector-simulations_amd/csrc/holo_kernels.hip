@@ -263,20 +263,16 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
         float q = __builtin_fmaf(m2ry[r], gj.y, gj.w);
         if constexpr (!PLANAR) q = __builtin_fmaf(m2rz[r], gj.z, q);
         q = __builtin_fmaf(m2rx[r], gj.x, q);
-        const float tt = q * __builtin_fmaf(c2[r], q, alpha[r]);
+        // w e^{i s 2 pi t} = |w| e^{2 pi i (arg w + s t)}: the weight's phase
+        // joins the pair's, its modulus the amplitude
+        const float pc = __builtin_fmaf(c2[r], q, alpha[r]);
+        const float tt = SIGN > 0 ? __builtin_fmaf(q, pc, wj.y) : __builtin_fmaf(-q, pc, wj.y);
         float sn, cs;
         sincos_rev_f32(tt, sn, cs);
-        float idd = __builtin_fmaf(q, gq[r], invd[r]);
-        if constexpr (OBL && !PLANAR) idd *= __builtin_fabsf(rzf[r] - gj.z);
-        const float pr = idd * cs;
-        const float pi = idd * sn;
-        if constexpr (SIGN > 0) {
-          sr[r] = __builtin_fmaf(wj.x, pr, __builtin_fmaf(-wj.y, pi, sr[r]));
-          si[r] = __builtin_fmaf(wj.x, pi, __builtin_fmaf(wj.y, pr, si[r]));
-        } else {
-          sr[r] = __builtin_fmaf(wj.x, pr, __builtin_fmaf(wj.y, pi, sr[r]));
-          si[r] = __builtin_fmaf(-wj.x, pi, __builtin_fmaf(wj.y, pr, si[r]));
-        }
+        float amp = wj.x * __builtin_fmaf(q, gq[r], invd[r]);
+        if constexpr (OBL && !PLANAR) amp *= __builtin_fabsf(rzf[r] - gj.z);
+        sr[r] = __builtin_fmaf(amp, cs, sr[r]);
+        si[r] = __builtin_fmaf(amp, sn, si[r]);
       }
     }
 #pragma unroll
@@ -290,8 +286,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (near[r])
-          exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r],
-                                      inv_lambda, ar[r], ai[r]);
+          exact_tile_pairs<SIGN, OBL, true>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r],
+                                            inv_lambda, ar[r], ai[r]);
     }
   }
   double2* dst = out + (size_t)blockIdx.y * M;
@@ -349,13 +345,26 @@ hipError_t launch_superpose_f32s(int sign, bool obliquity, const TileSet& ts,
 __global__ __launch_bounds__(kBlock) void k_tile_weights(
     int wf, const double* __restrict__ w0, const double* __restrict__ w1,
     const int32_t* __restrict__ perm, uint32_t n_slots,
-    float2* __restrict__ wc) {
+    float2* __restrict__ wc, int polar_out) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n_slots) return;
   const int32_t n = perm[s];
   float2 r = make_float2(0.f, 0.f);
   if (n >= 0) {
-    if (wf == kPolar) {
+    constexpr double kInv2Pi = 0.15915494309189535;
+    if (polar_out) {  // (|w|, arg w in cycles, reduced to [-1/2, 1/2))
+      double a, t;
+      if (wf == kPolar) {
+        a = w0[n];
+        t = w1[n] * kInv2Pi + (a < 0.0 ? 0.5 : 0.0);
+        a = __builtin_fabs(a);
+      } else {
+        const double re = w0[2 * (size_t)n], im = w0[2 * (size_t)n + 1];
+        a = hypot(re, im);
+        t = atan2(im, re) * kInv2Pi;
+      }
+      r = make_float2((float)a, (float)(t - __builtin_rint(t)));
+    } else if (wf == kPolar) {
       double sn, cs;
       sincos(w1[n], &sn, &cs);
       r = make_float2((float)(w0[n] * cs), (float)(w0[n] * sn));
@@ -369,11 +378,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_weights(
 hipError_t launch_tile_weights(WeightForm wf, const double* w0,
                                const double* w1, const int32_t* perm,
                                size_t n_slots, float2* wc,
-                               hipStream_t stream) {
+                               hipStream_t stream, bool polar_out) {
   if (!n_slots) return hipSuccess;
   const unsigned blocks = (unsigned)((n_slots + kBlock - 1) / kBlock);
   k_tile_weights<<<blocks, kBlock, 0, stream>>>((int)wf, w0, w1, perm,
-                                               (uint32_t)n_slots, wc);
+                                               (uint32_t)n_slots, wc, polar_out ? 1 : 0);
   return hipGetLastError();
 }
 
