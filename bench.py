@@ -108,10 +108,11 @@ def main():
                     help="comma list of chain, fp32, mc, adjoint (profiling runs pick one)")
     ap.add_argument("--hologram-steps", type=int, default=3,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
-    ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16), default=16,
-                    help="tile edge of the FP32-split lattice kernel on uniform lattices "
-                         "(option grid16; 8 selects the 8 x 8 kernel, 0 the tiled kernel "
-                         "for arbitrary point sets, option grid = 0)")
+    ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32), default=32,
+                    help="tile columns of the FP32-split lattice kernel on uniform lattices "
+                         "(32: 16 x 32 tiles, options grid16 and grid32; 16: 16 x 16 tiles, "
+                         "grid32 = 0; 8: the 8 x 8 kernel, grid16 = 0; 0: the tiled kernel "
+                         "for arbitrary point sets, grid = 0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration)")
     a = ap.parse_args()
@@ -125,7 +126,8 @@ def main():
     import torch
     import __graft_entry__
     holo = __graft_entry__.load_package()
-    holo.set_option("grid16", 1 if a.lattice_tiles == 16 else 0)
+    holo.set_option("grid16", 1 if a.lattice_tiles in (16, 32) else 0)
+    holo.set_option("grid32", 1 if a.lattice_tiles == 32 else 0)
     holo.set_option("grid", 0 if a.lattice_tiles == 0 else 1)
     import importlib
     wl = importlib.import_module("holo_amd.workloads")
@@ -192,7 +194,7 @@ def main():
             out["fp32_split"] = {
                 "workload": "1024^2 -> 1024^2 (z 0 -> 0.35 m, 10 / 7.56 um, amp U[0,1), "
                             "phase U[0,2pi)), " + (
-                                f"lattice kernel, {a.lattice_tiles}x{a.lattice_tiles} tiles"
+                                f"lattice kernel, {min(a.lattice_tiles, 16)}x{a.lattice_tiles} tiles"
                                 if a.lattice_tiles else "tiled kernel, tiles of 64"),
                 "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
                 "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,

@@ -110,6 +110,12 @@ struct holo_plan {
   double g16_cx0 = 0, g16_cx1 = 0, g16_cy0 = 0, g16_cy1 = 0;
   float g16_rho_x = 0, g16_rho_y = 0;
   DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_ts;
+  // the 16 x 32 tiles formed by pairs of 16 x 16 tiles (16x16 tiles 2T, 2T+1)
+  // when ny is a multiple of 32 (g32_tiles = 0 otherwise)
+  size_t g32_tiles = 0;
+  double g32_cx0 = 0, g32_cx1 = 0, g32_cy0 = 0, g32_cy1 = 0;
+  float g32_rho_y = 0;
+  DevBuf g32_centre;
 };
 
 namespace {
@@ -367,7 +373,9 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
 // both are multiples of 16 and the positions are x_0 + i p_x, y_0 + j p_y
 // (to 1e-9 of a pitch), it is also cut into 16 x 16 tiles t = I*(ny/16) + J,
 // slot 16a + b = source (16I+a, 16J+b), centred at the lattice points
-// x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.
+// x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.  When ny is a multiple of 32,
+// tiles 2T and 2T + 1 (the same rows, columns 32J' .. 32J' + 31) also form
+// the 16 x 32 tile T, centred at x_0 + (16I + 7.5) p_x, y_0 + (32J' + 15.5) p_y.
 struct HostGrid {
   size_t nx = 0, ny = 0;
   double cx0 = INFINITY, cx1 = -INFINITY, cy0 = INFINITY, cy1 = -INFINITY, z = 0;
@@ -380,6 +388,9 @@ struct HostGrid {
   float rho16_x = 0, rho16_y = 0;
   std::vector<double4> centre16;
   std::vector<int32_t> perm16;
+  double cx0_32 = INFINITY, cx1_32 = -INFINITY, cy0_32 = INFINITY, cy1_32 = -INFINITY;
+  float rho32_y = 0;
+  std::vector<double4> centre32;
 };
 
 void build_grid16(const double* u, HostGrid& g) {
@@ -413,6 +424,20 @@ void build_grid16(const double* u, HostGrid& g) {
       for (int a = 0; a < 16; ++a)
         for (int b = 0; b < 16; ++b)
           g.perm16.push_back((int32_t)((16 * I + a) * ny + 16 * J + b));
+    }
+  if (TJ % 2) return;
+  g.rho32_y = std::nextafter((float)(15.5 * std::fabs(py)), INFINITY);
+  const float2 rho32{g.rho16_x, g.rho32_y};
+  g.centre32.reserve(TI * TJ / 2);
+  for (size_t I = 0; I < TI; ++I)
+    for (size_t J = 0; J < TJ / 2; ++J) {
+      const double cx = x0 + ((double)(16 * I) + 7.5) * px;
+      const double cy = y0 + ((double)(32 * J) + 15.5) * py;
+      g.centre32.push_back(double4{cx, cy, g.z, __builtin_bit_cast(double, rho32)});
+      g.cx0_32 = std::min(g.cx0_32, cx);
+      g.cx1_32 = std::max(g.cx1_32, cx);
+      g.cy0_32 = std::min(g.cy0_32, cy);
+      g.cy1_32 = std::max(g.cy1_32, cy);
     }
 }
 
@@ -558,6 +583,18 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
                               hipMemcpyHostToDevice, ctx->stream));
       HOLO_HIP(hipMemcpyAsync(p->g16_perm.ptr, grid.perm16.data(),
                               256 * n16 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+      const size_t n32 = grid.centre32.size();
+      if (n32) {
+        p->g32_tiles = n32;
+        p->g32_cx0 = grid.cx0_32;
+        p->g32_cx1 = grid.cx1_32;
+        p->g32_cy0 = grid.cy0_32;
+        p->g32_cy1 = grid.cy1_32;
+        p->g32_rho_y = grid.rho32_y;
+        HOLO_TRY(ensure(p->g32_centre, n32 * sizeof(double4), ctx->device));
+        HOLO_HIP(hipMemcpyAsync(p->g32_centre.ptr, grid.centre32.data(), n32 * sizeof(double4),
+                                hipMemcpyHostToDevice, ctx->stream));
+      }
     }
   }
   // host vectors are freed on return: wait for the copies
@@ -614,30 +651,41 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
                                         ctx->stream));
   } else if (p->g16_tiles && holo::tuning().grid && holo::tuning().grid16) {
     const size_t nt = p->g16_tiles;
+    const bool pairs = p->g32_tiles && holo::tuning().grid32;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g16_perm.ptr),
                                        256 * nt, static_cast<float2*>(p->g16_wc.ptr),
                                        ctx->stream));
     HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
-                                         static_cast<uint4*>(p->g16_wq.ptr),
+                                         pairs, static_cast<uint4*>(p->g16_wq.ptr),
                                          static_cast<float*>(p->g16_ts.ptr), ctx->stream));
     holo::GridTiles gt;
-    gt.centre = static_cast<const double4*>(p->g16_centre.ptr);
     gt.perm = static_cast<const int32_t*>(p->g16_perm.ptr);
     gt.u = static_cast<const double*>(p->u.ptr);
-    gt.n_tiles = (uint32_t)nt;
-    gt.cx0 = p->g16_cx0;
-    gt.cx1 = p->g16_cx1;
-    gt.cy0 = p->g16_cy0;
-    gt.cy1 = p->g16_cy1;
     gt.z = p->grid_z;
     gt.rho_x = p->g16_rho_x;
-    gt.rho_y = p->g16_rho_y;
     gt.px = p->g16_px;
     gt.py = p->g16_py;
-    const int S = holo::grid_splits(M, nt);
+    if (pairs) {
+      gt.centre = static_cast<const double4*>(p->g32_centre.ptr);
+      gt.n_tiles = (uint32_t)p->g32_tiles;
+      gt.cx0 = p->g32_cx0;
+      gt.cx1 = p->g32_cx1;
+      gt.cy0 = p->g32_cy0;
+      gt.cy1 = p->g32_cy1;
+      gt.rho_y = p->g32_rho_y;
+    } else {
+      gt.centre = static_cast<const double4*>(p->g16_centre.ptr);
+      gt.n_tiles = (uint32_t)nt;
+      gt.cx0 = p->g16_cx0;
+      gt.cx1 = p->g16_cx1;
+      gt.cy0 = p->g16_cy0;
+      gt.cy1 = p->g16_cy1;
+      gt.rho_y = p->g16_rho_y;
+    }
+    const int S = holo::grid_splits(M, gt.n_tiles);
     const size_t pe = S > 1 ? (size_t)S * M : 0;
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
-    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, gt,
+    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, pairs ? 2 : 1, gt,
                                            static_cast<const float2*>(p->g16_wc.ptr),
                                            static_cast<const uint4*>(p->g16_wq.ptr),
                                            static_cast<const float*>(p->g16_ts.ptr), v, M,
@@ -801,6 +849,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid = value != 0;
   } else if (n == "grid16") {
     t.grid16 = value != 0;
+  } else if (n == "grid32") {
+    t.grid32 = value != 0;
   } else if (n == "mc_lattice") {
     t.mc_lattice = value != 0;
   } else {
@@ -820,6 +870,7 @@ long holo_get_option(const char* name) {
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
+  if (n == "grid32") return t.grid32 ? 1 : 0;
   if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;
   return -1;
 }

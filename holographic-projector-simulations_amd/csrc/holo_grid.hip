@@ -353,40 +353,54 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
   return launch_reduce_partials(S, M, partial, y, stream);
 }
 
-// ---- 16 x 16 tiles on uniform lattices -------------------------------------
+// ---- 16 x 16 and 16 x 32 tiles on uniform lattices ------------------------
 //
-// On a lattice with constant pitches (p_x, p_y) the sources of a 16 x 16 tile
-// sit at gx_a = p_x k_a, gy_b = p_y k_b from its centre, k = i - 7.5 (i < 16).
+// On a lattice with constant pitches (p_x, p_y) the sources of a tile of 16
+// rows (along x) and 16 NKC columns (along y, NKC = 1 or 2) sit at
+// gx_a = p_x k_a, gy_b = p_y k_b from its centre, k_a = a - 7.5 (a < 16),
+// k_b = b - (8 NKC - 1/2) (b < 16 NKC).  A 16 x 32 tile is the pair of 16 x 16
+// tiles 2T, 2T + 1 of the same rows, so both kernels share the slots (16 a + b
+// of each half), the weights' layout and the exact path's source order.
 // In units of the pitch, P = p_x^2 Pt with Pt = k_a^2 + qx k_a,
 // qx = -2 r_x / p_x (likewise Q, Qt, qy), and the row phase is
 //   alpha P + c2 P^2 = sx k_a + ax k_a^2 + cx Pt^2,
 //   sx = -2 alpha r_x p_x (cycles per pitch), ax = alpha p_x^2, cx = c2 p_x^4.
 // The linear part spans several cycles over a tile, so it is evaluated
-// without rounding: sx = sh + sl with sh keeping 20 significant bits, which
-// makes sh k_a exact in FP32 (2 k_a is an odd integer below 16); its fraction
-// is exact too (v_fract_f32), and only then are sl k_a, the quadratic terms
-// and f added, so the FP32 argument of sin/cos stays below 1.5 cycles however
-// large the tile's phase span.  Per tile and target that leaves the same
-// rounding as the 8 x 8 kernel with a quarter of the per-target constants and
-// half the phasors per pair.
+// without rounding: sx = sh + sl with sh keeping 20 significant bits (19 for
+// 32 columns), which makes sh k exact in FP32 (2k is an odd integer below 16,
+// or 32); its fraction is exact too (v_fract_f32), and only then are sl k, the
+// quadratic terms and f added, so the FP32 argument of sin/cos stays below 1.5
+// cycles however large the tile's phase span.  Per tile and target that leaves
+// the same rounding as the 8 x 8 kernel with a quarter of the per-target
+// constants and half the phasors per pair (16 x 16), or 3/8 (16 x 32).
 //
-// T for 16 targets is a 32 x 32 real product (complex 16 x 16), computed as two
-// row blocks rb (source rows 8 rb .. 8 rb + 7) of 16 x 16 x 32 f16 products:
-// lane l = 16 g + i holds row i = 2a' + c (a = 8 rb + a') of the A operand
-// and k-slots e < 8 of column 4g + e/2, component e%2 -- the same slots as
-// the Y operand of target j = l%16.  Per row block four MFMAs accumulate
-// W_hi Y_hi, W_hi Y_lo, W_lo Y_hi and the correction (i k_a 2^-13 W / tscale)
-// (Y Qt kb), kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13; the lane receives T of
-// rows 8 rb + 2g, 8 rb + 2g + 1 for target j.
+// The cross term 2 c2 P Q: its part linear in k_a goes through the GEMM (the
+// correction below); of its k_a^2 part, 2 c2 p_x^2 k_a^2 Q, the share of the
+// midpoint m = 28.25 of k_a^2 (0.25 .. 56.25) depends on the column only and
+// joins the column phase, alpha_y = alpha + 2 c2 p_x^2 m; only
+// 2 c2 p_x^2 (k_a^2 - m) Q, |k_a^2 - m| <= 28, is dropped (the guard bounds it).
+//
+// T for 16 targets is a 32 x 32 real product (complex 16 x 16) per 16 columns,
+// computed as two row blocks rb (source rows 8 rb .. 8 rb + 7) of 16 x 16 x 32
+// f16 products per chunk of 16 columns: lane l = 16 g + i holds row
+// i = 2a' + c (a = 8 rb + a') of the A operand and k-slots e < 8 of column
+// 16 kc + 4g + e/2, component e%2 -- the same slots as the Y operand of target
+// j = l%16.  Per row block and chunk four MFMAs accumulate W_hi Y_hi,
+// W_hi Y_lo, W_lo Y_hi and the correction (i k_a 2^-13 W / tscale) (Y Qt kb),
+// kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13; the lane receives T of rows
+// 8 rb + 2g, 8 rb + 2g + 1 for target j.
 
 constexpr int kGrid16Flush = 2;  // tiles summed in FP32 between FP64 flushes
+constexpr double kRowMid = 28.25;  // midpoint of k_a^2 over a tile's 16 rows
+constexpr float kRowDev = 28.f;    // largest |k_a^2 - kRowMid|
 
-// Accuracy bounds of the 16 x 16 kernel: those of grid_pairs_ok, with the
+// Accuracy bounds of the 16-row kernels: those of grid_pairs_ok, with the
 // FP32 rounding of the phase argument bounded by its reduced size (below one
 // cycle after the exact fraction, plus f and the quadratic terms).  rx2, ry2:
-// largest gx^2, gy^2.  NaN fails.
+// largest gx^2, gy^2; rx2c: largest |gx^2 - p_x^2 kRowMid|, the dropped share
+// of the cross term.  NaN fails.
 __device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, float ry2,
-                                                float ia, float idf, float kappa) {
+                                                float rx2c, float ia, float idf, float kappa) {
   const float qmax = Pm + Qm;
   const float round_err = 6e-8f * (2.f + 0.5f * ia * (rx2 + ry2));
   const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
@@ -394,12 +408,15 @@ __device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, f
   const float cross = beta * Pm * Qm;
   const float ampsep = kappa * kappa * Pm * Qm;
   return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
-         beta * rx2 * Qm <= 1e-6f && ampsep <= 1e-6f;
+         beta * rx2c * Qm <= 1e-6f && ampsep <= 1e-6f;
 }
 
-// x rounded to 20 significant bits (low 4 bits of the FP32 significand cleared)
-__device__ __forceinline__ float hi20(double x) {
-  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, (float)x) & 0xfffffff0u);
+// x rounded to BITS significant bits (the low 24 - BITS bits of the FP32
+// significand cleared)
+template <int BITS>
+__device__ __forceinline__ float hi_bits(double x) {
+  constexpr uint32_t mask = ~((1u << (24 - BITS)) - 1u);
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, (float)x) & mask);
 }
 
 // MFMA operands of the 16 x 16 tiles' weights, one wave per tile: for row
@@ -407,8 +424,10 @@ __device__ __forceinline__ float hi20(double x) {
 // row i = 2a' + c (a = 8 rb + a') and columns 4g .. 4g + 3 (re, im) of the real
 // form of W / tscale split into f16 hi (q = 0) and lo (q = 1), and of
 // i k_a 2^-13 W / tscale (q = 2).  wc: 256 slots per tile, slot 16 a + b.
+// pair: tiles 2T and 2T + 1 (the two halves of a 16 x 32 tile) share one
+// tscale, the power of two above the larger of their largest components.
 __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restrict__ wc,
-                                                           uint32_t n_tiles,
+                                                           uint32_t n_tiles, uint32_t pair,
                                                            uint4* __restrict__ wq,
                                                            float* __restrict__ tscale) {
   const uint32_t lane = threadIdx.x & 63;
@@ -424,6 +443,15 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
       w[rb][q] = wc[(size_t)t * 256 + 16 * (8 * rb + ap) + 4 * g + q];
       mx = fmaxf(mx, fmaxf(fabsf(w[rb][q].x), fabsf(w[rb][q].y)));
     }
+  if (pair) {  // n_tiles is even: the partner t ^ 1 exists
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float2 x = wc[(size_t)(t ^ 1u) * 256 + 16 * (8 * rb + ap) + 4 * g + q];
+        mx = fmaxf(mx, fmaxf(fabsf(x.x), fabsf(x.y)));
+      }
+  }
   for (int off = 32; off; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
   int e = 0;
   if (mx > 0.f && __builtin_isfinite(mx)) frexpf(mx, &e);
@@ -457,27 +485,35 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
   if (lane == 0) tscale[t] = ldexpf(1.f, e);
 }
 
-hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, uint4* wq, float* tscale,
-                                 hipStream_t stream) {
+hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, uint4* wq,
+                                 float* tscale, hipStream_t stream) {
   if (!n_tiles) return hipSuccess;
+  if (pair && (n_tiles & 1u)) return hipErrorInvalidValue;
   const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
-  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, wq, tscale);
+  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, pair ? 1u : 0u, wq, tscale);
   return hipGetLastError();
 }
 
-template <int SIGN, bool OBL>
+// n_tiles tiles of 16 x 16 NKC sources: centre[T], slots 256 NKC T ..
+// (perm, wc), MFMA operands of chunk kc in the layout of 16 x 16 tile NKC T + kc
+// (wq), scale tscale[NKC T].
+template <int SIGN, bool OBL, int NKC>
 __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     const double4* __restrict__ centre, const int32_t* __restrict__ perm,
     const double* __restrict__ u, const float2* __restrict__ wc,
     const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
     uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
     double py, GridBounds gb, double2* __restrict__ out) {
+  static_assert(NKC == 1 || NKC == 2, "16 or 32 columns");
   // per-(target, tile) constants (all zero when the target takes the exact
   // path, which makes its X, and so its share of the GEMM, exactly zero):
   //   k0 = (sx_hi, sx_lo, sy_hi, sy_lo)   linear phase, cycles per pitch
   //   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
   //   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
   //   k3 = (f, A, kb, 0)
+  // s_red: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in slot
+  // [16 B + j][g] (in LDS rather than registers: they are touched once per
+  // kGrid16Flush tiles, and the registers keep the kernel at 3 or 4 waves)
   __shared__ float4 s_k[4][kGridWaves][64];
   __shared__ double2 s_red[kGridWaves][64][4];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -488,27 +524,30 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   const uint32_t mo = base + lane;
   const size_t mm = mo < M ? mo : M - 1;
   const double vx = v[3 * mm], vy = v[3 * mm + 1], vz = v[3 * mm + 2];
-  // this lane's rows 8 rb + 2g + c and columns 4g + e, as k = index - 7.5
-  float kr[4], kc[4];
+  // this lane's rows 8 rb + 2g + c and columns 16 kc + 4g + e, as k = index - centre
+  float kr[4], kc[NKC][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     kr[q] = (float)(8 * (q >> 1) + 2 * (int)g + (q & 1)) - 7.5f;
-    kc[q] = (float)(4 * (int)g + q) - 7.5f;
+#pragma unroll
+    for (int c = 0; c < NKC; ++c) kc[c][q] = (float)(16 * c + 4 * (int)g + q) - (8.f * NKC - 0.5f);
   }
   const float px2f = (float)(px * px), py2f = (float)(py * py);
   const float px4f = px2f * px2f, py4f = py2f * py2f;
   const float ipxf = (float)(1.0 / px), ipyf = (float)(1.0 / py);
   // kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13 = c2 r_x kbf
   const float kbf = (float)((double)SIGN * -8.0 * M_PI * px * (py * py) * 8192.0);
+  // alpha_y = alpha (1 + mid / d^2): the row-midpoint share of the cross term
+  const double mid = -0.5 * kRowMid * px * px;
   double ex_r = 0.0, ex_i = 0.0;
-  double ar[4], ai[4];
   float fr[4], fi[4];
 #pragma unroll
   for (int B = 0; B < 4; ++B) {
-    ar[B] = ai[B] = 0.0;
     fr[B] = fi[B] = 0.f;
+    s_red[wv][16 * B + j][g] = make_double2(0.0, 0.0);
   }
   const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
+  const float rx2c = kRowDev * px2f;
   bool far_all;
   {
     const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
@@ -517,7 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
     const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
     const bool ok = grid16_pairs_ok(2.f * rxm * gb.rho_x + rx2, 2.f * rym * gb.rho_y + ry2, rx2,
-                                    ry2, ia, idf, kmax);
+                                    ry2, rx2c, ia, idf, kmax);
     far_all = __all(ok);
   }
   int since = 0;
@@ -530,8 +569,9 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       // the linear phase needs FP64 (it spans several cycles and is split
       // exactly); the small coefficients are FP32 products
       const double al = 0.5 * id * inv_lambda;
-      const double sx = -2.0 * al * rx * px, sy = -2.0 * al * ry * py;
-      const float sxh = hi20(sx), syh = hi20(sy);
+      const double aly = al * __builtin_fma(mid, id * id, 1.0);
+      const double sx = -2.0 * al * rx * px, sy = -2.0 * aly * ry * py;
+      const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
       float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
       const float idf = (float)id, alf = (float)al;
       const float rxf = (float)rx, ryf = (float)ry;
@@ -542,9 +582,9 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       bool near = false;
       if (!far_all)
         near = !grid16_pairs_ok(2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
-                                2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2, rx2, ry2, 2.f * alf,
-                                idf, kf);
-      float4 k1 = make_float4(alf * px2f, alf * py2f, c2f * px4f, c2f * py4f);
+                                2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2, rx2, ry2, rx2c,
+                                2.f * alf, idf, kf);
+      float4 k1 = make_float4(alf * px2f, (float)aly * py2f, c2f * px4f, c2f * py4f);
       float4 k2 = make_float4(-2.f * rxf * ipxf, -2.f * ryf * ipyf, kf * px2f, kf * py2f);
       float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp, c2f * rxf * kbf, 0.f);
       if (near) {
@@ -556,51 +596,56 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       s_k[2][wv][lane] = k2;
       s_k[3][wv][lane] = k3;
       if (near)
-        exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256, wc + (size_t)t * 256, 256, vx,
-                                    vy, vz, inv_lambda, ex_r, ex_i);
+        exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC, wc + (size_t)t * 256 * NKC,
+                                    256 * NKC, vx, vy, vz, inv_lambda, ex_r, ex_i);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    h8v W[6];
+    h8v W[6 * NKC];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 + q) * 64 + lane]);
-    const float ts = tscale[t];
+    for (int q = 0; q < 6 * NKC; ++q)
+      W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 * NKC + q) * 64 + lane]);
+    const float ts = tscale[(size_t)t * NKC];
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
       const float4 k0 = s_k[0][wv][16 * B + j];
       const float4 k1 = s_k[1][wv][16 * B + j];
       const float4 k2 = s_k[2][wv][16 * B + j];
       const float4 k3 = s_k[3][wv][16 * B + j];
-      h8v yh, yl, yq;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float k = kc[q], kk = k * k;
-        const float Qt = __builtin_fmaf(k2.y, k, kk);
-        const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
-        float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
-        tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
-        float sn, cs;
-        sincos_rev_f32(r + tq, sn, cs);
-        const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
-        h2v hi, lo, cq;
-        y_operand(cs, SIGN > 0 ? sn : -sn, am, Qt * k3.z, hi, lo, cq);
-        yh[2 * q] = hi[0];
-        yh[2 * q + 1] = hi[1];
-        yl[2 * q] = lo[0];
-        yl[2 * q + 1] = lo[1];
-        yq[2 * q] = cq[0];
-        yq[2 * q + 1] = cq[1];
-      }
       f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
-      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yh, D0, 0, 0, 0);
-      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yh, D1, 0, 0, 0);
-      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yl, D0, 0, 0, 0);
-      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yl, D1, 0, 0, 0);
-      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[1], yh, D0, 0, 0, 0);
-      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[4], yh, D1, 0, 0, 0);
-      D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[2], yq, D0, 0, 0, 0);
-      D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[5], yq, D1, 0, 0, 0);  // T / (tscale A)
+#pragma unroll
+      for (int c = 0; c < NKC; ++c) {
+        h8v yh, yl, yq;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float k = kc[c][q], kk = k * k;
+          const float Qt = __builtin_fmaf(k2.y, k, kk);
+          const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
+          float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
+          tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
+          float sn, cs;
+          sincos_rev_f32(r + tq, sn, cs);
+          const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
+          h2v hi, lo, cq;
+          y_operand(cs, SIGN > 0 ? sn : -sn, am, Qt * k3.z, hi, lo, cq);
+          yh[2 * q] = hi[0];
+          yh[2 * q + 1] = hi[1];
+          yl[2 * q] = lo[0];
+          yl[2 * q + 1] = lo[1];
+          yq[2 * q] = cq[0];
+          yq[2 * q + 1] = cq[1];
+        }
+        const h8v* Wc = W + 6 * c;
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yh, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yh, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yl, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yl, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[1], yh, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[4], yh, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[2], yq, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[5], yq, D1, 0, 0, 0);  // T / (tscale A)
+      }
       const float at = k3.y * ts, akt = at * k2.z;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -623,14 +668,17 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       since = 0;
 #pragma unroll
       for (int B = 0; B < 4; ++B) {
-        ar[B] += fr[B];
-        ai[B] += fi[B];
+        double2& a = s_red[wv][16 * B + j][g];
+        a = make_double2(a.x + fr[B], a.y + fi[B]);
         fr[B] = fi[B] = 0.f;
       }
     }
   }
 #pragma unroll
-  for (int B = 0; B < 4; ++B) s_red[wv][16 * B + j][g] = make_double2(ar[B] + fr[B], ai[B] + fi[B]);
+  for (int B = 0; B < 4; ++B) {
+    double2& a = s_red[wv][16 * B + j][g];
+    a = make_double2(a.x + fr[B], a.y + fi[B]);
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -644,23 +692,29 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   if (mo < M) out[(size_t)blockIdx.y * M + mo] = make_double2(sr, si);
 }
 
-hipError_t launch_superpose_grid16(int sign, bool obliquity, const GridTiles& gt,
+hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
                                    const float2* wc, const uint4* wq, const float* tscale,
                                    const double* v, size_t M, double lambda, double2* y,
                                    double2* partial, size_t partial_cap, hipStream_t stream) {
   if (M == 0) return hipSuccess;
+  if (nkc != 1 && nkc != 2) return hipErrorInvalidValue;
   int S = grid_splits(M, gt.n_tiles);
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = gt.n_tiles ? (gt.n_tiles + S - 1) / S : 0;
   const dim3 grid((unsigned)((M + kBlock - 1) / kBlock), (unsigned)S);
   double2* out = S > 1 ? partial : y;
   const double il = 1.0 / lambda;
+  const GridBounds gb{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y};
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
-      k_superpose_grid16<sg.value, (bool)obl.value><<<grid, kBlock, 0, stream>>>(
-          gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-          gt.px, gt.py, GridBounds{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y},
-          out);
+      if (nkc == 2)
+        k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
+            gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
+            gt.px, gt.py, gb, out);
+      else
+        k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
+            gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
+            gt.px, gt.py, gb, out);
     });
   });
   hipError_t e = hipGetLastError();

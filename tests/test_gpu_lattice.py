@@ -1,7 +1,9 @@
 """The matrix-core lattice kernels (HOLO_FP32_SPLIT on product-grid sources).
 
-Every test runs twice: with option "grid16" on (uniform lattices whose sides
-are multiples of 16 use 16 x 16 tiles) and off (8 x 8 tiles everywhere).
+Every test runs three times: with options "grid16" and "grid32" on (uniform
+lattices whose sides are multiples of 16 use 16-row tiles, of 32 columns where
+ny is a multiple of 32, else of 16), with "grid32" off (16 x 16 tiles), and
+with "grid16" off (8 x 8 tiles everywhere).
 Bars: relative L2 <= 1e-6 against the FP64 oracle, as for every FP32-split
 path; the lattice and the tiled kernel agree to the same bar; results do not
 depend on the launch split or on the weights' scale.
@@ -23,11 +25,13 @@ def _announce(request):
     yield
 
 
-@pytest.fixture(autouse=True, params=[16, 8], ids=["tiles16", "tiles8"])
+@pytest.fixture(autouse=True, params=[32, 16, 8], ids=["tiles32", "tiles16", "tiles8"])
 def tiles(request, pkg):
-    pkg.set_option("grid16", 1 if request.param == 16 else 0)
+    pkg.set_option("grid16", 0 if request.param == 8 else 1)
+    pkg.set_option("grid32", 1 if request.param == 32 else 0)
     yield request.param
     pkg.set_option("grid16", 1)
+    pkg.set_option("grid32", 1)
 
 
 def rel_l2(a, b):
@@ -166,20 +170,38 @@ def test_lattice_complex_weights_and_adjoint(pkg, ctx, oracle, workloads):
     assert rel_l2(yc, oracle.transform_cartesian(-1, LAM, gz, v, u)) <= 1e-6
 
 
-def test_lattice_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloads, tiles):
-    """A uniform 128 x 96 lattice runs the 16 x 16 kernel when "grid16" is on:
-    both kernels meet the bar and round differently."""
+def test_lattice_32_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloads, tiles):
+    """A uniform 128 x 96 lattice runs the 16 x 32 kernel with "grid16" and
+    "grid32" on, the 16 x 16 kernel with "grid32" off and the 8 x 8 kernel with
+    "grid16" off: all three meet the bar and round differently."""
     u = workloads.grid(128, 96, 10e-6, 0.0)
     amp, phase = weights(17, len(u))
     v = targets(18, 500, 0.35, 4e-3)
     plan = ctx.plan(u)
     ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
-    pkg.set_option("grid16", 1)
-    y16 = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
-    pkg.set_option("grid16", 0)
-    y8 = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
-    assert rel_l2(y16, ref) <= 1e-6 and rel_l2(y8, ref) <= 1e-6
-    assert 0 < rel_l2(y16, y8) <= 1e-6
+    ys = {}
+    for name, g16, g32 in (("32", 1, 1), ("16", 1, 0), ("8", 0, 1)):
+        pkg.set_option("grid16", g16)
+        pkg.set_option("grid32", g32)
+        ys[name] = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+        assert rel_l2(ys[name], ref) <= 1e-6, name
+    for a, b in (("32", "16"), ("32", "8"), ("16", "8")):
+        assert 0 < rel_l2(ys[a], ys[b]) <= 1e-6, (a, b)
+
+
+@pytest.mark.parametrize("half, scale", [(0, 1e-30), (1, 1e-30), (1, 0.0)])
+def test_lattice_tile_pair_weight_scales(pkg, ctx, oracle, workloads, half, scale, tiles):
+    """The two 16 x 16 halves of a 16 x 32 tile share one weight scale: one
+    half's weights far below (or zero beside) the other's still meet the bar."""
+    u = workloads.grid(32, 64, 10e-6, 0.0)
+    amp, phase = weights(23, len(u))
+    a2 = amp.reshape(32, 64).copy()
+    for c0 in (0, 32):  # the first or second half of both 16 x 32 tiles of each row
+        a2[:, c0 + 16 * half:c0 + 16 * half + 16] *= scale
+    amp = a2.reshape(-1)
+    v = targets(24, 300, 0.3, 2e-3)
+    y = ctx.plan(u).transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
 
 
 @pytest.mark.parametrize("half, z", [(2e-2, 0.1), (5e-2, 0.2), (8e-3, 0.05)])
