@@ -511,11 +511,13 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   //   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
   //   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
   //   k3 = (f, A, kb, 0)
-  // s_red: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in slot
-  // [16 B + j][g] (in LDS rather than registers: they are touched once per
-  // kGrid16Flush tiles, and the registers keep the kernel at 3 or 4 waves)
+  // s_rr, s_ri: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in
+  // slot [B][lane] (lane-contiguous, so free of bank conflicts; in LDS rather
+  // than registers: they are touched once per kGrid16Flush tiles, and the
+  // registers keep the kernel at 3 or 4 waves)
   __shared__ float4 s_k[4][kGridWaves][64];
-  __shared__ double2 s_red[kGridWaves][64][4];
+  __shared__ double s_rr[kGridWaves][4][64];
+  __shared__ double s_ri[kGridWaves][4][64];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t g = lane >> 4, j = lane & 15;
   const uint32_t base = blockIdx.x * kBlock + wv * 64;
@@ -544,7 +546,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
 #pragma unroll
   for (int B = 0; B < 4; ++B) {
     fr[B] = fi[B] = 0.f;
-    s_red[wv][16 * B + j][g] = make_double2(0.0, 0.0);
+    s_rr[wv][B][lane] = 0.0;
+    s_ri[wv][B][lane] = 0.0;
   }
   const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
   const float rx2c = kRowDev * px2f;
@@ -668,16 +671,16 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       since = 0;
 #pragma unroll
       for (int B = 0; B < 4; ++B) {
-        double2& a = s_red[wv][16 * B + j][g];
-        a = make_double2(a.x + fr[B], a.y + fi[B]);
+        s_rr[wv][B][lane] += fr[B];
+        s_ri[wv][B][lane] += fi[B];
         fr[B] = fi[B] = 0.f;
       }
     }
   }
 #pragma unroll
   for (int B = 0; B < 4; ++B) {
-    double2& a = s_red[wv][16 * B + j][g];
-    a = make_double2(a.x + fr[B], a.y + fi[B]);
+    s_rr[wv][B][lane] += fr[B];
+    s_ri[wv][B][lane] += fi[B];
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -685,9 +688,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   double sr = ex_r, si = ex_i;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {  // fixed order: deterministic
-    const double2 p = s_red[wv][lane][q];
-    sr += p.x;
-    si += p.y;
+    sr += s_rr[wv][lane >> 4][16 * q + (lane & 15)];
+    si += s_ri[wv][lane >> 4][16 * q + (lane & 15)];
   }
   if (mo < M) out[(size_t)blockIdx.y * M + mo] = make_double2(sr, si);
 }
