@@ -88,6 +88,7 @@ struct holo_plan {
   int device = 0;
   size_t N = 0;
   DevBuf u;  // AoS xyz, 3N doubles
+  bool uniform_z = false;  // every source has the z of source 0
   // FP32-split tiling
   unsigned tile = 16;
   size_t n_tiles = 0;
@@ -607,6 +608,14 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
   return HOLO_OK;
 }
 
+// Every source has the z of source 0 (== on the values: +0 and -0 give the
+// same dz bits up to sign, and dz enters only as dz^2 and |dz|).
+bool uniform_z(const double* u, size_t N) {
+  for (size_t n = 1; n < N; ++n)
+    if (!(u[3 * n + 2] == u[2])) return false;
+  return N > 0;
+}
+
 holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
                        unsigned tile, bool dev) {
   p->device = ctx->device;
@@ -632,6 +641,7 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
                               ctx->stream));
     }
   }
+  p->uniform_z = uniform_z(uh, N);
   return plan_upload_tiles(ctx, p, uh);
 }
 
@@ -644,7 +654,7 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     const size_t pe = split_elems_f64(M, p->N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, w0, w1,
+    HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, p->uniform_z, w0, w1,
                                         static_cast<const double*>(p->u.ptr),
                                         p->N, v, M, K, reinterpret_cast<double2*>(y),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
@@ -845,6 +855,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.target_waves = (int)value;
   } else if (n == "f64_libm") {
     t.f64_libm = value != 0;
+  } else if (n == "f64_planar") {
+    t.f64_planar = value != 0;
   } else if (n == "grid") {
     t.grid = value != 0;
   } else if (n == "grid16") {
@@ -868,6 +880,7 @@ long holo_get_option(const char* name) {
   if (n == "splits") return t.splits;
   if (n == "target_waves") return t.target_waves;
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
+  if (n == "f64_planar") return t.f64_planar ? 1 : 0;
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
   if (n == "grid32") return t.grid32 ? 1 : 0;
@@ -901,7 +914,9 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
     const size_t pe = split_elems_f64(M, N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, da, dp, du, N, dv, M, K,
+    // device positions are not scanned: the general path gives the same bits
+    const bool uz = !dev && uniform_z(u, N);
+    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, uz, da, dp, du, N, dv, M, K,
                                         reinterpret_cast<double2*>(dy),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));

@@ -29,14 +29,12 @@ struct PairF64 {
 // compiler's sqrt and IEEE division.  Both give the same d and arg bits.
 // OBL: amplitude times the obliquity factor |dz| / d
 // (matlab/holofunctions.m:226-227), r = (a / d) (|dz| / d).
+// pair_f64_d takes dx = vx - ux, dy, dz already formed (dz once per target
+// when every source has the same z); pair_f64 forms them.
 template <int SIGN, int MATH = 0, bool OBL = false>
-__device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
-                                            double uy, double uz, double vx,
-                                            double vy, double vz, double K) {
+__device__ __forceinline__ PairF64 pair_f64_d(double a, double phi, double dx, double dy,
+                                              double dz, double K) {
 #pragma clang fp contract(off)
-  const double dx = vx - ux;
-  const double dy = vy - uy;
-  const double dz = vz - uz;
   const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
   PairF64 p;
   if constexpr (MATH == 0) {
@@ -52,6 +50,14 @@ __device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
     if constexpr (OBL) p.r *= __builtin_fabs(dz) / d;
   }
   return p;
+}
+
+template <int SIGN, int MATH = 0, bool OBL = false>
+__device__ __forceinline__ PairF64 pair_f64(double a, double phi, double ux,
+                                            double uy, double uz, double vx,
+                                            double vy, double vz, double K) {
+#pragma clang fp contract(off)
+  return pair_f64_d<SIGN, MATH, OBL>(a, phi, vx - ux, vy - uy, vz - uz, K);
 }
 
 // r * (cos x, sin x) accumulated into (ar, ai): hm_sincos with the table at

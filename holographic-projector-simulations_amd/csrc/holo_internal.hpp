@@ -52,6 +52,7 @@ struct Tuning {
   int splits = 0;         // source splits (0 = automatic)
   int target_waves = 16384;
   bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
+  bool f64_planar = true; // FP64 kernel: dz once per target when all sources share one z
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
   bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
   bool grid32 = true;     // ... as 16 x 32 tiles where the columns pair up
@@ -94,9 +95,11 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
 // y (interleaved complex, M) = sum_n pair(n, m).  Polar weights: w0 = amp,
 // w1 = phase.  Cartesian: w0 = interleaved complex weights, w1 unused.
 // obliquity: every pair's amplitude is also multiplied by |dz| / d.
-// `partial` holds partial_cap double2; S source splits need S*M of them.
-hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const double* w0,
-                                const double* w1, const double* u, size_t N,
+// uniform_z: every source has the z of source 0 (same results, one operation
+// fewer per pair).  `partial` holds partial_cap double2; S source splits need
+// S*M of them.
+hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool uniform_z,
+                                const double* w0, const double* w1, const double* u, size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream);

@@ -40,7 +40,9 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units) {
 // ---------------------------------------------------------------------------
 // FP64 brute force.
 
-template <int SIGN, int R, int WF, int MATH, bool OBL>
+// PZ: every source has the z of source 0, so dz = vz - uz is formed once per
+// target (the same bits as per pair).
+template <int SIGN, int R, int WF, int MATH, bool OBL, bool PZ>
 __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     const double* __restrict__ w0, const double* __restrict__ w1,
     const double* __restrict__ u, uint32_t N, uint32_t chunk,
@@ -67,16 +69,20 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     ar[r] = 0.0;
     ai[r] = 0.0;
   }
+  double dz0[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) dz0[r] = PZ && N ? vz[r] - u[2] : 0.0;
   for (uint32_t n = n0; n < n1; ++n) {
     const double ux = u[3 * (size_t)n];
     const double uy = u[3 * (size_t)n + 1];
-    const double uz = u[3 * (size_t)n + 2];
+    const double uz = PZ ? 0.0 : u[3 * (size_t)n + 2];
     if constexpr (WF == kPolar) {
       const double a = w0[n];
       const double phi = w1[n];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const PairF64 p = pair_f64<SIGN, MATH, OBL>(a, phi, ux, uy, uz, vx[r], vy[r], vz[r], K);
+        const PairF64 p = pair_f64_d<SIGN, MATH, OBL>(a, phi, vx[r] - ux, vy[r] - uy,
+                                                      PZ ? dz0[r] : vz[r] - uz, K);
         accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r], s_tab);
       }
     } else {
@@ -84,7 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
       const double wi = w0[2 * (size_t)n + 1];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double dx = vx[r] - ux, dy = vy[r] - uy, dz = vz[r] - uz;
+        const double dx = vx[r] - ux, dy = vy[r] - uy, dz = PZ ? dz0[r] : vz[r] - uz;
         double d, id;
         hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
         if constexpr (OBL) id *= __builtin_fabs(dz) * id;
@@ -133,8 +139,8 @@ hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, doubl
   return hipGetLastError();
 }
 
-hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const double* w0,
-                                const double* w1, const double* u, size_t N,
+hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool uniform_z,
+                                const double* w0, const double* w1, const double* u, size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream) {
@@ -150,12 +156,19 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, const d
   with_sign(sign, [&](auto sg) {
     with_r(R, [&](auto rr) {
       with_bool(wf == kCartesian, [&](auto cart) {
-        with_bool(tuning().f64_libm, [&](auto libm) {
-          with_bool(obliquity, [&](auto obl) {
-            k_superpose_f64<sg.value, rr.value, cart.value ? kCartesian : kPolar, libm.value,
-                            (bool)obl.value><<<grid, kBlock, 0, stream>>>(
-                w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K, out);
-          });
+        with_bool(obliquity, [&](auto obl) {
+          constexpr int wfv = cart.value ? kCartesian : kPolar;
+          if (tuning().f64_libm) {
+            k_superpose_f64<sg.value, rr.value, wfv, 1, (bool)obl.value, false>
+                <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K,
+                                              out);
+          } else {
+            with_bool(uniform_z && tuning().f64_planar, [&](auto pz) {
+              k_superpose_f64<sg.value, rr.value, wfv, 0, (bool)obl.value, (bool)pz.value>
+                  <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M,
+                                                K, out);
+            });
+          }
         });
       });
     });

@@ -86,17 +86,19 @@ holo_status holo_synchronize(holo_ctx *ctx);
 
 /* Tuning knobs (process-wide): "r_f64", "r_f32" targets per lane (1, 2, 4),
  * "splits" source splits (0 = automatic), "target_waves", "f64_libm" (0/1:
- * use the device libm sqrt/div/sincos in the FP64 kernel), "grid" (0/1:
- * HOLO_FP32_SPLIT on lattice sources uses the matrix-core lattice kernel;
- * default 1), "grid16" (0/1: on a uniform lattice with nx, ny multiples of
- * 16 that kernel uses 16 x 16 tiles; default 1), "grid32" (0/1: with
- * "grid16", and ny a multiple of 32, tiles of 16 x 32; default 1),
- * "mc_lattice" (0/1: per-target FP32-split Monte Carlo on product-grid
+ * use the device libm sqrt/div/sincos in the FP64 kernel), "f64_planar"
+ * (0/1: when every source of a plan, or of a host-pointer transform, has the
+ * same z, the FP64 kernel forms dz once per target; same bits; default 1),
+ * "grid" (0/1: HOLO_FP32_SPLIT on lattice sources uses the matrix-core
+ * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice with nx,
+ * ny multiples of 16 that kernel uses 16 x 16 tiles; default 1), "grid32"
+ * (0/1: with "grid16", and ny a multiple of 32, tiles of 16 x 32; default
+ * 1), "mc_lattice" (0/1: per-target FP32-split Monte Carlo on product-grid
  * sources reads positions from row and column tables; same results; default
  * 1).  Results do not depend on them beyond summation order and, for "grid",
  * "grid16" and "grid32", the FP32-split rounding (all paths stay within
- * ~1e-6).  Replaces the blockDim/gridDim/
- * thread_size/n_streams hints of cuda/input.h:116-122. */
+ * ~1e-6).  Replaces the blockDim/gridDim/thread_size/n_streams hints of
+ * cuda/input.h:116-122. */
 holo_status holo_set_option(const char *name, long value);
 long holo_get_option(const char *name);
 

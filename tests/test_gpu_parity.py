@@ -67,6 +67,38 @@ def test_fp64_plain_small_all_targets(pkg, ctx, oracle, workloads, sign, libm):
     assert np.max(np.abs(y - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("obliquity", [False, True])
+def test_fp64_uniform_z_path_same_bits(pkg, ctx, oracle, workloads, obliquity):
+    """Sources that share one z (every plane of the reference) take the FP64
+    kernel's dz-once-per-target path: the same bits as the general path
+    (option f64_planar = 0), through a plan, a host-pointer transform and
+    complex weights; a source set with one z moved takes the general path."""
+    st = workloads.plain_small(0).stages[0]
+    g = np.random.default_rng(3)
+    gz = g.standard_normal(len(st.u)) + 1j * g.standard_normal(len(st.u))
+    plan = ctx.plan(st.u)
+    runs = {}
+    for planar in (1, 0):
+        pkg.set_option("f64_planar", planar)
+        runs[planar] = (
+            plan.transform(st.amp, st.phase, st.v, +1, st.lam, pkg.FP64, obliquity=obliquity),
+            ctx.transform_full(st.amp, st.phase, st.u, st.v, -1, st.lam, pkg.FP64,
+                               obliquity=obliquity),
+            plan.transform_complex(gz, st.v, -1, st.lam, pkg.FP64))
+    for a, b in zip(runs[1], runs[0]):
+        assert np.array_equal(a, b)
+    ref = oracle.transform_full(+1, st.lam, st.amp, st.phase, st.u, st.v, obliquity=obliquity)
+    assert rel_l2(runs[1][0], ref) <= 1e-12
+    u2 = st.u.copy()
+    u2[7, 2] += 1e-6
+    pkg.set_option("f64_planar", 1)
+    y2 = ctx.transform_full(st.amp, st.phase, u2, st.v, +1, st.lam, pkg.FP64,
+                            obliquity=obliquity)
+    ref2 = oracle.transform_full(+1, st.lam, st.amp, st.phase, u2, st.v, obliquity=obliquity)
+    assert rel_l2(y2, ref2) <= 1e-12
+    assert not np.array_equal(y2, runs[1][1])
+
+
 def test_fp64_golden_fixture(pkg, ctx):
     """Committed golden vectors (tests/golden/plain_small_*.dat, written by the
     oracle) of the 32x32 -> 64x64 case, sign +1."""
