@@ -668,13 +668,21 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
     HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
                                          pairs, static_cast<uint4*>(p->g16_wq.ptr),
                                          static_cast<float*>(p->g16_ts.ptr), ctx->stream));
-    holo::GridTiles gt;
-    gt.perm = static_cast<const int32_t*>(p->g16_perm.ptr);
-    gt.u = static_cast<const double*>(p->u.ptr);
-    gt.z = p->grid_z;
-    gt.rho_x = p->g16_rho_x;
-    gt.px = p->g16_px;
-    gt.py = p->g16_py;
+    holo::GridTiles g16;  // the 16 x 16 tiles
+    g16.perm = static_cast<const int32_t*>(p->g16_perm.ptr);
+    g16.u = static_cast<const double*>(p->u.ptr);
+    g16.z = p->grid_z;
+    g16.rho_x = p->g16_rho_x;
+    g16.px = p->g16_px;
+    g16.py = p->g16_py;
+    g16.centre = static_cast<const double4*>(p->g16_centre.ptr);
+    g16.n_tiles = (uint32_t)nt;
+    g16.cx0 = p->g16_cx0;
+    g16.cx1 = p->g16_cx1;
+    g16.cy0 = p->g16_cy0;
+    g16.cy1 = p->g16_cy1;
+    g16.rho_y = p->g16_rho_y;
+    holo::GridTiles gt = g16;  // the tiles of the launch: pairs of them, or themselves
     if (pairs) {
       gt.centre = static_cast<const double4*>(p->g32_centre.ptr);
       gt.n_tiles = (uint32_t)p->g32_tiles;
@@ -683,19 +691,11 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
       gt.cy0 = p->g32_cy0;
       gt.cy1 = p->g32_cy1;
       gt.rho_y = p->g32_rho_y;
-    } else {
-      gt.centre = static_cast<const double4*>(p->g16_centre.ptr);
-      gt.n_tiles = (uint32_t)nt;
-      gt.cx0 = p->g16_cx0;
-      gt.cx1 = p->g16_cx1;
-      gt.cy0 = p->g16_cy0;
-      gt.cy1 = p->g16_cy1;
-      gt.rho_y = p->g16_rho_y;
     }
     const int S = holo::grid_splits(M, gt.n_tiles);
     const size_t pe = S > 1 ? (size_t)S * M : 0;
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
-    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, pairs ? 2 : 1, gt,
+    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, pairs ? 2 : 1, gt, g16,
                                            static_cast<const float2*>(p->g16_wc.ptr),
                                            static_cast<const uint4*>(p->g16_wq.ptr),
                                            static_cast<const float*>(p->g16_ts.ptr), v, M,

@@ -494,38 +494,57 @@ hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, 
   return hipGetLastError();
 }
 
-// n_tiles tiles of 16 x 16 NKC sources: centre[T], slots 256 NKC T ..
-// (perm, wc), MFMA operands of chunk kc in the layout of 16 x 16 tile NKC T + kc
-// (wq), scale tscale[NKC T].
+// The workgroup's LDS: per-(target, tile) constants k (all zero when the
+// target takes the exact path, which makes its X, and so its share of the
+// GEMM, exactly zero):
+//   k0 = (sx_hi, sx_lo, sy_hi, sy_lo)   linear phase, cycles per pitch
+//   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
+//   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
+//   k3 = (f, A, kb, 0)
+// and rr, ri: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in slot
+// [B][lane] (lane-contiguous, so free of bank conflicts; in LDS rather than
+// registers: they are touched once per kGrid16Flush tiles, and the registers
+// keep the kernel at 3 or 4 waves).
+struct Grid16Lds {
+  float4 k[4][kGridWaves][64];
+  double rr[kGridWaves][4][64];
+  double ri[kGridWaves][4][64];
+};
+
+// The accuracy bounds of one target against every tile at once (lateral
+// extent of the tile centres, largest radii, smallest distance).
+template <bool OBL>
+__device__ __forceinline__ bool grid16_far(double vx, double vy, double vz, double inv_lambda,
+                                           const GridBounds& gb, float rx2c) {
+  const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
+  const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
+  const float rym = (float)fmax(fabs(vy - gb.cy0), fabs(vy - gb.cy1));
+  const double dz = fabs(vz - gb.z);
+  const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
+  const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
+  return grid16_pairs_ok(2.f * rxm * gb.rho_x + rx2, 2.f * rym * gb.rho_y + ry2, rx2, ry2, rx2c,
+                         ia, idf, kmax);
+}
+
+// Tiles t0 .. t1 - 1 of 16 x 16 NKC sources for the wave's 64 targets: centre[t],
+// slots 256 NKC t .. (perm, wc), MFMA operands of chunk kc in the layout of
+// 16 x 16 tile NKC t + kc (wq), scale tscale[NKC t].  far_all: every target of
+// the wave passed grid16_far for these tiles (no per-tile check).  Adds into
+// lds.rr / lds.ri (FP32 partial sums, flushed every kGrid16Flush tiles) and, for
+// (target, tile) pairs that fail the bounds, into ex_r / ex_i.
 template <int SIGN, bool OBL, int NKC>
-__global__ __launch_bounds__(kBlock) void k_superpose_grid16(
-    const double4* __restrict__ centre, const int32_t* __restrict__ perm,
+__device__ __forceinline__ void grid16_tiles(
+    Grid16Lds& lds, const double4* __restrict__ centre, const int32_t* __restrict__ perm,
     const double* __restrict__ u, const float2* __restrict__ wc,
-    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
-    uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
-    double py, GridBounds gb, double2* __restrict__ out) {
+    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t t0, uint32_t t1,
+    double vx, double vy, double vz, double inv_lambda, double px, double py,
+    const GridBounds& gb, bool far_all, double& ex_r, double& ex_i) {
+  // every fused operation is written out, so that the same tiles give the same
+  // bits whichever kernel this is inlined into
+#pragma clang fp contract(off)
   static_assert(NKC == 1 || NKC == 2, "16 or 32 columns");
-  // per-(target, tile) constants (all zero when the target takes the exact
-  // path, which makes its X, and so its share of the GEMM, exactly zero):
-  //   k0 = (sx_hi, sx_lo, sy_hi, sy_lo)   linear phase, cycles per pitch
-  //   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
-  //   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
-  //   k3 = (f, A, kb, 0)
-  // s_rr, s_ri: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in
-  // slot [B][lane] (lane-contiguous, so free of bank conflicts; in LDS rather
-  // than registers: they are touched once per kGrid16Flush tiles, and the
-  // registers keep the kernel at 3 or 4 waves)
-  __shared__ float4 s_k[4][kGridWaves][64];
-  __shared__ double s_rr[kGridWaves][4][64];
-  __shared__ double s_ri[kGridWaves][4][64];
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t g = lane >> 4, j = lane & 15;
-  const uint32_t base = blockIdx.x * kBlock + wv * 64;
-  const uint32_t t0 = blockIdx.y * chunk;
-  const uint32_t t1 = min(n_tiles, t0 + chunk);
-  const uint32_t mo = base + lane;
-  const size_t mm = mo < M ? mo : M - 1;
-  const double vx = v[3 * mm], vy = v[3 * mm + 1], vz = v[3 * mm + 2];
   // this lane's rows 8 rb + 2g + c and columns 16 kc + 4g + e, as k = index - centre
   float kr[4], kc[NKC][4];
 #pragma unroll
@@ -541,27 +560,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   const float kbf = (float)((double)SIGN * -8.0 * M_PI * px * (py * py) * 8192.0);
   // alpha_y = alpha (1 + mid / d^2): the row-midpoint share of the cross term
   const double mid = -0.5 * kRowMid * px * px;
-  double ex_r = 0.0, ex_i = 0.0;
-  float fr[4], fi[4];
-#pragma unroll
-  for (int B = 0; B < 4; ++B) {
-    fr[B] = fi[B] = 0.f;
-    s_rr[wv][B][lane] = 0.0;
-    s_ri[wv][B][lane] = 0.0;
-  }
   const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
   const float rx2c = kRowDev * px2f;
-  bool far_all;
-  {
-    const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
-    const float rym = (float)fmax(fabs(vy - gb.cy0), fabs(vy - gb.cy1));
-    const double dz = fabs(vz - gb.z);
-    const float idf = (float)(1.0 / dz), ia = (float)(1.0 / (dz / inv_lambda));
-    const float kmax = (OBL ? 1.f : 0.5f) * idf * idf;
-    const bool ok = grid16_pairs_ok(2.f * rxm * gb.rho_x + rx2, 2.f * rym * gb.rho_y + ry2, rx2,
-                                    ry2, rx2c, ia, idf, kmax);
-    far_all = __all(ok);
-  }
+  float fr[4], fi[4];
+#pragma unroll
+  for (int B = 0; B < 4; ++B) fr[B] = fi[B] = 0.f;
   int since = 0;
   for (uint32_t t = t0; t < t1; ++t) {
     {
@@ -594,10 +597,10 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
         k0 = make_float4(0.f, 0.f, 0.f, 0.f);
         k1 = k2 = k3 = k0;
       }
-      s_k[0][wv][lane] = k0;
-      s_k[1][wv][lane] = k1;
-      s_k[2][wv][lane] = k2;
-      s_k[3][wv][lane] = k3;
+      lds.k[0][wv][lane] = k0;
+      lds.k[1][wv][lane] = k1;
+      lds.k[2][wv][lane] = k2;
+      lds.k[3][wv][lane] = k3;
       if (near)
         exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC, wc + (size_t)t * 256 * NKC,
                                     256 * NKC, vx, vy, vz, inv_lambda, ex_r, ex_i);
@@ -612,10 +615,10 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     const float ts = tscale[(size_t)t * NKC];
 #pragma unroll
     for (int B = 0; B < 4; ++B) {
-      const float4 k0 = s_k[0][wv][16 * B + j];
-      const float4 k1 = s_k[1][wv][16 * B + j];
-      const float4 k2 = s_k[2][wv][16 * B + j];
-      const float4 k3 = s_k[3][wv][16 * B + j];
+      const float4 k0 = lds.k[0][wv][16 * B + j];
+      const float4 k1 = lds.k[1][wv][16 * B + j];
+      const float4 k2 = lds.k[2][wv][16 * B + j];
+      const float4 k3 = lds.k[3][wv][16 * B + j];
       f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int c = 0; c < NKC; ++c) {
@@ -671,16 +674,55 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
       since = 0;
 #pragma unroll
       for (int B = 0; B < 4; ++B) {
-        s_rr[wv][B][lane] += fr[B];
-        s_ri[wv][B][lane] += fi[B];
+        lds.rr[wv][B][lane] += fr[B];
+        lds.ri[wv][B][lane] += fi[B];
         fr[B] = fi[B] = 0.f;
       }
     }
   }
 #pragma unroll
   for (int B = 0; B < 4; ++B) {
-    s_rr[wv][B][lane] += fr[B];
-    s_ri[wv][B][lane] += fi[B];
+    lds.rr[wv][B][lane] += fr[B];
+    lds.ri[wv][B][lane] += fi[B];
+  }
+}
+
+// One wave per 64 targets.  NKC = 1: the 16 x 16 tiles t0 .. t1 - 1 of centre
+// and gb.  NKC = 2: the 16 x 32 tiles of centre and gb when every target of
+// the wave passes their far-field bounds; otherwise the wave takes the same
+// sources as the 16 x 16 tiles 2 t0 .. 2 t1 - 1 (centre16, gb16), whose
+// looser bounds send fewer (target, tile) pairs to the exact path.
+template <int SIGN, bool OBL, int NKC>
+__global__ __launch_bounds__(kBlock) void k_superpose_grid16(
+    const double4* __restrict__ centre, const int32_t* __restrict__ perm,
+    const double* __restrict__ u, const float2* __restrict__ wc,
+    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
+    uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
+    double py, GridBounds gb, const double4* __restrict__ centre16, GridBounds gb16,
+    double2* __restrict__ out) {
+  __shared__ Grid16Lds lds;
+  const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t base = blockIdx.x * kBlock + wv * 64;
+  const uint32_t t0 = blockIdx.y * chunk;
+  const uint32_t t1 = min(n_tiles, t0 + chunk);
+  const uint32_t mo = base + lane;
+  const size_t mm = mo < M ? mo : M - 1;
+  const double vx = v[3 * mm], vy = v[3 * mm + 1], vz = v[3 * mm + 2];
+#pragma unroll
+  for (int B = 0; B < 4; ++B) {
+    lds.rr[wv][B][lane] = 0.0;
+    lds.ri[wv][B][lane] = 0.0;
+  }
+  const float rx2c = kRowDev * (float)(px * px);
+  double ex_r = 0.0, ex_i = 0.0;
+  const bool far = __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb, rx2c));
+  if (NKC == 1 || far) {
+    grid16_tiles<SIGN, OBL, NKC>(lds, centre, perm, u, wc, wq, tscale, t0, t1, vx, vy, vz,
+                                 inv_lambda, px, py, gb, far, ex_r, ex_i);
+  } else {
+    const bool far16 = __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb16, rx2c));
+    grid16_tiles<SIGN, OBL, 1>(lds, centre16, perm, u, wc, wq, tscale, 2 * t0, 2 * t1, vx, vy,
+                               vz, inv_lambda, px, py, gb16, far16, ex_r, ex_i);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -688,18 +730,21 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   double sr = ex_r, si = ex_i;
 #pragma unroll
   for (int q = 0; q < 4; ++q) {  // fixed order: deterministic
-    sr += s_rr[wv][lane >> 4][16 * q + (lane & 15)];
-    si += s_ri[wv][lane >> 4][16 * q + (lane & 15)];
+    sr += lds.rr[wv][lane >> 4][16 * q + (lane & 15)];
+    si += lds.ri[wv][lane >> 4][16 * q + (lane & 15)];
   }
   if (mo < M) out[(size_t)blockIdx.y * M + mo] = make_double2(sr, si);
 }
 
 hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
-                                   const float2* wc, const uint4* wq, const float* tscale,
-                                   const double* v, size_t M, double lambda, double2* y,
-                                   double2* partial, size_t partial_cap, hipStream_t stream) {
+                                   const GridTiles& halves, const float2* wc, const uint4* wq,
+                                   const float* tscale, const double* v, size_t M, double lambda,
+                                   double2* y, double2* partial, size_t partial_cap,
+                                   hipStream_t stream) {
   if (M == 0) return hipSuccess;
   if (nkc != 1 && nkc != 2) return hipErrorInvalidValue;
+  if (nkc == 2 && (halves.n_tiles != 2 * gt.n_tiles || !halves.centre))
+    return hipErrorInvalidValue;
   int S = grid_splits(M, gt.n_tiles);
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = gt.n_tiles ? (gt.n_tiles + S - 1) / S : 0;
@@ -707,16 +752,18 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
   double2* out = S > 1 ? partial : y;
   const double il = 1.0 / lambda;
   const GridBounds gb{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y};
+  const GridBounds gb16{halves.cx0, halves.cx1, halves.cy0, halves.cy1, halves.z, halves.rho_x,
+                        halves.rho_y};
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
       if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, out);
+            gt.px, gt.py, gb, halves.centre, gb16, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, out);
+            gt.px, gt.py, gb, gt.centre, gb, out);
     });
   });
   hipError_t e = hipGetLastError();

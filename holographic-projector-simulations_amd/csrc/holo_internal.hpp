@@ -130,17 +130,20 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
 // 16 x 16 tiles of a uniform lattice (pitches gt.px, gt.py; gt.goff unused):
 // wc holds 256 slots per tile (slot 16a + b), launch_grid16_weights lays them
 // out as wq (6 x 8 f16 per tile and lane) and tscale; with pair, tiles 2T and
-// 2T + 1 share one tscale.  launch_superpose_grid16 with nkc = 1 takes 16 x 16
-// tiles (gt.n_tiles of them, centres gt.centre); with nkc = 2 it takes the
-// 16 x 32 tiles formed by the pairs (gt.n_tiles pairs, their centres in
-// gt.centre, rho_y and the bounds those of the pairs), reading the same wc,
-// wq, perm and tscale (from launch_grid16_weights with pair).
+// 2T + 1 share one tscale.  launch_superpose_grid16 with nkc = 1 takes the
+// 16 x 16 tiles gt (halves unused); with nkc = 2 it takes the 16 x 32 tiles
+// formed by the pairs (gt.n_tiles pairs, their centres in gt.centre, rho_y and
+// the bounds those of the pairs), reading the same wc, wq, perm and tscale
+// (from launch_grid16_weights with pair), and halves describes the same
+// sources as 16 x 16 tiles (2 gt.n_tiles of them): a wave whose targets fail
+// the 16 x 32 far-field bounds takes those.
 hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, uint4* wq,
                                  float* tscale, hipStream_t stream);
 hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
-                                   const float2* wc, const uint4* wq, const float* tscale,
-                                   const double* v, size_t M, double lambda, double2* y,
-                                   double2* partial, size_t partial_cap, hipStream_t stream);
+                                   const GridTiles& halves, const float2* wc, const uint4* wq,
+                                   const float* tscale, const double* v, size_t M, double lambda,
+                                   double2* y, double2* partial, size_t partial_cap,
+                                   hipStream_t stream);
 // Source splits the lattice kernel uses for M targets over n_tiles tiles.
 int grid_splits(size_t M, size_t n_tiles);
 

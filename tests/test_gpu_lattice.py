@@ -189,6 +189,27 @@ def test_lattice_32_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloa
         assert 0 < rel_l2(ys[a], ys[b]) <= 1e-6, (a, b)
 
 
+def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, tiles):
+    """Targets between the two tile sizes' far-field bounds (z = 0.1 m, up to
+    0.5 mm off axis): every wave fails the 16 x 32 bounds somewhere and takes
+    the 16 x 16 halves of the same tiles, none of which needs the exact path.
+    With amplitudes 0.75 every tile's largest weight component lies in
+    [0.53, 0.75], so tiles and pairs alike have scale 1, and the result is the
+    16 x 16 kernel's, bit for bit."""
+    u = workloads.grid(64, 64, 10e-6, 0.0)
+    amp = np.full(len(u), 0.75)
+    phase = np.random.default_rng(25).random(len(u)) * 2 * np.pi
+    v = targets(26, 512, 0.1, 5e-4)
+    plan = ctx.plan(u)
+    ys = {}
+    for g32 in (1, 0):
+        pkg.set_option("grid16", 1)
+        pkg.set_option("grid32", g32)
+        ys[g32] = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+    assert np.array_equal(ys[1], ys[0])
+    assert rel_l2(ys[1], oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-6
+
+
 @pytest.mark.parametrize("half, scale", [(0, 1e-30), (1, 1e-30), (1, 0.0)])
 def test_lattice_tile_pair_weight_scales(pkg, ctx, oracle, workloads, half, scale, tiles):
     """The two 16 x 16 halves of a 16 x 32 tile share one weight scale: one
