@@ -189,7 +189,9 @@ def test_lattice_32_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloa
         assert 0 < rel_l2(ys[a], ys[b]) <= 1e-6, (a, b)
 
 
-def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, tiles):
+@pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
+def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, sign, obliquity,
+                                                tiles):
     """Targets between the two tile sizes' far-field bounds (z = 0.1 m, up to
     0.5 mm off axis): every wave fails the 16 x 32 bounds somewhere and takes
     the 16 x 16 halves of the same tiles, none of which needs the exact path.
@@ -205,9 +207,10 @@ def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, til
     for g32 in (1, 0):
         pkg.set_option("grid16", 1)
         pkg.set_option("grid32", g32)
-        ys[g32] = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
+        ys[g32] = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
     assert np.array_equal(ys[1], ys[0])
-    assert rel_l2(ys[1], oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-6
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=obliquity)
+    assert rel_l2(ys[1], ref) <= 1e-6
 
 
 @pytest.mark.parametrize("half, scale", [(0, 1e-30), (1, 1e-30), (1, 0.0)])
