@@ -204,6 +204,7 @@ def main():
                                 if a.lattice_tiles else "tiled kernel, tiles of 64"),
                 "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
                 "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
+                "fp32_split_gflops_29": p3.pairs / t32 * FLOP_PER_PAIR / 1e9,
                 "rel_l2_vs_fp64": rel_l2(f32, f64),
                 "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)))}
             log(f"[bench] fp32-split: {out['fp32_split']}")
@@ -220,7 +221,8 @@ def main():
                     torch.cuda.synchronize(dev)
                     tk = time.perf_counter() - t0
                     key = ("batched_" if shared else "") + str(K)
-                    mc[key] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk}
+                    mc[key] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk,
+                               "gflops_29": K * v3.shape[0] / tk * FLOP_PER_PAIR / 1e9}
                     if f64 is not None:
                         mc[key]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
             out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0; K draws "
