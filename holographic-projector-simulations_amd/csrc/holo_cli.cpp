@@ -8,7 +8,8 @@
 //                 y0_phase.dat, v0.dat and out.json into DIR.  With -F the
 //                 reference computes the x -> v transform forwards and writes
 //                 it as z0_0 / w0_0 (cuda/main.cu:218-221); that is kept.
-//   (no -f)       generate X object planes (circle of -x points) and the
+//   (no -f)       generate X object planes (circle of -x points, or the line,
+//                 log-line or dotted circle HOLO_SHAPE selects) and the
 //                 projector / projection lattices, run x -> y (backwards) and
 //                 y -> z (forwards) per plane pair, write to ../tmp/.
 // The target count is -y (not the size of v0.dat); missing targets are the
@@ -103,6 +104,7 @@ void show_help(const char* p) {  // cuda/input.h:7-39
          "\t HOLO_PROJECT_PHASE \t 0: amplitude-only projector with reference wave (default 1)\n"
          "\t HOLO_PRECISION \t fp64 (default) or fp32split\n"
          "\t HOLO_OBLIQUITY \t 1: weight every pair by |dz|/d (default 0)\n"
+         "\t HOLO_SHAPE \t generated object: circle (default), line, logline, dottedcircle\n"
          "\n", p);
 }
 
@@ -219,22 +221,71 @@ void plane(std::vector<double>& v, const Plane& p) {
     for (int d = 0; d < 3; ++d) v[3 * i + d] = n ? v[d] : 0.0;
 }
 
-// cuda/init.h:160-234, Shape::Circle (the shape cuda/main.cu:44 uses).
-void sparse_plane_circle(std::vector<double>& u, double width, const Cartesian<double>& offset,
-                         double modulate) {
+// cuda/main.h:9; cuda/main.cu:55 generates a Circle, HOLO_SHAPE another.
+enum class Shape { Line, LogLine, Circle, DottedCircle };
+Shape g_shape = Shape::Circle;
+
+// cuda/algebra.h:77-91: len values from a to b, evenly spaced in log10
+std::vector<double> geomspace(size_t len, double a, double b) {
+  std::vector<double> v = linspace(len, std::log10(a), std::log10(b));
+  for (double& x : v) x = std::pow(10., x);
+  return v;
+}
+
+// cuda/init.h:160-234: n points of the shape at z = offset.z, then shifted by
+// (offset.x, offset.y) when either is nonzero.  Line: x from -w/2 to w/2;
+// LogLine: x = geomspace(1, 1 + w) - 1 + w/2; Circle: n points on the circle
+// of diameter w, starting at angle 0.1125 + 2 pi modulate; DottedCircle: point
+// 0 at the centre, n - 1 points on the circle.  y is 0 on the lines.  (The
+// reference leaves point 0 of a DottedCircle as its reused vector held it: the
+// centre on the first plane.)
+void sparse_plane(std::vector<double>& u, Shape shape, double width,
+                  const Cartesian<double>& offset, double modulate) {
   const size_t n = u.size() / 3;
-  for (size_t i = 0; i < n; ++i) u[3 * i + 2] = offset.z;
+  for (size_t i = 0; i < n; ++i) {
+    u[3 * i] = u[3 * i + 1] = 0.0;
+    u[3 * i + 2] = offset.z;
+  }
+  if (n == 0) return;
   if (n == 1) {
     u[0] = offset.x;
     u[1] = offset.y;
     return;
   }
-  const double radius = width / 2.0, d_phase = 2 * M_PI / (double)n,
-               arbitrary_offset = 0.1125 + modulate * 2 * M_PI;
-  for (size_t i = 0; i < n; ++i) {
-    u[3 * i] = std::sin(i * d_phase + arbitrary_offset) * radius + offset.x;
-    u[3 * i + 1] = std::cos(i * d_phase + arbitrary_offset) * radius + offset.y;
+  const double radius = width / 2.0, arbitrary_offset = 0.1125 + modulate * 2 * M_PI;
+  switch (shape) {
+    case Shape::Line: {
+      const double du = width / (double)(n - 1), half_width = width / 2.0;
+      for (size_t i = 0; i < n; ++i) u[3 * i] = i * du - half_width;
+      break;
+    }
+    case Shape::LogLine: {
+      const std::vector<double> x = geomspace(n, 1, 1 + width);
+      for (size_t i = 0; i < n; ++i) u[3 * i] = x[i] - 1 + width / 2.;
+      break;
+    }
+    case Shape::Circle: {
+      const double d_phase = 2 * M_PI / (double)n;
+      for (size_t i = 0; i < n; ++i) {
+        u[3 * i] = std::sin(i * d_phase + arbitrary_offset) * radius;
+        u[3 * i + 1] = std::cos(i * d_phase + arbitrary_offset) * radius;
+      }
+      break;
+    }
+    case Shape::DottedCircle: {
+      const double d_phase = 2 * M_PI / (double)(n - 1);
+      for (size_t i = 1; i < n; ++i) {
+        u[3 * i] = std::sin(i * d_phase + arbitrary_offset) * radius;
+        u[3 * i + 1] = std::cos(i * d_phase + arbitrary_offset) * radius;
+      }
+      break;
+    }
   }
+  if (offset.x != 0 || offset.y != 0)
+    for (size_t i = 0; i < n; ++i) {
+      u[3 * i] += offset.x;
+      u[3 * i + 1] += offset.y;
+    }
 }
 
 std::vector<double> read_dat(const std::string& path) {  // cuda/util.h:248-283
@@ -334,7 +385,7 @@ int body(Params& params) {
                                          lerp(params.obj_offset.y, ratio),
                                          gerp(params.obj_offset.z, ratio)};
       const Plane x_plane{lerp(params.obj_width, ratio), obj_offset, 1., false};
-      sparse_plane_circle(u, x_plane.width, x_plane.offset, i / (double)params.n_planes.obj);
+      sparse_plane(u, g_shape, x_plane.width, x_plane.offset, i / (double)params.n_planes.obj);
       const Plane y_plane{lerp(params.projector_width, ratio), {0., 0., 0.},
                           params.aspect_ratio.projector, params.randomize};
       plane(v, y_plane);
@@ -400,6 +451,14 @@ int main(int argc, char** argv) {
     const std::string v(ob);
     if (v == "1") holo::settings().obliquity = true;
     else if (v != "0") die("HOLO_OBLIQUITY must be 0 or 1");
+  }
+  if (const char* sh = getenv("HOLO_SHAPE")) {
+    const std::string v(sh);
+    if (v == "circle") g_shape = Shape::Circle;
+    else if (v == "line") g_shape = Shape::Line;
+    else if (v == "logline") g_shape = Shape::LogLine;
+    else if (v == "dottedcircle") g_shape = Shape::DottedCircle;
+    else die("HOLO_SHAPE must be circle, line, logline or dottedcircle");
   }
   const char* pp = getenv("HOLO_PROJECT_PHASE");
   const bool project_phase = !(pp && std::string(pp) == "0");

@@ -145,6 +145,40 @@ def test_generated_mode_writes_reference_files(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", ["circle", "line", "logline", "dottedcircle"])
+def test_generated_object_shapes(tmp_path, shape):
+    """HOLO_SHAPE selects the object of cuda/init.h:160-234 (width 1 mm, 16
+    points, first plane: no lateral offset, modulate 0), checked against the
+    closed forms."""
+    work = tmp_path / "run"
+    (tmp_path / "tmp").mkdir()
+    work.mkdir()
+    r = run(["-x", "16", "-y", "1024", "-z", "1024"], cwd=str(work), env={"HOLO_SHAPE": shape})
+    assert r.returncode == 0, r.stderr
+    u = np.fromfile(tmp_path / "tmp" / "u0.dat", "<f8").reshape(-1, 3)
+    n, w = 16, 1e-3
+    i = np.arange(n)
+    if shape == "line":
+        x, y = i * (w / (n - 1)) - w / 2, np.zeros(n)
+    elif shape == "logline":
+        x, y = 10.0 ** (i * (np.log10(1 + w) / (n - 1))) - 1 + w / 2, np.zeros(n)
+    else:
+        a = 2 * np.pi * i / (n if shape == "circle" else n - 1) + 0.1125
+        x, y = np.sin(a) * w / 2, np.cos(a) * w / 2
+        if shape == "dottedcircle":
+            x[0] = y[0] = 0.0
+    assert u.shape == (n, 3)
+    np.testing.assert_allclose(u[:, 0], x, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(u[:, 1], y, rtol=0, atol=1e-15)
+    assert np.all(u[:, 2] == u[0, 2])
+
+
+def test_invalid_shape_exits_nonzero():
+    r = run(["-x", "4"], env={"HOLO_SHAPE": "square"})
+    assert r.returncode != 0 and "HOLO_SHAPE" in r.stderr
+
+
+@pytest.mark.gpu
 def test_reference_wave_mode(tmp_path, oracle):
     """HOLO_PROJECT_PHASE=0 (the reference's PROJECT_PHASE=0 build,
     cuda/transform.cu:379-395): y is normalised, a unit point source 0.25 m
