@@ -454,6 +454,84 @@ def write_dat(path: str, values) -> None:
     check(lib().holo_write_dat(path.encode(), _ptr(a), a.size))
 
 
+def parse_json(lines) -> dict:
+    """The out.json lines of a run, grouped by the first letter of "phasor"
+    (x, y, z; u, v, w stay empty), as py/util.py:686-698."""
+    import json
+    params = {k: [] for k in "xyzuvw"}
+    for line in lines:
+        if isinstance(line, bytes):
+            line = line.decode()
+        if line.strip():
+            p = json.loads(line)
+            k = p["phasor"][:1]
+            if k not in "xyz":
+                raise ValueError(f"unexpected phasor key {p['phasor']!r}")
+            params[k].append(p)
+    return params
+
+
+def parse_file(dir: str = "../tmp", zipfilename: Optional[str] = "out.zip", prefix: str = "out",
+               read_data: bool = True) -> Tuple[dict, ...]:
+    """A run's metadata and arrays, as py/util.py:700-744: from the archive
+    dir/zipfilename when it exists (the reference's layout), else from the
+    files in dir.  params[k] lists the out.json records of phasor k;
+    data[k] the (len, 2) [amp, phase] arrays of each record and data[p] its
+    (len, 3) positions (p = first letter of "pos").  A record whose files are
+    shorter than its "len" gets len 0 and no arrays, as in the reference.
+    Returns (params,) when read_data is False."""
+    import zipfile
+    archive = None
+    path = os.path.join(dir, zipfilename) if zipfilename else None
+    if path and os.path.exists(path):
+        archive = zipfile.ZipFile(path)
+
+    def read_bytes(name: str, count: int, precision: int) -> np.ndarray:
+        if precision not in (4, 8, 16):
+            raise NotImplementedError(f"precision {precision}")
+        if archive is not None:
+            size = archive.getinfo(name).file_size
+        else:
+            size = os.path.getsize(os.path.join(dir, name))
+        if size < count * precision:
+            raise EOFError(f"{name}: {size} bytes < {count} x {precision}")
+        if archive is not None:
+            with archive.open(name) as f:
+                raw = f.read(count * precision)
+        else:
+            with open(os.path.join(dir, name), "rb") as f:
+                raw = f.read(count * precision)
+        return np.frombuffer(raw, dtype=f"<f{precision}", count=count).copy()
+
+    try:
+        if archive is not None:
+            with archive.open(prefix + ".json") as f:
+                params = parse_json(f.readlines())
+        else:
+            with open(os.path.join(dir, prefix + ".json")) as f:
+                params = parse_json(f.readlines())
+        if not read_data:
+            return (params,)
+        data = {k: [] for k in "xyzuvw"}
+        for k in "xyz":
+            for p in params[k]:
+                k2 = p["pos"][:1]
+                p["len"] = int(p["len"])
+                try:
+                    amp = read_bytes(p["phasor"] + "_amp.dat", p["len"], p["precision"])
+                    phase = read_bytes(p["phasor"] + "_phase.dat", p["len"], p["precision"])
+                    pos = read_bytes(p["pos"] + ".dat", p["len"] * p["dims"], p["precision"])
+                except (EOFError, KeyError, FileNotFoundError):
+                    p["len"] = 0
+                    continue
+                data[k].append(np.stack([amp, phase], axis=1))
+                data[k2].append(pos.reshape(-1, p["dims"]))
+        return params, data
+    finally:
+        if archive is not None:
+            archive.close()
+
+
 # ---------------------------------------------------------------------------
 # Mirrors of the reference's C++ templates (cuda/transform.cu)
 

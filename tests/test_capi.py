@@ -46,6 +46,65 @@ def test_options(pkg):
     with pytest.raises(pkg.HoloError):
         pkg.set_option("no_such_option", 1)
     assert pkg.get_option("no_such_option") == -1
+    for name in ("f64_libm", "f64_planar", "grid", "grid16", "grid32", "mc_lattice"):
+        old = pkg.get_option(name)
+        assert old in (0, 1)
+        pkg.set_option(name, 1 - old)
+        assert pkg.get_option(name) == 1 - old
+        pkg.set_option(name, old)
+
+
+def _write_run(pkg, d, len_y=5):
+    """A run directory as the CLI writes it (cuda/util.h:358-444): one x and
+    one y record."""
+    import json
+    g = np.random.default_rng(1)
+    rec = []
+    for k, pos, n in (("x0", "u0", 3), ("y0", "v0", len_y)):
+        pkg.write_dat(str(d / f"{k}_amp.dat"), g.random(n))
+        pkg.write_dat(str(d / f"{k}_phase.dat"), g.random(n))
+        pkg.write_dat(str(d / f"{pos}.dat"), g.random(3 * n))
+        rec.append({"phasor": k, "amp_sum": 1.0, "phase_sum": 0.0, "pos": pos, "len": n,
+                    "precision": 8, "dims": 3, "x_offset": 0, "y_offset": 0, "z_offset": 0,
+                    "width": 1e-3, "randomized": 0, "aspect_ratio": 1, "timestamp": 0,
+                    "runtime": 0.0, "flops": 0.0})
+    with open(d / "out.json", "w") as f:
+        for r in rec:
+            f.write(json.dumps(r) + "\n")
+
+
+def test_parse_file_directory_and_zip(pkg, tmp_path):
+    """py/util.py:700-744: the reference's reader layout, from the files and
+    from out.zip."""
+    import zipfile
+    _write_run(pkg, tmp_path)
+    params, data = pkg.parse_file(str(tmp_path), zipfilename=None)
+    assert [p["phasor"] for p in params["x"] + params["y"]] == ["x0", "y0"]
+    assert data["x"][0].shape == (3, 2) and data["y"][0].shape == (5, 2)
+    assert data["u"][0].shape == (3, 3) and data["v"][0].shape == (5, 3)
+    assert np.array_equal(data["y"][0][:, 0], pkg.read_dat(str(tmp_path / "y0_amp.dat")))
+    assert np.array_equal(data["v"][0].reshape(-1), pkg.read_dat(str(tmp_path / "v0.dat")))
+    with zipfile.ZipFile(tmp_path / "out.zip", "w") as z:
+        for name in os.listdir(tmp_path):
+            if name != "out.zip":
+                z.write(tmp_path / name, name)
+    for name in os.listdir(tmp_path):
+        if name != "out.zip":
+            os.remove(tmp_path / name)
+    params2, data2 = pkg.parse_file(str(tmp_path))
+    assert params2 == params
+    for k in "xyuv":
+        assert np.array_equal(data2[k][0], data[k][0])
+    assert pkg.parse_file(str(tmp_path), read_data=False) == (params2,)
+
+
+def test_parse_file_short_record(pkg, tmp_path):
+    """A record whose files are shorter than its len gets len 0 (py/util.py:738-741)."""
+    _write_run(pkg, tmp_path)
+    pkg.write_dat(str(tmp_path / "y0_amp.dat"), np.zeros(2))
+    params, data = pkg.parse_file(str(tmp_path), zipfilename=None)
+    assert params["y"][0]["len"] == 0 and data["y"] == [] and data["v"] == []
+    assert len(data["x"]) == 1
 
 
 def test_dat_roundtrip_is_raw_little_endian_f8(pkg, tmp_path):

@@ -126,8 +126,9 @@ def test_target_count_comes_from_y_flag(tmp_path, oracle):
 
 
 @pytest.mark.gpu
-def test_generated_mode_writes_reference_files(tmp_path):
-    """No -f: object circle -> projector lattice -> projection lattice in ../tmp/."""
+def test_generated_mode_writes_reference_files(tmp_path, pkg):
+    """No -f: object circle -> projector lattice -> projection lattice in ../tmp/,
+    readable by the reference's reader layout (parse_file)."""
     work = tmp_path / "run"
     (tmp_path / "tmp").mkdir()
     work.mkdir()
@@ -142,6 +143,12 @@ def test_generated_mode_writes_reference_files(tmp_path):
     assert w.shape == (1024, 3) and np.allclose(w[:, 2], 0.35)
     u = np.fromfile(out / "u0.dat", "<f8").reshape(-1, 3)
     assert np.allclose(np.hypot(u[:, 0], u[:, 1]), 0.5e-3)  # circle of width 1 mm
+    params, data = pkg.parse_file(str(out), zipfilename=None)
+    assert [len(params[k]) for k in "xyz"] == [1, 1, 2]
+    assert data["x"][0].shape == (16, 2) and data["u"][0].shape == (16, 3)
+    assert data["y"][0].shape == (4096, 2) and np.array_equal(data["y"][0][:, 0], y)
+    assert [a.shape for a in data["z"]] == [(1024, 2)] * 2
+    assert np.array_equal(data["w"][1], w)
 
 
 @pytest.mark.gpu
