@@ -164,9 +164,10 @@ holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
 /* Lattice the plan detected: the sources are a product grid x_i * y_j at one
  * z, in the order n = i*ny + j, with nx and ny multiples of 8 (the
  * reference's plane generators, cuda/init.h:89-158).  HOLO_FP32_SPLIT then
- * runs the lattice kernel (8 x 8 tiles as complex GEMMs on the matrix
- * cores).  nx = ny = 0 when the plan is not a lattice.  No reference
- * counterpart. */
+ * runs the lattice kernel: tiles of 16 x 32 or 16 x 16 sources on a uniform
+ * lattice with nx, ny multiples of 16, else 8 x 8, each summed as a complex
+ * GEMM on the matrix cores.  nx = ny = 0 when the plan is not a lattice.
+ * No reference counterpart. */
 holo_status holo_plan_lattice(const holo_plan *plan, size_t *nx, size_t *ny);
 
 /* Host-only preview of the tiling a plan builds (no device needed): tile
