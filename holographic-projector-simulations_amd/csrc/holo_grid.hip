@@ -401,6 +401,7 @@ constexpr float kRowDev = 28.f;    // largest |k_a^2 - kRowMid|
 // of the cross term.  NaN fails.
 __device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, float ry2,
                                                 float rx2c, float ia, float idf, float kappa) {
+#pragma clang fp contract(off)  // the same decision in every kernel it is inlined into
   const float qmax = Pm + Qm;
   const float round_err = 6e-8f * (2.f + 0.5f * ia * (rx2 + ry2));
   const float trunc_err = 0.0625f * qmax * qmax * qmax * ia * idf * idf * idf * idf;
@@ -516,6 +517,7 @@ struct Grid16Lds {
 template <bool OBL>
 __device__ __forceinline__ bool grid16_far(double vx, double vy, double vz, double inv_lambda,
                                            const GridBounds& gb, float rx2c) {
+#pragma clang fp contract(off)
   const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
   const float rxm = (float)fmax(fabs(vx - gb.cx0), fabs(vx - gb.cx1));
   const float rym = (float)fmax(fabs(vy - gb.cy0), fabs(vy - gb.cy1));
