@@ -213,6 +213,20 @@ def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, sig
     assert rel_l2(ys[1], ref) <= 1e-6
 
 
+@pytest.mark.parametrize("nx, ny", [(16, 32), (16, 16), (32, 32), (8, 8), (48, 96)])
+@pytest.mark.parametrize("M", [1, 65])
+def test_lattice_smallest_tilings(pkg, ctx, oracle, workloads, nx, ny, M, tiles):
+    """One 16 x 32 pair, one 16 x 16 tile, two pairs, one 8 x 8 tile and an odd
+    number of tile rows, against the oracle for one and for 65 targets."""
+    u = workloads.grid(nx, ny, 10e-6, 0.0)
+    amp, phase = weights(27, len(u))
+    v = targets(28, M, 0.3, 2e-3)
+    plan = ctx.plan(u)
+    assert plan.lattice() == (nx, ny)
+    y = plan.transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
+
+
 @pytest.mark.parametrize("half, scale", [(0, 1e-30), (1, 1e-30), (1, 0.0)])
 def test_lattice_tile_pair_weight_scales(pkg, ctx, oracle, workloads, half, scale, tiles):
     """The two 16 x 16 halves of a 16 x 32 tile share one weight scale: one
