@@ -185,6 +185,39 @@ def test_invalid_shape_exits_nonzero():
     assert r.returncode != 0 and "HOLO_SHAPE" in r.stderr
 
 
+DROPIN = os.path.join(ROOT, "holographic-projector-simulations_amd", "_build", "transform_dropin")
+
+
+@pytest.mark.gpu
+def test_cpp_dropin_templates(tmp_path, oracle):
+    """include/holo/transform.hpp used as cuda/main.cu uses the reference's
+    templates (tests/cpp/transform_dropin.cpp): transform_full<Backwards,
+    Reduced, true> returns the padded field of the oracle's sum, time_transform
+    its normalised polar form, and a target in the first source's plane is
+    refused as the reference asserts (cuda/transform.cu:193)."""
+    if not os.path.exists(DROPIN):
+        pytest.skip("drop-in check program not built")
+    r = subprocess.run([DROPIN, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "caught EINVAL" in r.stdout
+    rd = lambda n: np.fromfile(tmp_path / n, "<f8")
+    amp, phase, u, v = rd("x_amp.dat"), rd("x_phase.dat"), rd("u.dat"), rd("v.dat")
+    M = v.size // 3
+    y = rd("y.dat").view(np.complex128)
+    ref = oracle.transform_full(-1, LAM_REF, amp, phase, u, v)
+    assert y.size == 2048 and np.all(y[M:] != 0)  # padded targets sit at the origin
+    assert np.linalg.norm(y[:M] - ref) / np.linalg.norm(ref) <= 1e-12
+    # the padded targets enter the max-normalisation, as in the reference
+    pad = np.zeros((y.size, 3))
+    pad[:M] = v.reshape(-1, 3)
+    full = oracle.transform_full(-1, LAM_REF, amp, phase, u, pad.reshape(-1))
+    ra, rp, _ = oracle.normalize_polar(full)
+    pa, pp = rd("p_amp.dat"), rd("p_phase.dat")
+    assert pa.size == M
+    assert np.max(np.abs(pa - ra[:M])) <= 1e-12
+    assert np.max(np.abs(np.angle(np.exp(1j * (pp - rp[:M]))))) <= 1e-9
+
+
 @pytest.mark.gpu
 def test_reference_wave_mode(tmp_path, oracle):
     """HOLO_PROJECT_PHASE=0 (the reference's PROJECT_PHASE=0 build,
