@@ -82,6 +82,8 @@ struct holo_ctx {
   // staging buffers for host-pointer calls and scratch
   DevBuf amp, phase, u, v, y, w, partial, small, b, pos, ph;
   DevBuf mc_wc, mc_xy, mc_info;  // product-grid Monte Carlo (McLattice)
+  // recorded on the previous stream when holo_set_stream switches streams
+  hipEvent_t switched = nullptr;
 };
 
 struct holo_plan {
@@ -117,6 +119,19 @@ struct holo_plan {
   double g32_cx0 = 0, g32_cx1 = 0, g32_cy0 = 0, g32_cy1 = 0;
   float g32_rho_y = 0;
   DevBuf g32_centre;
+  // Every call rewrites the weight buffers above.  `ran` is recorded after
+  // each call on its stream; a call on another stream waits for it first.
+  mutable hipEvent_t ran = nullptr;
+  mutable hipStream_t ran_on = nullptr;
+  ~holo_plan() {
+    if (ran) {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(device);
+      (void)hipEventDestroy(ran);
+      (void)hipSetDevice(cur);
+    }
+  }
 };
 
 namespace {
@@ -642,14 +657,19 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
     }
   }
   p->uniform_z = uniform_z(uh, N);
-  return plan_upload_tiles(ctx, p, uh);
+  HOLO_TRY(plan_upload_tiles(ctx, p, uh));
+  // the uploads are ordered before a first call on another stream
+  HOLO_HIP(hipEventCreateWithFlags(&p->ran, hipEventDisableTiming));
+  HOLO_HIP(hipEventRecord(p->ran, ctx->stream));
+  p->ran_on = ctx->stream;
+  return HOLO_OK;
 }
 
 // y (device) = sum over the plan's sources; weights already device-resident.
-holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
-                     bool obliquity, double lambda, holo::WeightForm wf, const double* w0,
-                     const double* w1, const double* v, size_t M, double* y) {
-  if (M == 0) return HOLO_OK;
+holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
+                          bool obliquity, double lambda, holo::WeightForm wf,
+                          const double* w0, const double* w1, const double* v, size_t M,
+                          double* y) {
   if (precision == HOLO_FP64) {
     const size_t pe = split_elems_f64(M, p->N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
@@ -763,6 +783,23 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
   return HOLO_OK;
 }
 
+// A plan may serve contexts on different streams.  A call on a stream other
+// than the one of the plan's previous call waits on the device (not the host)
+// until that call has finished with the plan's buffers.
+holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
+                     bool obliquity, double lambda, holo::WeightForm wf, const double* w0,
+                     const double* w1, const double* v, size_t M, double* y) {
+  if (M == 0) return HOLO_OK;
+  if (p->ran && p->ran_on != ctx->stream) HOLO_HIP(hipStreamWaitEvent(ctx->stream, p->ran, 0));
+  const holo_status s = plan_run_body(ctx, p, precision, sign, obliquity, lambda, wf, w0, w1,
+                                      v, M, y);
+  // recorded after a failed call too: some of its kernels may be queued
+  if (!p->ran) HOLO_HIP(hipEventCreateWithFlags(&p->ran, hipEventDisableTiming));
+  HOLO_HIP(hipEventRecord(p->ran, ctx->stream));
+  p->ran_on = ctx->stream;
+  return s;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -818,18 +855,27 @@ void holo_destroy(holo_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->switched) (void)hipEventDestroy(ctx->switched);
   delete ctx;
 }
 
 holo_status holo_set_stream(holo_ctx* ctx, void* stream) {
   HOLO_TRY(use_device(ctx));
-  if (ctx->own_stream && ctx->stream == static_cast<hipStream_t>(stream)) return HOLO_OK;
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  if (ctx->stream == s) return HOLO_OK;
   if (ctx->own_stream) {
     HOLO_HIP(hipStreamSynchronize(ctx->stream));
     HOLO_HIP(hipStreamDestroy(ctx->stream));
     ctx->own_stream = false;
+  } else {
+    // work queued on the previous stream may still use the context's scratch
+    // buffers: work queued on `s` from now on waits for it on the device
+    if (!ctx->switched)
+      HOLO_HIP(hipEventCreateWithFlags(&ctx->switched, hipEventDisableTiming));
+    HOLO_HIP(hipEventRecord(ctx->switched, ctx->stream));
+    HOLO_HIP(hipStreamWaitEvent(s, ctx->switched, 0));
   }
-  ctx->stream = static_cast<hipStream_t>(stream);
+  ctx->stream = s;
   return HOLO_OK;
 }
 

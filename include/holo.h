@@ -80,7 +80,11 @@ const char *holo_last_error(void);
 holo_status holo_create(holo_ctx **ctx, int device, void *stream);
 void holo_destroy(holo_ctx *ctx);
 /* Use `stream` from now on; NULL selects the device's default (null) stream,
- * as torch's default stream is.  A stream the context created is released. */
+ * as torch's default stream is.  A stream the context created is released
+ * (after a host wait for it).  Otherwise work later queued on `stream` waits
+ * on the device for the work queued so far on the previous stream, which may
+ * still use the context's scratch buffers; the host does not wait.  A stream
+ * must stay valid while it is the context's stream. */
 holo_status holo_set_stream(holo_ctx *ctx, void *stream);
 holo_status holo_synchronize(holo_ctx *ctx);
 
@@ -151,7 +155,10 @@ holo_status holo_normalize_polar(holo_ctx *ctx, const double *y, size_t M,
  * A plan keeps the N source positions resident on the device and, for
  * HOLO_FP32_SPLIT, groups them into spatial tiles of `tile` sources
  * (8, 16, 32 or 64; 0 = 64).  No reference counterpart: it replaces the
- * per-call host-to-device copies of cuda/transform.cu:209-213. */
+ * per-call host-to-device copies of cuda/transform.cu:209-213.
+ * Each call rewrites the plan's weight buffers.  A plan may serve several
+ * contexts: a call on a stream other than that of the plan's previous call
+ * waits on the device for that call.  Calls must not overlap on the host. */
 holo_status holo_plan_create(holo_ctx *ctx, holo_plan **plan, const double *u,
                              size_t N, unsigned tile, unsigned flags);
 void holo_plan_destroy(holo_plan *plan);
