@@ -32,9 +32,10 @@ METRIC = "phasor pairs/s (FP64 vs FP32-split @rel err≤1e-6) + ms per adjoint g
 FLOP_PER_PAIR = 29            # cuda/macros.h:71-80
 PEAK_FP64 = 78.6e12           # MI355X FP64 vector (and matrix) peak, FLOP/s
 # FP64 FLOPs per pair that k_superpose_f64 executes (SQ_INSTS_VALU_FLOPS_FP64 x 64
-# per pair, profiles/k_superpose_f64_counters.md) on the chain's planar sources:
-# the hand-written sqrt and sin/cos cost more than the reference's nominal 29.
-COUNTED_FLOP_PER_PAIR = 47.25
+# per pair, profiles/k_superpose_f64_counters.md) on the chain's lattice sources
+# (one plane, rows of one x): the hand-written sqrt and sin/cos cost more than
+# the reference's nominal 29.
+COUNTED_FLOP_PER_PAIR = 45.25
 
 
 def log(*a):

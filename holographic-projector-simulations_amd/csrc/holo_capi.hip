@@ -91,6 +91,7 @@ struct holo_plan {
   size_t N = 0;
   DevBuf u;  // AoS xyz, 3N doubles
   bool uniform_z = false;  // every source has the z of source 0
+  size_t row = 0;          // sources in rows of `row` with one x (0: not)
   // FP32-split tiling
   unsigned tile = 16;
   size_t n_tiles = 0;
@@ -631,6 +632,19 @@ bool uniform_z(const double* u, size_t N) {
   return N > 0;
 }
 
+// Length of the rows the sources form: runs of `row` consecutive sources with
+// one x (the x-slow lattices of cuda/init.h:89-158 give row = ny).  The run
+// length of source 0's x is the candidate; 0 when that is under 4 sources or
+// when any source differs from the first of its row.
+size_t row_length(const double* u, size_t N) {
+  size_t L = 1;
+  while (L < N && u[3 * L] == u[0]) ++L;
+  if (L < 4 || N % L) return 0;
+  for (size_t n = L; n < N; ++n)
+    if (!(u[3 * n] == u[3 * (n - n % L)])) return 0;
+  return L;
+}
+
 holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
                        unsigned tile, bool dev) {
   p->device = ctx->device;
@@ -657,6 +671,7 @@ holo_status plan_build(holo_ctx* ctx, holo_plan* p, const double* u, size_t N,
     }
   }
   p->uniform_z = uniform_z(uh, N);
+  p->row = row_length(uh, N);
   HOLO_TRY(plan_upload_tiles(ctx, p, uh));
   // the uploads are ordered before a first call on another stream
   HOLO_HIP(hipEventCreateWithFlags(&p->ran, hipEventDisableTiming));
@@ -674,7 +689,7 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     const size_t pe = split_elems_f64(M, p->N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, p->uniform_z, w0, w1,
+    HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, p->uniform_z, p->row, w0, w1,
                                         static_cast<const double*>(p->u.ptr),
                                         p->N, v, M, K, reinterpret_cast<double2*>(y),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
@@ -903,6 +918,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.f64_libm = value != 0;
   } else if (n == "f64_planar") {
     t.f64_planar = value != 0;
+  } else if (n == "f64_rows") {
+    t.f64_rows = value != 0;
   } else if (n == "grid") {
     t.grid = value != 0;
   } else if (n == "grid16") {
@@ -927,6 +944,7 @@ long holo_get_option(const char* name) {
   if (n == "target_waves") return t.target_waves;
   if (n == "f64_libm") return t.f64_libm ? 1 : 0;
   if (n == "f64_planar") return t.f64_planar ? 1 : 0;
+  if (n == "f64_rows") return t.f64_rows ? 1 : 0;
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
   if (n == "grid32") return t.grid32 ? 1 : 0;
@@ -962,7 +980,9 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
     // device positions are not scanned: the general path gives the same bits
     const bool uz = !dev && uniform_z(u, N);
-    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, uz, da, dp, du, N, dv, M, K,
+    const size_t row = uz ? row_length(u, N) : 0;
+    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, uz, row, da, dp, du, N, dv, M,
+                                        K,
                                         reinterpret_cast<double2*>(dy),
                                         static_cast<double2*>(ctx->partial.ptr), pe,
                                         ctx->stream));

@@ -29,13 +29,14 @@ struct PairF64 {
 // compiler's sqrt and IEEE division.  Both give the same d and arg bits.
 // OBL: amplitude times the obliquity factor |dz| / d
 // (matlab/holofunctions.m:226-227), r = (a / d) (|dz| / d).
-// pair_f64_d takes dx = vx - ux, dy, dz already formed (dz once per target
-// when every source has the same z); pair_f64 forms them.
+// pair_f64_s takes dxx = dx * dx (rounded), dy and dz already formed (dz once
+// per target when every source has the same z, dxx once per row of sources
+// with one x); pair_f64_d takes dx = vx - ux; pair_f64 forms them all.
 template <int SIGN, int MATH = 0, bool OBL = false>
-__device__ __forceinline__ PairF64 pair_f64_d(double a, double phi, double dx, double dy,
+__device__ __forceinline__ PairF64 pair_f64_s(double a, double phi, double dxx, double dy,
                                               double dz, double K) {
 #pragma clang fp contract(off)
-  const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  const double s = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dxx));
   PairF64 p;
   if constexpr (MATH == 0) {
     double d, h;  // h = (1/d) / 2
@@ -50,6 +51,13 @@ __device__ __forceinline__ PairF64 pair_f64_d(double a, double phi, double dx, d
     if constexpr (OBL) p.r *= __builtin_fabs(dz) / d;
   }
   return p;
+}
+
+template <int SIGN, int MATH = 0, bool OBL = false>
+__device__ __forceinline__ PairF64 pair_f64_d(double a, double phi, double dx, double dy,
+                                              double dz, double K) {
+#pragma clang fp contract(off)
+  return pair_f64_s<SIGN, MATH, OBL>(a, phi, dx * dx, dy, dz, K);
 }
 
 template <int SIGN, int MATH = 0, bool OBL = false>

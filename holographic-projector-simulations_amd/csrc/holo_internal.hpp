@@ -53,6 +53,7 @@ struct Tuning {
   int target_waves = 16384;
   bool f64_libm = false;  // FP64 kernel: device libm sqrt/div/sincos instead of hm_*
   bool f64_planar = true; // FP64 kernel: dz once per target when all sources share one z
+  bool f64_rows = true;   // FP64 kernel: dx^2 once per (target, row) on such sources in rows
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
   bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
   bool grid32 = true;     // ... as 16 x 32 tiles where the columns pair up
@@ -98,8 +99,11 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
 // uniform_z: every source has the z of source 0 (same results, one operation
 // fewer per pair).  `partial` holds partial_cap double2; S source splits need
 // S*M of them.
+// row > 0: the sources form rows of `row` consecutive sources with one x
+// (used with uniform_z).
 hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool uniform_z,
-                                const double* w0, const double* w1, const double* u, size_t N,
+                                size_t row, const double* w0, const double* w1, const double* u,
+                                size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream);

@@ -42,12 +42,16 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units) {
 
 // PZ: every source has the z of source 0, so dz = vz - uz is formed once per
 // target (the same bits as per pair).
-template <int SIGN, int R, int WF, int MATH, bool OBL, bool PZ>
+// ROWS (with PZ): the sources come in rows of `row` consecutive sources that
+// share one x (a lattice in x-slow order, cuda/init.h:89-158), so dx^2 is
+// formed once per (target, row): again the same bits as per pair.
+template <int SIGN, int R, int WF, int MATH, bool OBL, bool PZ, bool ROWS>
 __global__ __launch_bounds__(kBlock) void k_superpose_f64(
     const double* __restrict__ w0, const double* __restrict__ w1,
-    const double* __restrict__ u, uint32_t N, uint32_t chunk,
+    const double* __restrict__ u, uint32_t N, uint32_t chunk, uint32_t row,
     const double* __restrict__ v, uint32_t M, double K,
     double2* __restrict__ out) {
+  static_assert(!ROWS || PZ, "rows need sources in one plane");
   // the sin/cos table, 32 KB, in LDS (5 workgroups per CU, as the VGPRs allow)
   constexpr int kTab = sizeof(kSinCosTab) / sizeof(double);
   static_assert(kTab % kBlock == 0, "table copy");
@@ -72,41 +76,66 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
   double dz0[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) dz0[r] = PZ && N ? vz[r] - u[2] : 0.0;
-  for (uint32_t n = n0; n < n1; ++n) {
-    const double ux = u[3 * (size_t)n];
-    const double uy = u[3 * (size_t)n + 1];
-    const double uz = PZ ? 0.0 : u[3 * (size_t)n + 2];
-    if constexpr (WF == kPolar) {
-      const double a = w0[n];
-      const double phi = w1[n];
+  // sources [a, b); with ROWS their dx^2 per target is dxx
+  auto sources = [&](uint32_t a, uint32_t b, const double(&dxx)[R]) {
+    for (uint32_t n = a; n < b; ++n) {
+      const double ux = ROWS ? 0.0 : u[3 * (size_t)n];
+      const double uy = u[3 * (size_t)n + 1];
+      const double uz = PZ ? 0.0 : u[3 * (size_t)n + 2];
+      if constexpr (WF == kPolar) {
+        const double a = w0[n];
+        const double phi = w1[n];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const PairF64 p = pair_f64_d<SIGN, MATH, OBL>(a, phi, vx[r] - ux, vy[r] - uy,
-                                                      PZ ? dz0[r] : vz[r] - uz, K);
-        accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r], s_tab);
-      }
-    } else {
-      const double wr = w0[2 * (size_t)n];
-      const double wi = w0[2 * (size_t)n + 1];
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const double dx = vx[r] - ux, dy = vy[r] - uy, dz = PZ ? dz0[r] : vz[r] - uz;
-        double d, id;
-        hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
-        if constexpr (OBL) id *= __builtin_fabs(dz) * id;
-        const double arg = (SIGN > 0 ? d : -d) * K;
-        double s, c;
-        if constexpr (MATH == 0) {
-          hm_sincos(arg, s_tab, s, c);
-        } else {
-          sincos(arg, &s, &c);
+        for (int r = 0; r < R; ++r) {
+          const double dy = vy[r] - uy, dz = PZ ? dz0[r] : vz[r] - uz;
+          PairF64 p;
+          if constexpr (ROWS)
+            p = pair_f64_s<SIGN, MATH, OBL>(a, phi, dxx[r], dy, dz, K);
+          else
+            p = pair_f64_d<SIGN, MATH, OBL>(a, phi, vx[r] - ux, dy, dz, K);
+          accumulate_phasor<MATH>(p.r, p.arg, ar[r], ai[r], s_tab);
         }
-        c *= id;
-        s *= id;
-        ar[r] += wr * c - wi * s;
-        ai[r] += wr * s + wi * c;
+      } else {
+        const double wr = w0[2 * (size_t)n];
+        const double wi = w0[2 * (size_t)n + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const double dx = vx[r] - ux, dy = vy[r] - uy, dz = PZ ? dz0[r] : vz[r] - uz;
+          double d, id;
+          hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, ROWS ? dxx[r] : dx * dx)),
+                      d, id);
+          if constexpr (OBL) id *= __builtin_fabs(dz) * id;
+          const double arg = (SIGN > 0 ? d : -d) * K;
+          double s, c;
+          if constexpr (MATH == 0) {
+            hm_sincos(arg, s_tab, s, c);
+          } else {
+            sincos(arg, &s, &c);
+          }
+          c *= id;
+          s *= id;
+          ar[r] += wr * c - wi * s;
+          ai[r] += wr * s + wi * c;
+        }
       }
     }
+  };
+  if constexpr (ROWS) {
+    for (uint32_t n = n0; n < n1;) {
+      const uint32_t e = min(n1, (n / row + 1) * row);  // end of n's row
+      const double ux = u[3 * (size_t)n];
+      double dxx[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double dx = vx[r] - ux;
+        dxx[r] = dx * dx;
+      }
+      sources(n, e, dxx);
+      n = e;
+    }
+  } else {
+    const double none[R] = {};
+    sources(n0, n1, none);
   }
   double2* dst = out + (size_t)blockIdx.y * M;
 #pragma unroll
@@ -140,7 +169,8 @@ hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, doubl
 }
 
 hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool uniform_z,
-                                const double* w0, const double* w1, const double* u, size_t N,
+                                size_t row, const double* w0, const double* w1, const double* u,
+                                size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream) {
@@ -159,14 +189,18 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool un
         with_bool(obliquity, [&](auto obl) {
           constexpr int wfv = cart.value ? kCartesian : kPolar;
           if (tuning().f64_libm) {
-            k_superpose_f64<sg.value, rr.value, wfv, 1, (bool)obl.value, false>
-                <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M, K,
-                                              out);
+            k_superpose_f64<sg.value, rr.value, wfv, 1, (bool)obl.value, false, false>
+                <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, 0u, v,
+                                              (uint32_t)M, K, out);
+          } else if (uniform_z && tuning().f64_planar && row && tuning().f64_rows) {
+            k_superpose_f64<sg.value, rr.value, wfv, 0, (bool)obl.value, true, true>
+                <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, (uint32_t)row, v,
+                                              (uint32_t)M, K, out);
           } else {
             with_bool(uniform_z && tuning().f64_planar, [&](auto pz) {
-              k_superpose_f64<sg.value, rr.value, wfv, 0, (bool)obl.value, (bool)pz.value>
-                  <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, v, (uint32_t)M,
-                                                K, out);
+              k_superpose_f64<sg.value, rr.value, wfv, 0, (bool)obl.value, (bool)pz.value, false>
+                  <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, 0u, v,
+                                                (uint32_t)M, K, out);
             });
           }
         });

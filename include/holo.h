@@ -93,6 +93,9 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * use the device libm sqrt/div/sincos in the FP64 kernel), "f64_planar"
  * (0/1: when every source of a plan, or of a host-pointer transform, has the
  * same z, the FP64 kernel forms dz once per target; same bits; default 1),
+ * "f64_rows" (0/1: with "f64_planar", when the sources also come in rows of
+ * consecutive sources with one x, as lattices in x-slow order do, dx^2 is
+ * formed once per target and row; same bits; default 1),
  * "grid" (0/1: HOLO_FP32_SPLIT on lattice sources uses the matrix-core
  * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice with nx,
  * ny multiples of 16 that kernel uses 16 x 16 tiles; default 1), "grid32"

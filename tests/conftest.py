@@ -47,6 +47,7 @@ def _reset_options():
     yield
     if os.path.exists(holo.LIB_PATH):
         for k, v in (("r_f64", 4), ("r_f32", 2), ("splits", 0), ("target_waves", 16384),
-                     ("f64_libm", 0), ("f64_planar", 1), ("grid", 1), ("grid16", 1),
+                     ("f64_libm", 0), ("f64_planar", 1), ("f64_rows", 1),
+                     ("grid", 1), ("grid16", 1),
                      ("grid32", 1), ("mc_lattice", 1)):
             holo.set_option(k, v)

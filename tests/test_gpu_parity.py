@@ -99,6 +99,41 @@ def test_fp64_uniform_z_path_same_bits(pkg, ctx, oracle, workloads, obliquity):
     assert not np.array_equal(y2, runs[1][1])
 
 
+@pytest.mark.parametrize("splits", [0, 7])
+@pytest.mark.parametrize("r_f64", [1, 4])
+@pytest.mark.parametrize("shape", [(32, 32), (12, 20), (1, 300)])
+def test_fp64_rows_path_same_bits(pkg, ctx, oracle, workloads, shape, r_f64, splits):
+    """Planar sources in rows of one x (lattices in x-slow order; (1, n) is a
+    line, one row) take the dx^2-once-per-row path: the same bits as without it
+    (option f64_rows = 0), also when the source splits cut rows (splits = 7),
+    through a plan (polar and complex weights) and a host-pointer transform;
+    sources with one x moved are not rows."""
+    nx, ny = shape
+    u = workloads.grid(nx, ny, 10e-6, 0.0)
+    g = np.random.default_rng(nx * 1000 + ny)
+    amp, phase = g.random(nx * ny), g.random(nx * ny) * 2 * np.pi
+    gz = g.standard_normal(nx * ny) + 1j * g.standard_normal(nx * ny)
+    v = workloads.grid(24, 40, 7.56e-6, 0.35)
+    pkg.set_option("r_f64", r_f64)
+    pkg.set_option("splits", splits)
+    plan = ctx.plan(u)
+    runs = {}
+    for rows in (1, 0):
+        pkg.set_option("f64_rows", rows)
+        runs[rows] = (plan.transform(amp, phase, v, +1, LAM, pkg.FP64),
+                      plan.transform_complex(gz, v, -1, LAM, pkg.FP64),
+                      ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64))
+    for a, b in zip(runs[1], runs[0]):
+        assert np.array_equal(a, b)
+    assert rel_l2(runs[1][0], oracle.transform_full(+1, LAM, amp, phase, u, v)) <= 1e-12
+    assert rel_l2(runs[1][2], oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-12
+    u2 = u.copy()
+    u2[len(u) // 2, 0] += 1e-7
+    pkg.set_option("f64_rows", 1)
+    y2 = ctx.transform_full(amp, phase, u2, v, +1, LAM, pkg.FP64)
+    assert rel_l2(y2, oracle.transform_full(+1, LAM, amp, phase, u2, v)) <= 1e-12
+
+
 def test_fp64_golden_fixture(pkg, ctx):
     """Committed golden vectors (tests/golden/plain_small_*.dat, written by the
     oracle) of the 32x32 -> 64x64 case, sign +1."""
