@@ -124,7 +124,8 @@ def main():
     sections = set(a.sections.split(","))
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the "
-              "superposition shards by targets with no exchange, see DESIGN.md)",
+              "superposition would shard by targets, each GPU holding every source, "
+              "with no exchange beyond gathering the outputs)",
               file=sys.stderr)
         sys.exit(2)
 
