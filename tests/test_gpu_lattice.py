@@ -271,3 +271,60 @@ def test_lattice_descending_pitch(pkg, ctx, oracle, workloads, flip, tiles):
     y = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT, obliquity=True)
     ref = oracle.transform_full(+1, LAM, amp, phase, u, v, obliquity=True)
     assert rel_l2(y, ref) <= 1e-6
+
+
+def _random_lattice_case(seed):
+    """A seeded random lattice geometry: sides, pitches (not square), lateral
+    offsets, source z, target distance (3 cm to 1 m, log-uniform), wavelength,
+    sign and obliquity."""
+    g = np.random.default_rng(1000 + seed)
+    nx = 16 * int(g.integers(1, 11))
+    ny = (32 if g.random() < 0.5 else 16) * int(g.integers(1, 6))
+    px, py = g.uniform(2e-6, 20e-6, 2)
+    ox, oy = g.uniform(-2e-3, 2e-3, 2)
+    z0 = g.uniform(-1e-2, 1e-2)
+    x = (np.arange(nx) - (nx - 1) / 2.0) * px + ox
+    y = (np.arange(ny) - (ny - 1) / 2.0) * py + oy
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    u = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z0)], 1)
+    z = z0 + float(np.exp(g.uniform(np.log(0.03), np.log(1.0))))
+    spread = g.uniform(1e-3, 0.05) * (z - z0)
+    v = np.zeros((int(g.integers(200, 1500)), 3))
+    v[:, :2] = (g.random((len(v), 2)) - 0.5) * 2 * spread + g.uniform(-spread, spread, 2)
+    v[:, 2] = z
+    lam = g.uniform(400e-9, 700e-9)
+    sign = 1 if g.random() < 0.5 else -1
+    obl = bool(g.random() < 0.3)
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    return u, (nx, ny), amp, phase, v, lam, sign, obl
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_lattice_random_geometries(pkg, ctx, seed, tiles):
+    """Seeded random lattices and target sets across distances, pitches,
+    wavelengths and angles: FP32-split within 1e-6 of FP64 everywhere (the
+    accuracy guard sends what the far-field expansion cannot hold to the exact
+    path)."""
+    u, shape, amp, phase, v, lam, sign, obl = _random_lattice_case(seed)
+    plan = ctx.plan(u)
+    assert plan.lattice() == shape
+    y64 = plan.transform(amp, phase, v, sign, lam, pkg.FP64, obliquity=obl)
+    y32 = plan.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
+    assert rel_l2(y32, y64) <= 1e-6
+    if tiles == 32:
+        # the tiled kernel on the same lattice, and on the lattice with every
+        # source moved by up to half a pitch (the reference's -r jitter)
+        pkg.set_option("grid", 0)
+        yt = plan.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
+        pkg.set_option("grid", 1)
+        assert rel_l2(yt, y64) <= 1e-6
+        g = np.random.default_rng(seed)
+        du = np.diff(np.unique(u[:, 0]))[0], np.diff(np.unique(u[:, 1]))[0]
+        uj = u.copy()
+        uj[:, 0] += (g.random(len(u)) - 0.5) * 0.975 * du[0]
+        uj[:, 1] += (g.random(len(u)) - 0.5) * 0.975 * du[1]
+        pj = ctx.plan(uj)
+        assert pj.lattice() == (0, 0)
+        yj64 = pj.transform(amp, phase, v, sign, lam, pkg.FP64, obliquity=obl)
+        yj32 = pj.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
+        assert rel_l2(yj32, yj64) <= 1e-6
