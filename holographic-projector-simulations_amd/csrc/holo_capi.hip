@@ -633,13 +633,15 @@ bool uniform_z(const double* u, size_t N) {
 }
 
 // Length of the rows the sources form: runs of `row` consecutive sources with
-// one x (the x-slow lattices of cuda/init.h:89-158 give row = ny).  The run
+// one x, the last run possibly shorter (the x-slow lattices of
+// cuda/init.h:89-158 give row = ny, and the copies of source 0 that fill a
+// plane whose point count is not nx * ny form the short last run).  The run
 // length of source 0's x is the candidate; 0 when that is under 4 sources or
-// when any source differs from the first of its row.
+// when any source differs in x from the first of its row.
 size_t row_length(const double* u, size_t N) {
   size_t L = 1;
   while (L < N && u[3 * L] == u[0]) ++L;
-  if (L < 4 || N % L) return 0;
+  if (L < 4) return 0;
   for (size_t n = L; n < N; ++n)
     if (!(u[3 * n] == u[3 * (n - n % L)])) return 0;
   return L;

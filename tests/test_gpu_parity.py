@@ -101,18 +101,21 @@ def test_fp64_uniform_z_path_same_bits(pkg, ctx, oracle, workloads, obliquity):
 
 @pytest.mark.parametrize("splits", [0, 7])
 @pytest.mark.parametrize("r_f64", [1, 4])
-@pytest.mark.parametrize("shape", [(32, 32), (12, 20), (1, 300)])
+@pytest.mark.parametrize("shape", [(32, 32, 0), (12, 20, 0), (1, 300, 0), (13, 21, 5)])
 def test_fp64_rows_path_same_bits(pkg, ctx, oracle, workloads, shape, r_f64, splits):
     """Planar sources in rows of one x (lattices in x-slow order; (1, n) is a
-    line, one row) take the dx^2-once-per-row path: the same bits as without it
-    (option f64_rows = 0), also when the source splits cut rows (splits = 7),
-    through a plan (polar and complex weights) and a host-pointer transform;
-    sources with one x moved are not rows."""
-    nx, ny = shape
+    line, one row; (13, 21, 5) ends with 5 copies of source 0, as the
+    reference's plane generator pads a point count that is not nx * ny,
+    cuda/init.h:153-157) take the dx^2-once-per-row path: the same bits as
+    without it (option f64_rows = 0), also when the source splits cut rows
+    (splits = 7), through a plan (polar and complex weights) and a
+    host-pointer transform; sources with one x moved are not rows."""
+    nx, ny, pad = shape
     u = workloads.grid(nx, ny, 10e-6, 0.0)
+    u = np.concatenate([u, np.repeat(u[:1], pad, 0)])
     g = np.random.default_rng(nx * 1000 + ny)
-    amp, phase = g.random(nx * ny), g.random(nx * ny) * 2 * np.pi
-    gz = g.standard_normal(nx * ny) + 1j * g.standard_normal(nx * ny)
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    gz = g.standard_normal(len(u)) + 1j * g.standard_normal(len(u))
     v = workloads.grid(24, 40, 7.56e-6, 0.35)
     pkg.set_option("r_f64", r_f64)
     pkg.set_option("splits", splits)
