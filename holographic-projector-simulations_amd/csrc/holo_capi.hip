@@ -110,6 +110,9 @@ struct holo_plan {
   // 16 x 16 tiles of the same lattice when its pitches are uniform and nx, ny
   // are multiples of 16 (g16_tiles = 0 otherwise)
   size_t g16_tiles = 0;
+  // sources after a padded lattice, summed exactly: their slots follow the
+  // tiles' in g16_perm and g16_wc
+  size_t g16_tail = 0;
   double g16_px = 0, g16_py = 0;
   double g16_cx0 = 0, g16_cx1 = 0, g16_cy0 = 0, g16_cy1 = 0;
   float g16_rho_x = 0, g16_rho_y = 0;
@@ -408,11 +411,14 @@ struct HostGrid {
   double cx0_32 = INFINITY, cx1_32 = -INFINITY, cy0_32 = INFINITY, cy1_32 = -INFINITY;
   float rho32_y = 0;
   std::vector<double4> centre32;
+  std::vector<int32_t> tail;  // sources after the lattice (padded lattices)
 };
 
-void build_grid16(const double* u, HostGrid& g) {
+// pad: sides that are not multiples of 16 are padded to the next multiple
+// with weight-0 slots (perm -1) at the lattice points beyond the edge.
+void build_grid16(const double* u, HostGrid& g, bool pad = false) {
   const size_t nx = g.nx, ny = g.ny;
-  if (nx % 16 || ny % 16) return;
+  if (!pad && (nx % 16 || ny % 16)) return;
   const double x0 = u[0], y0 = u[1];
   const double px = (u[3 * (nx - 1) * ny] - x0) / (double)(nx - 1);
   const double py = (u[3 * (ny - 1) + 1] - y0) / (double)(ny - 1);
@@ -426,7 +432,7 @@ void build_grid16(const double* u, HostGrid& g) {
   g.rho16_x = std::nextafter((float)(7.5 * std::fabs(px)), INFINITY);
   g.rho16_y = std::nextafter((float)(7.5 * std::fabs(py)), INFINITY);
   const float2 rho{g.rho16_x, g.rho16_y};
-  const size_t TI = nx / 16, TJ = ny / 16;
+  const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16;
   g.centre16.reserve(TI * TJ);
   g.perm16.reserve(256 * TI * TJ);
   for (size_t I = 0; I < TI; ++I)
@@ -438,9 +444,9 @@ void build_grid16(const double* u, HostGrid& g) {
       g.cx1_16 = std::max(g.cx1_16, cx);
       g.cy0_16 = std::min(g.cy0_16, cy);
       g.cy1_16 = std::max(g.cy1_16, cy);
-      for (int a = 0; a < 16; ++a)
-        for (int b = 0; b < 16; ++b)
-          g.perm16.push_back((int32_t)((16 * I + a) * ny + 16 * J + b));
+      for (size_t a = 16 * I; a < 16 * I + 16; ++a)
+        for (size_t b = 16 * J; b < 16 * J + 16; ++b)
+          g.perm16.push_back(a < nx && b < ny ? (int32_t)(a * ny + b) : -1);
     }
   if (TJ % 2) return;
   g.rho32_y = std::nextafter((float)(15.5 * std::fabs(py)), INFINITY);
@@ -514,6 +520,35 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   return true;
 }
 
+// A uniform lattice that build_grid does not take: sides of at least 16 that
+// are not both multiples of 8, and possibly followed by fewer than max(nx, ny)
+// other sources (the reference's plane generator fills a point count that is
+// not nx * ny with copies of source 0, cuda/init.h:153-157).  Only the 16-row
+// tiles are built, over the lattice padded to multiples of 16; every target
+// sums the remaining sources (tail) exactly.
+bool build_grid16_padded(const double* u, size_t N, HostGrid& g) {
+  g = HostGrid();
+  if (N < 256) return false;
+  size_t ny = 1;
+  while (ny < N && u[3 * ny] == u[0]) ++ny;
+  if (ny < 16 || ny == N) return false;
+  const size_t nx = N / ny, body = nx * ny;
+  if (nx < 16 || N - body >= std::max(nx, ny)) return false;
+  const double z = u[2];
+  for (size_t n = 0; n < body; ++n) {
+    const size_t i = n / ny, j = n % ny;
+    if (u[3 * n] != u[3 * i * ny] || u[3 * n + 1] != u[3 * j + 1] || u[3 * n + 2] != z)
+      return false;
+  }
+  g.nx = nx;
+  g.ny = ny;
+  g.z = z;
+  build_grid16(u, g, true);
+  if (g.centre16.empty()) return false;
+  for (size_t n = body; n < N; ++n) g.tail.push_back((int32_t)n);
+  return true;
+}
+
 // Builds and uploads the spatial tiling for k_superpose_f32s (u_host: the
 // plan's positions on the host, or NULL to read them back from the device).
 // Waits for the copies before returning.
@@ -552,6 +587,50 @@ holo_status plan_build_tiles(holo_ctx* ctx, const holo_plan* p, const double* u_
   return HOLO_OK;
 }
 
+// The 16-row tiles of grid (and their tail), if any, onto the device.
+holo_status upload_grid16(holo_ctx* ctx, holo_plan* p, const HostGrid& grid) {
+  const size_t n16 = grid.centre16.size(), nt = grid.tail.size();
+  if (!n16) return HOLO_OK;
+  if (256 * n16 + nt > kMaxCount) return fail(HOLO_EINVAL, "too many tile slots");
+  p->g16_tiles = n16;
+  p->g16_tail = nt;
+  p->g16_px = grid.px;
+  p->g16_py = grid.py;
+  p->g16_cx0 = grid.cx0_16;
+  p->g16_cx1 = grid.cx1_16;
+  p->g16_cy0 = grid.cy0_16;
+  p->g16_cy1 = grid.cy1_16;
+  p->g16_rho_x = grid.rho16_x;
+  p->g16_rho_y = grid.rho16_y;
+  p->grid_z = grid.z;
+  HOLO_TRY(ensure(p->g16_centre, n16 * sizeof(double4), ctx->device));
+  HOLO_TRY(ensure(p->g16_perm, (256 * n16 + nt) * sizeof(int32_t), ctx->device));
+  HOLO_TRY(ensure(p->g16_wc, (256 * n16 + nt) * sizeof(float2), ctx->device));
+  HOLO_TRY(ensure(p->g16_wq, 6 * 64 * n16 * sizeof(uint4), ctx->device));
+  HOLO_TRY(ensure(p->g16_ts, n16 * sizeof(float), ctx->device));
+  HOLO_HIP(hipMemcpyAsync(p->g16_centre.ptr, grid.centre16.data(), n16 * sizeof(double4),
+                          hipMemcpyHostToDevice, ctx->stream));
+  HOLO_HIP(hipMemcpyAsync(p->g16_perm.ptr, grid.perm16.data(), 256 * n16 * sizeof(int32_t),
+                          hipMemcpyHostToDevice, ctx->stream));
+  if (nt)
+    HOLO_HIP(hipMemcpyAsync(static_cast<int32_t*>(p->g16_perm.ptr) + 256 * n16,
+                            grid.tail.data(), nt * sizeof(int32_t), hipMemcpyHostToDevice,
+                            ctx->stream));
+  const size_t n32 = grid.centre32.size();
+  if (n32) {
+    p->g32_tiles = n32;
+    p->g32_cx0 = grid.cx0_32;
+    p->g32_cx1 = grid.cx1_32;
+    p->g32_cy0 = grid.cy0_32;
+    p->g32_cy1 = grid.cy1_32;
+    p->g32_rho_y = grid.rho32_y;
+    HOLO_TRY(ensure(p->g32_centre, n32 * sizeof(double4), ctx->device));
+    HOLO_HIP(hipMemcpyAsync(p->g32_centre.ptr, grid.centre32.data(), n32 * sizeof(double4),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
+  return HOLO_OK;
+}
+
 holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host) {
   HostGrid grid;
   if (p->tile == 64 && build_grid(u_host, p->N, grid)) {
@@ -580,39 +659,11 @@ holo_status plan_upload_tiles(holo_ctx* ctx, holo_plan* p, const double* u_host)
                             hipMemcpyHostToDevice, ctx->stream));
     HOLO_HIP(hipMemcpyAsync(p->g_perm.ptr, grid.perm.data(), 64 * nt * sizeof(int32_t),
                             hipMemcpyHostToDevice, ctx->stream));
-    const size_t n16 = grid.centre16.size();
-    if (n16) {
-      p->g16_tiles = n16;
-      p->g16_px = grid.px;
-      p->g16_py = grid.py;
-      p->g16_cx0 = grid.cx0_16;
-      p->g16_cx1 = grid.cx1_16;
-      p->g16_cy0 = grid.cy0_16;
-      p->g16_cy1 = grid.cy1_16;
-      p->g16_rho_x = grid.rho16_x;
-      p->g16_rho_y = grid.rho16_y;
-      HOLO_TRY(ensure(p->g16_centre, n16 * sizeof(double4), ctx->device));
-      HOLO_TRY(ensure(p->g16_perm, 256 * n16 * sizeof(int32_t), ctx->device));
-      HOLO_TRY(ensure(p->g16_wc, 256 * n16 * sizeof(float2), ctx->device));
-      HOLO_TRY(ensure(p->g16_wq, 6 * 64 * n16 * sizeof(uint4), ctx->device));
-      HOLO_TRY(ensure(p->g16_ts, n16 * sizeof(float), ctx->device));
-      HOLO_HIP(hipMemcpyAsync(p->g16_centre.ptr, grid.centre16.data(), n16 * sizeof(double4),
-                              hipMemcpyHostToDevice, ctx->stream));
-      HOLO_HIP(hipMemcpyAsync(p->g16_perm.ptr, grid.perm16.data(),
-                              256 * n16 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-      const size_t n32 = grid.centre32.size();
-      if (n32) {
-        p->g32_tiles = n32;
-        p->g32_cx0 = grid.cx0_32;
-        p->g32_cx1 = grid.cx1_32;
-        p->g32_cy0 = grid.cy0_32;
-        p->g32_cy1 = grid.cy1_32;
-        p->g32_rho_y = grid.rho32_y;
-        HOLO_TRY(ensure(p->g32_centre, n32 * sizeof(double4), ctx->device));
-        HOLO_HIP(hipMemcpyAsync(p->g32_centre.ptr, grid.centre32.data(), n32 * sizeof(double4),
-                                hipMemcpyHostToDevice, ctx->stream));
-      }
-    }
+    HOLO_TRY(upload_grid16(ctx, p, grid));
+  } else if (p->tile == 64 && build_grid16_padded(u_host, p->N, grid)) {
+    p->grid_nx = grid.nx;
+    p->grid_ny = grid.ny;
+    HOLO_TRY(upload_grid16(ctx, p, grid));
   }
   // host vectors are freed on return: wait for the copies
   HOLO_HIP(hipStreamSynchronize(ctx->stream));
@@ -700,8 +751,8 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     const size_t nt = p->g16_tiles;
     const bool pairs = p->g32_tiles && holo::tuning().grid32;
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g16_perm.ptr),
-                                       256 * nt, static_cast<float2*>(p->g16_wc.ptr),
-                                       ctx->stream));
+                                       256 * nt + p->g16_tail,
+                                       static_cast<float2*>(p->g16_wc.ptr), ctx->stream));
     HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
                                          pairs, static_cast<uint4*>(p->g16_wq.ptr),
                                          static_cast<float*>(p->g16_ts.ptr), ctx->stream));
@@ -719,6 +770,9 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     g16.cy0 = p->g16_cy0;
     g16.cy1 = p->g16_cy1;
     g16.rho_y = p->g16_rho_y;
+    g16.tail_perm = g16.perm + 256 * nt;
+    g16.tail_wc = static_cast<const float2*>(p->g16_wc.ptr) + 256 * nt;
+    g16.n_tail = (uint32_t)p->g16_tail;
     holo::GridTiles gt = g16;  // the tiles of the launch: pairs of them, or themselves
     if (pairs) {
       gt.centre = static_cast<const double4*>(p->g32_centre.ptr);

@@ -693,7 +693,9 @@ __device__ __forceinline__ void grid16_tiles(
 // and gb.  NKC = 2: the 16 x 32 tiles of centre and gb when every target of
 // the wave passes their far-field bounds; otherwise the wave takes the same
 // sources as the 16 x 16 tiles 2 t0 .. 2 t1 - 1 (centre16, gb16), whose
-// looser bounds send fewer (target, tile) pairs to the exact path.
+// looser bounds send fewer (target, tile) pairs to the exact path.  The
+// n_tail sources after a padded lattice (tail_perm, tail_wc) are summed
+// exactly by every target in the first source split.
 template <int SIGN, bool OBL, int NKC>
 __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     const double4* __restrict__ centre, const int32_t* __restrict__ perm,
@@ -701,6 +703,7 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
     uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
     double py, GridBounds gb, const double4* __restrict__ centre16, GridBounds gb16,
+    const int32_t* __restrict__ tail_perm, const float2* __restrict__ tail_wc, uint32_t n_tail,
     double2* __restrict__ out) {
   __shared__ Grid16Lds lds;
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -726,6 +729,11 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
     grid16_tiles<SIGN, OBL, 1>(lds, centre16, perm, u, wc, wq, tscale, 2 * t0, 2 * t1, vx, vy,
                                vz, inv_lambda, px, py, gb16, far16, ex_r, ex_i);
   }
+  // in blocks of 256 sources, so that each FP32 partial sum stays short
+  if (blockIdx.y == 0)
+    for (uint32_t s0 = 0; s0 < n_tail; s0 += 256)
+      exact_tile_pairs<SIGN, OBL>(u, tail_perm + s0, tail_wc + s0, (int)min(256u, n_tail - s0),
+                                  vx, vy, vz, inv_lambda, ex_r, ex_i);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -747,6 +755,7 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
   if (nkc != 1 && nkc != 2) return hipErrorInvalidValue;
   if (nkc == 2 && (halves.n_tiles != 2 * gt.n_tiles || !halves.centre))
     return hipErrorInvalidValue;
+  if (gt.n_tail && (!gt.tail_perm || !gt.tail_wc)) return hipErrorInvalidValue;
   int S = grid_splits(M, gt.n_tiles);
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = gt.n_tiles ? (gt.n_tiles + S - 1) / S : 0;
@@ -761,11 +770,11 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
       if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, halves.centre, gb16, out);
+            gt.px, gt.py, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, gt.centre, gb, out);
+            gt.px, gt.py, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
     });
   });
   hipError_t e = hipGetLastError();

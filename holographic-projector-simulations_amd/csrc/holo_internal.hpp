@@ -44,6 +44,10 @@ struct GridTiles {
   double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, z = 0;
   float rho_x = 0, rho_y = 0;
   double px = 0, py = 0;  // pitches of a uniform lattice (16 x 16 tiles)
+  // sources after a padded lattice (16 x 16 tiles): slots, weights, count
+  const int32_t* tail_perm = nullptr;
+  const float2* tail_wc = nullptr;
+  uint32_t n_tail = 0;
 };
 
 struct Tuning {

@@ -172,10 +172,14 @@ holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
                            double *max_offset);
 
 /* Lattice the plan detected: the sources are a product grid x_i * y_j at one
- * z, in the order n = i*ny + j, with nx and ny multiples of 8 (the
- * reference's plane generators, cuda/init.h:89-158).  HOLO_FP32_SPLIT then
+ * z, in the order n = i*ny + j (the reference's plane generators,
+ * cuda/init.h:89-158), either with nx and ny multiples of 8, or uniform
+ * (x_0 + i p_x, y_0 + j p_y) with nx, ny >= 16 and followed by fewer than
+ * max(nx, ny) other sources (the generator fills a point count that is not
+ * nx * ny with copies of source 0, cuda/init.h:153-157).  HOLO_FP32_SPLIT then
  * runs the lattice kernel: tiles of 16 x 32 or 16 x 16 sources on a uniform
- * lattice with nx, ny multiples of 16, else 8 x 8, each summed as a complex
+ * lattice (padded with weight-0 slots to multiples of 16, the sources after
+ * it summed exactly by every target), else 8 x 8, each summed as a complex
  * GEMM on the matrix cores.  nx = ny = 0 when the plan is not a lattice.
  * No reference counterpart. */
 holo_status holo_plan_lattice(const holo_plan *plan, size_t *nx, size_t *ny);

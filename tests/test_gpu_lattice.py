@@ -53,7 +53,13 @@ def targets(seed, M, z, half=3e-3):
 
 def test_lattice_detection(pkg, ctx, workloads):
     assert ctx.plan(workloads.grid(64, 48, 10e-6, 0.0)).lattice() == (64, 48)
-    assert ctx.plan(workloads.grid(60, 48, 10e-6, 0.0)).lattice() == (0, 0)  # 60 % 8
+    # 60 is not a multiple of 8: a uniform lattice, padded for the 16-row tiles
+    assert ctx.plan(workloads.grid(60, 48, 10e-6, 0.0)).lattice() == (60, 48)
+    g = np.random.default_rng(0)
+    xs = np.sort(g.random(60)) * 1e-3
+    X, Y = np.meshgrid(xs, np.arange(48) * 1e-5, indexing="ij")
+    un = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    assert ctx.plan(un).lattice() == (0, 0)  # not uniform, 60 % 8
     u = workloads.grid(64, 48, 10e-6, 0.0)
     assert ctx.plan(u[np.random.default_rng(0).permutation(len(u))]).lattice() == (0, 0)
     ux = u.reshape(64, 48, 3).transpose(1, 0, 2).reshape(-1, 3)  # x fast
@@ -328,3 +334,45 @@ def test_lattice_random_geometries(pkg, ctx, seed, tiles):
         yj64 = pj.transform(amp, phase, v, sign, lam, pkg.FP64, obliquity=obl)
         yj32 = pj.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
         assert rel_l2(yj32, yj64) <= 1e-6
+
+
+def _reference_plane(n, width, z, aspect=1.0, offset=(0.0, 0.0)):
+    """The reference's plane generator without jitter (cuda/init.h:96-157):
+    x = floor(sqrt(n aspect)) rows, y = n // x columns, x slow, and the
+    n - x y points left over set to point 0."""
+    x = int(np.sqrt(n * aspect))
+    y = n // x
+    height = width / aspect
+    dx, dy = width / x, height / y
+    u = np.empty((n, 3))
+    i, j = np.meshgrid(np.arange(x), np.arange(y), indexing="ij")
+    u[:x * y, 0] = (i * dx - (width - dx) / 2 + offset[0]).ravel()
+    u[:x * y, 1] = (j * dy - (height - dy) / 2 + offset[1]).ravel()
+    u[:x * y, 2] = z
+    u[x * y:] = u[0]
+    return u, (x, y)
+
+
+@pytest.mark.parametrize("n, aspect", [(1000, 1.0), (61 * 45, 61 / 45), (1024 * 1080 // 64, 1.0),
+                                       (40 * 50 + 17, 0.8)])
+@pytest.mark.parametrize("sign", [+1, -1])
+def test_lattice_padded_planes(pkg, ctx, oracle, n, aspect, sign, tiles):
+    """Planes of the reference's generator whose sides are not multiples of 16,
+    ending with the copies of point 0 that fill n: 16-row tiles over the
+    lattice padded with weight-0 slots, the copies summed exactly (with
+    "grid16" off: the tiled kernel)."""
+    u, (x, y) = _reference_plane(n, 1.2e-3, 0.0, aspect, (1e-4, -2e-4))
+    g = np.random.default_rng(n)
+    amp, phase = g.random(n), g.random(n) * 2 * np.pi
+    v = targets(n % 97, 500, 0.3)
+    plan = ctx.plan(u)
+    assert plan.lattice() == (x, y)
+    y32 = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=(sign < 0))
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=(sign < 0))
+    assert rel_l2(y32, ref) <= 1e-6
+    # the copies of point 0 are sources too
+    amp2 = amp.copy()
+    amp2[x * y:] = 0.0
+    y2 = plan.transform(amp2, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=(sign < 0))
+    if n > x * y:
+        assert rel_l2(y2, y32) > 1e-4
