@@ -461,8 +461,11 @@ __global__ __launch_bounds__(kBlock) void k_mc(
     if (lat.info && !lat.info[1]) ny = lat.info[0];
   }
   const double inv_ny = ny ? 1.0 / (double)ny : 0.0;
+  // sources after the last whole row (a padded plane's copies of source 0)
+  // come from the packed records
+  const uint32_t body = ny ? (N / ny) * ny : 0u;
   const auto source = [&](uint32_t n) -> double4 {
-    if (!ny) return pos[n];
+    if (n >= body) return pos[n];
     uint32_t i = (uint32_t)((double)n * inv_ny);
     if (i * ny > n) --i;
     else if ((i + 1) * ny <= n) ++i;
@@ -647,7 +650,9 @@ hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
 
 // Row length of an x-slow product grid: the first n with x_n != x_0, among
 // n < min(N, kLatticeMaxRow); N when all x are equal and N <= kLatticeMaxRow;
-// 0 (not a grid) otherwise or when it does not divide N.  Clears info[1].
+// 0 (not a grid) otherwise.  The sources after the last whole row (a padded
+// plane's copies of source 0, cuda/init.h:153-157) are not part of the grid.
+// Clears info[1].
 __global__ __launch_bounds__(kBlock) void k_lattice_row(const double* __restrict__ u,
                                                         uint32_t N, uint32_t* __restrict__ info) {
   __shared__ uint32_t s_min[kBlock / 64];
@@ -665,22 +670,20 @@ __global__ __launch_bounds__(kBlock) void k_lattice_row(const double* __restrict
   if (threadIdx.x == 0) {
     uint32_t f = lim;
     for (int w = 0; w < kBlock / 64; ++w) f = min(f, s_min[w]);
-    uint32_t ny = f < lim ? f : (N <= kLatticeMaxRow ? N : 0u);
-    if (ny && N % ny) ny = 0;
-    info[0] = ny;
+    info[0] = f < lim ? f : (N <= kLatticeMaxRow ? N : 0u);
     info[1] = 0;
   }
 }
 
-// Checks u[n] == (x of row n / ny, y of column n % ny, z_0) exactly, writing
-// 1 to info[1] where it fails (plain stores of the same value), and fills the
-// tables described with McLattice.
+// Checks u[n] == (x of row n / ny, y of column n % ny, z_0) exactly for the
+// sources of the whole rows, writing 1 to info[1] where it fails (plain stores
+// of the same value), and fills the tables described with McLattice.
 __global__ __launch_bounds__(kBlock) void k_lattice_check(const double* __restrict__ u,
                                                           uint32_t N, uint32_t* __restrict__ info,
                                                           double* __restrict__ xy) {
   const uint32_t ny = info[0];
   const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
-  if (!ny || n >= N) return;
+  if (!ny || n >= (N / ny) * ny) return;
   const uint32_t i = n / ny, j = n - i * ny;
   const double x = u[3 * (size_t)n], y = u[3 * (size_t)n + 1], z = u[3 * (size_t)n + 2];
   if (!(x == u[3 * (size_t)i * ny] && y == u[3 * (size_t)j + 1] && z == u[2])) info[1] = 1;

@@ -65,6 +65,24 @@ def test_mc_lattice_bitwise(pkg, ctx, workloads, shape, sign):
     assert np.array_equal(y1, y0)
 
 
+@pytest.mark.parametrize("shape, pad", [((31, 32), 8), ((37, 29), 28), ((64, 48), 1)])
+def test_mc_lattice_padded_plane(pkg, ctx, oracle, workloads, shape, pad):
+    """A lattice followed by copies of source 0 (the reference's plane
+    generator, cuda/init.h:153-157): the copies come from the packed records,
+    the same bits as the generic gather, and they are drawn."""
+    u = workloads.grid(shape[0], shape[1], 10e-6, 0.0)
+    u = np.concatenate([u, np.repeat(u[:1], pad, 0)])
+    amp, phase = weights(11, len(u))
+    v = targets(12, 600)
+    y1, y0 = both(pkg, ctx, amp, phase, u, v, 1024, -1)
+    assert np.array_equal(y1, y0)
+    amp2 = amp.copy()
+    amp2[-pad:] = 0.0
+    pkg.set_option("mc_lattice", 1)
+    y2 = ctx.transform_mc(amp2, phase, u, v, 1024, 7, -1, LAM, pkg.FP32_SPLIT)
+    assert not np.array_equal(y2, y1)
+
+
 def test_mc_lattice_nonuniform_offcentre_obliquity(pkg, ctx):
     g = np.random.default_rng(3)
     u = product_grid(np.sort(g.random(40)) * 1e-3 + 2e-4, np.sort(g.random(24)) * 1e-3 - 1e-3, 0.01)
