@@ -114,6 +114,10 @@ def main():
                          "(32: 16 x 32 tiles, options grid16 and grid32; 16: 16 x 16 tiles, "
                          "grid32 = 0; 8: the 8 x 8 kernel, grid16 = 0; 0: the tiled kernel "
                          "for arbitrary point sets, grid = 0)")
+    ap.add_argument("--source-points", type=int, default=0,
+                    help="fp32 and mc sections: this many sources from the reference's plane "
+                         "generator (10.24 mm wide; 1105920 = 1024 x 1080, a 1051 x 1052 lattice "
+                         "and 268 copies of point 0) instead of the 1024^2 lattice")
     ap.add_argument("--target-z", type=float, default=None,
                     help="distance of the 1024^2 target plane of the fp32 and mc sections "
                          "(default 0.35 m); nearer planes exercise the lattice kernel's "
@@ -174,7 +178,9 @@ def main():
     # ---------------- FP32-split vs FP64, Monte Carlo, adjoint ---------------
     f64 = None
     if sections & {"fp32", "mc"}:
-        p3 = wl.plane_1024(0).stages[0]
+        p3 = (wl.plane_points(0, a.source_points) if a.source_points else wl.plane_1024(0)).stages[0]
+        src_desc = (f"{a.source_points} points of the reference's plane generator"
+                    if a.source_points else "1024^2")
         if a.target_z is not None:
             p3.v[:, 2] = a.target_z
         amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
@@ -182,14 +188,14 @@ def main():
         y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
         if "fp32" in sections:
             y32 = torch.empty_like(y64)
-            log("[bench] plane 1024^2: FP64")
+            log(f"[bench] plane {src_desc}: FP64")
             torch.cuda.synchronize(dev)
             t0 = time.perf_counter()
             plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
             torch.cuda.synchronize(dev)
             t64 = time.perf_counter() - t0
             f64 = torch.view_as_complex(y64).cpu().numpy()
-            log("[bench] plane 1024^2: FP32-split")
+            log(f"[bench] plane {src_desc}: FP32-split")
             plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
             torch.cuda.synchronize(dev)
             reps = 3
@@ -200,7 +206,7 @@ def main():
             t32 = (time.perf_counter() - t0) / reps
             f32 = torch.view_as_complex(y32).cpu().numpy()
             out["fp32_split"] = {
-                "workload": f"1024^2 -> 1024^2 (z 0 -> {p3.v[0, 2]:g} m, 10 / 7.56 um, amp U[0,1), "
+                "workload": f"{src_desc} -> 1024^2 (z 0 -> {p3.v[0, 2]:g} m, 10 / 7.56 um, amp U[0,1), "
                             "phase U[0,2pi)), " + (
                                 f"lattice kernel, {min(a.lattice_tiles, 16)}x{a.lattice_tiles} tiles"
                                 if a.lattice_tiles else "tiled kernel, tiles of 64"),
@@ -227,7 +233,7 @@ def main():
                                "gflops_29": K * v3.shape[0] / tk * FLOP_PER_PAIR / 1e9}
                     if f64 is not None:
                         mc[key]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
-            out["monte_carlo"] = {"workload": "same 1024^2 -> 1024^2, FP32-split, seed 0; K draws "
+            out["monte_carlo"] = {"workload": f"same {src_desc} -> 1024^2, FP32-split, seed 0; K draws "
                                               "per target, or per group of 1024 targets (batched); "
                                               "expected rel. error sqrt(N/K) for a random-phase field",
                                   "K": mc}

@@ -99,6 +99,38 @@ def chain_xyz(seed: int = 0, n_x: int = 512, n_y: int = 1024, n_z: int = 512) ->
                     [Stage(amp, phase, u, v, +1), Stage(None, None, v, w, -1)])
 
 
+def reference_plane(n: int, width: float, z: float, aspect: float = 1.0,
+                    offset=(0.0, 0.0)) -> np.ndarray:
+    """(n, 3) positions from the reference's plane generator without jitter
+    (cuda/init.h:96-157): x = floor(sqrt(n aspect)) rows of y = n // x points,
+    x slow, centred on offset, pitches width / x and (width / aspect) / y; the
+    n - x y points left over are copies of point 0."""
+    x = int(np.sqrt(n * aspect))
+    y = n // x
+    height = width / aspect
+    dx, dy = width / x, height / y
+    u = np.empty((n, 3))
+    i, j = np.meshgrid(np.arange(x), np.arange(y), indexing="ij")
+    u[:x * y, 0] = (i * dx - (width - dx) / 2 + offset[0]).ravel()
+    u[:x * y, 1] = (j * dy - (height - dy) / 2 + offset[1]).ravel()
+    u[:x * y, 2] = z
+    u[x * y:] = u[0]
+    return u
+
+
+def plane_points(seed: int = 0, n_src: int = 1024 * 1080, n_tgt: int = 1024) -> Workload:
+    """n_src sources from the reference's plane generator, 10.24 mm wide at z=0
+    (amp U[0,1), phase U[0,2pi)) -> n_tgt^2 targets (z=0.35 m, 7.56 um), sign +1;
+    n_src = 1024 * 1080 is the object count of the reference's MATLAB
+    comparison (py/experiment.py:179)."""
+    g = rng(seed)
+    amp = g.random(n_src)
+    phase = g.random(n_src) * 2 * np.pi
+    return Workload(f"plane_{n_src}_points_to_{n_tgt}^2",
+                    [Stage(amp, phase, reference_plane(n_src, 1024 * 10e-6, 0.0),
+                           grid(n_tgt, n_tgt, 7.56e-6, 0.35), +1)])
+
+
 def plane_1024(seed: int = 0, n_src: int = 1024, n_tgt: int = 1024) -> Workload:
     """1024^2 sources (z=0, 10 um, amp U[0,1), phase U[0,2pi)) -> 1024^2 targets
     (z=0.35 m, 7.56 um), sign +1: the FP64 vs FP32-split and Monte Carlo case."""

@@ -336,32 +336,17 @@ def test_lattice_random_geometries(pkg, ctx, seed, tiles):
         assert rel_l2(yj32, yj64) <= 1e-6
 
 
-def _reference_plane(n, width, z, aspect=1.0, offset=(0.0, 0.0)):
-    """The reference's plane generator without jitter (cuda/init.h:96-157):
-    x = floor(sqrt(n aspect)) rows, y = n // x columns, x slow, and the
-    n - x y points left over set to point 0."""
-    x = int(np.sqrt(n * aspect))
-    y = n // x
-    height = width / aspect
-    dx, dy = width / x, height / y
-    u = np.empty((n, 3))
-    i, j = np.meshgrid(np.arange(x), np.arange(y), indexing="ij")
-    u[:x * y, 0] = (i * dx - (width - dx) / 2 + offset[0]).ravel()
-    u[:x * y, 1] = (j * dy - (height - dy) / 2 + offset[1]).ravel()
-    u[:x * y, 2] = z
-    u[x * y:] = u[0]
-    return u, (x, y)
-
-
 @pytest.mark.parametrize("n, aspect", [(1000, 1.0), (61 * 45, 61 / 45), (1024 * 1080 // 64, 1.0),
                                        (40 * 50 + 17, 0.8)])
 @pytest.mark.parametrize("sign", [+1, -1])
-def test_lattice_padded_planes(pkg, ctx, oracle, n, aspect, sign, tiles):
+def test_lattice_padded_planes(pkg, ctx, oracle, workloads, n, aspect, sign, tiles):
     """Planes of the reference's generator whose sides are not multiples of 16,
     ending with the copies of point 0 that fill n: 16-row tiles over the
     lattice padded with weight-0 slots, the copies summed exactly (with
     "grid16" off: the tiled kernel)."""
-    u, (x, y) = _reference_plane(n, 1.2e-3, 0.0, aspect, (1e-4, -2e-4))
+    u = workloads.reference_plane(n, 1.2e-3, 0.0, aspect, (1e-4, -2e-4))
+    x = int(np.sqrt(n * aspect))
+    y = n // x
     g = np.random.default_rng(n)
     amp, phase = g.random(n), g.random(n) * 2 * np.pi
     v = targets(n % 97, 500, 0.3)
