@@ -152,6 +152,30 @@ def test_generated_mode_writes_reference_files(tmp_path, pkg):
 
 
 @pytest.mark.gpu
+def test_generated_plane_padding(tmp_path, workloads):
+    """A projector point count that is not x * y (-y 1000): a 31 x 32 lattice
+    in x-slow order followed by 8 copies of point 0 (cuda/init.h:96-157), the
+    same layout as workloads.reference_plane; the field over it is finite."""
+    work = tmp_path / "run"
+    (tmp_path / "tmp").mkdir()
+    work.mkdir()
+    r = run(["-x", "16", "-y", "1000", "-z", "1024"], cwd=str(work))
+    assert r.returncode == 0, r.stderr
+    v = np.fromfile(tmp_path / "tmp" / "v0.dat", "<f8").reshape(-1, 3)
+    assert v.shape == (1000, 3)
+    lat = v[:992].reshape(31, 32, 3)
+    assert np.all(lat[:, :, 0] == lat[:, :1, 0]) and np.all(lat[:, :, 1] == lat[:1, :, 1])
+    assert np.all(np.diff(lat[:, 0, 0]) > 0) and np.all(np.diff(lat[0, :, 1]) > 0)
+    assert np.array_equal(v[992:], np.repeat(v[:1], 8, 0))
+    ref = workloads.reference_plane(1000, 1.0, 0.0)
+    dx = np.diff(lat[:, 0, 0]).mean() / np.diff(ref[:992:32, 0]).mean()
+    np.testing.assert_allclose(v[:, :2] - v[:1, :2], (ref[:, :2] - ref[:1, :2]) * dx,
+                               rtol=0, atol=1e-12 * dx)
+    y = np.fromfile(tmp_path / "tmp" / "y0_amp.dat", "<f8")
+    assert y.size == 1000 and np.all(np.isfinite(y))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("shape", ["circle", "line", "logline", "dottedcircle"])
 def test_generated_object_shapes(tmp_path, shape):
     """HOLO_SHAPE selects the object of cuda/init.h:160-234 (width 1 mm, 16
