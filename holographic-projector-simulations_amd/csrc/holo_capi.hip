@@ -520,9 +520,9 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   return true;
 }
 
-// A uniform lattice that build_grid does not take: sides of at least 16 that
-// are not both multiples of 8, and possibly followed by fewer than max(nx, ny)
-// other sources (the reference's plane generator fills a point count that is
+// A uniform lattice that build_grid does not take: sides of at least 16, not
+// both multiples of 8, or followed by fewer than max(nx, ny) other sources at
+// any positions (the reference's plane generator fills a point count that is
 // not nx * ny with copies of source 0, cuda/init.h:153-157).  Only the 16-row
 // tiles are built, over the lattice padded to multiples of 16; every target
 // sums the remaining sources (tail) exactly.
