@@ -389,11 +389,11 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
 // x depending on the row i only, y on the column j only, and one z (the
 // reference's plane generators, cuda/init.h:89-158, x slow as in
 // cuda/macros.h:59).  nx and ny must be multiples of 8; the grid is cut into
-// 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).  When
-// both are multiples of 16 and the positions are x_0 + i p_x, y_0 + j p_y
-// (to 1e-9 of a pitch), it is also cut into 16 x 16 tiles t = I*(ny/16) + J,
-// slot 16a + b = source (16I+a, 16J+b), centred at the lattice points
-// x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.  When ny is a multiple of 32,
+// 8 x 8 tiles t = I*(ny/8) + J, slot 8a + b = source (8I+a, 8J+b).  When the
+// positions are x_0 + i p_x, y_0 + j p_y (to 1e-9 of a pitch), it is also cut
+// into 16 x 16 tiles t = I*(TJ) + J, TJ = ceil(ny/16), slot 16a + b = source
+// (16I+a, 16J+b) or padding (-1) beyond the lattice, centred at the lattice
+// points x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.  When TJ is even,
 // tiles 2T and 2T + 1 (the same rows, columns 32J' .. 32J' + 31) also form
 // the 16 x 32 tile T, centred at x_0 + (16I + 7.5) p_x, y_0 + (32J' + 15.5) p_y.
 struct HostGrid {
@@ -516,7 +516,10 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
         for (int b = 0; b < 8; ++b)
           g.perm.push_back((int32_t)((8 * I + a) * ny + 8 * J + b));
     }
-  build_grid16(u, g);
+  // uniform lattices also get 16-row tiles, padded where a side is not a
+  // multiple of 16 (1000 x 1000: 3 % of the slots are padding, and the 16-row
+  // kernel runs at twice the 8 x 8 rate)
+  build_grid16(u, g, true);
   return true;
 }
 

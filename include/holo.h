@@ -97,9 +97,9 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * consecutive sources with one x, as lattices in x-slow order do, dx^2 is
  * formed once per target and row; same bits; default 1),
  * "grid" (0/1: HOLO_FP32_SPLIT on lattice sources uses the matrix-core
- * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice with nx,
- * ny multiples of 16 that kernel uses 16 x 16 tiles; default 1), "grid32"
- * (0/1: with "grid16", and ny a multiple of 32, tiles of 16 x 32; default
+ * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice that
+ * kernel uses 16 x 16 tiles, padded to multiples of 16; default 1), "grid32"
+ * (0/1: with "grid16", and ny padded to a multiple of 32, tiles of 16 x 32; default
  * 1), "mc_lattice" (0/1: per-target FP32-split Monte Carlo on product-grid
  * sources reads positions from row and column tables; same results; default
  * 1).  Results do not depend on them beyond summation order and, for "grid",
@@ -179,8 +179,8 @@ holo_status holo_plan_info(const holo_plan *plan, size_t *n_sources,
  * nx * ny with copies of source 0, cuda/init.h:153-157).  HOLO_FP32_SPLIT then
  * runs the lattice kernel: tiles of 16 x 32 or 16 x 16 sources on a uniform
  * lattice (padded with weight-0 slots to multiples of 16, the sources after
- * it summed exactly by every target), else 8 x 8, each summed as a complex
- * GEMM on the matrix cores.  nx = ny = 0 when the plan is not a lattice.
+ * it summed exactly by every target), 8 x 8 on other product grids (and with
+ * option "grid16" = 0), each summed as a complex GEMM on the matrix cores.  nx = ny = 0 when the plan is not a lattice.
  * No reference counterpart. */
 holo_status holo_plan_lattice(const holo_plan *plan, size_t *nx, size_t *ny);
 
