@@ -393,7 +393,8 @@ void build_tiles(const double* u, size_t N, unsigned T, HostTiles& out) {
 // positions are x_0 + i p_x, y_0 + j p_y (to 1e-9 of a pitch), it is also cut
 // into 16 x 16 tiles t = I*(TJ) + J, TJ = ceil(ny/16), slot 16a + b = source
 // (16I+a, 16J+b) or padding (-1) beyond the lattice, centred at the lattice
-// points x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y.  When TJ is even,
+// points x_0 + (16I + 7.5) p_x, y_0 + (16J + 7.5) p_y (with padding, an odd
+// TJ of 5 or more takes one more column of tiles).  When TJ is even,
 // tiles 2T and 2T + 1 (the same rows, columns 32J' .. 32J' + 31) also form
 // the 16 x 32 tile T, centred at x_0 + (16I + 7.5) p_x, y_0 + (32J' + 15.5) p_y.
 struct HostGrid {
@@ -432,7 +433,9 @@ void build_grid16(const double* u, HostGrid& g, bool pad = false) {
   g.rho16_x = std::nextafter((float)(7.5 * std::fabs(px)), INFINITY);
   g.rho16_y = std::nextafter((float)(7.5 * std::fabs(py)), INFINITY);
   const float2 rho{g.rho16_x, g.rho16_y};
-  const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16;
+  // an odd number of tile columns (5 or more) takes one more, of padding only,
+  // so that they pair into 16 x 32 tiles (1000 columns: 64 instead of 63)
+  const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16 + (pad && (ny + 15) / 16 % 2 && ny > 64);
   g.centre16.reserve(TI * TJ);
   g.perm16.reserve(256 * TI * TJ);
   for (size_t I = 0; I < TI; ++I)
@@ -517,8 +520,8 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
           g.perm.push_back((int32_t)((8 * I + a) * ny + 8 * J + b));
     }
   // uniform lattices also get 16-row tiles, padded where a side is not a
-  // multiple of 16 (1000 x 1000: 3 % of the slots are padding, and the 16-row
-  // kernel runs at twice the 8 x 8 rate)
+  // multiple of 16 (1000 x 1000: 1008 x 1024 slots, 3 % of them padding, at
+  // twice the 8 x 8 kernel's rate)
   build_grid16(u, g, true);
   return true;
 }
