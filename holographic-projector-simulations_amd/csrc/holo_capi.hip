@@ -415,11 +415,10 @@ struct HostGrid {
   std::vector<int32_t> tail;  // sources after the lattice (padded lattices)
 };
 
-// pad: sides that are not multiples of 16 are padded to the next multiple
-// with weight-0 slots (perm -1) at the lattice points beyond the edge.
-void build_grid16(const double* u, HostGrid& g, bool pad = false) {
+// Sides that are not multiples of 16 are padded to the next multiple with
+// weight-0 slots (perm -1) at the lattice points beyond the edge.
+void build_grid16(const double* u, HostGrid& g) {
   const size_t nx = g.nx, ny = g.ny;
-  if (!pad && (nx % 16 || ny % 16)) return;
   const double x0 = u[0], y0 = u[1];
   const double px = (u[3 * (nx - 1) * ny] - x0) / (double)(nx - 1);
   const double py = (u[3 * (ny - 1) + 1] - y0) / (double)(ny - 1);
@@ -435,7 +434,7 @@ void build_grid16(const double* u, HostGrid& g, bool pad = false) {
   const float2 rho{g.rho16_x, g.rho16_y};
   // an odd number of tile columns (5 or more) takes one more, of padding only,
   // so that they pair into 16 x 32 tiles (1000 columns: 64 instead of 63)
-  const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16 + (pad && (ny + 15) / 16 % 2 && ny > 64);
+  const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16 + ((ny + 15) / 16 % 2 && ny > 64);
   g.centre16.reserve(TI * TJ);
   g.perm16.reserve(256 * TI * TJ);
   for (size_t I = 0; I < TI; ++I)
@@ -522,7 +521,7 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   // uniform lattices also get 16-row tiles, padded where a side is not a
   // multiple of 16 (1000 x 1000: 1008 x 1024 slots, 3 % of them padding, at
   // twice the 8 x 8 kernel's rate)
-  build_grid16(u, g, true);
+  build_grid16(u, g);
   return true;
 }
 
@@ -549,7 +548,7 @@ bool build_grid16_padded(const double* u, size_t N, HostGrid& g) {
   g.nx = nx;
   g.ny = ny;
   g.z = z;
-  build_grid16(u, g, true);
+  build_grid16(u, g);
   if (g.centre16.empty()) return false;
   for (size_t n = body; n < N; ++n) g.tail.push_back((int32_t)n);
   return true;
