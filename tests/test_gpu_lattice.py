@@ -1,9 +1,10 @@
 """The matrix-core lattice kernels (HOLO_FP32_SPLIT on product-grid sources).
 
 Every test runs three times: with options "grid16" and "grid32" on (uniform
-lattices whose sides are multiples of 16 use 16-row tiles, of 32 columns where
-ny is a multiple of 32, else of 16), with "grid32" off (16 x 16 tiles), and
-with "grid16" off (8 x 8 tiles everywhere).
+lattices use 16-row tiles, padded to multiples of 16, of 32 columns where the
+tile columns pair up, else of 16), with "grid32" off (16 x 16 tiles), and with
+"grid16" off (8 x 8 tiles on product grids with sides multiples of 8, the
+tiled kernel on other planes).
 Bars: relative L2 <= 1e-6 against the FP64 oracle, as for every FP32-split
 path; the lattice and the tiled kernel agree to the same bar; results do not
 depend on the launch split or on the weights' scale.
