@@ -46,13 +46,14 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def read_traffic(kernel: str):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def read_profile(kernel: str, key: str):
+    """A figure (`key = value`) from the committed rocprofv3 summary
+    profiles/<kernel>_traffic.txt, if any."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(key):
             try:
                 return float(line.split("=", 1)[1].split()[0])
             except (IndexError, ValueError):
@@ -60,25 +61,39 @@ def read_traffic(kernel: str):
     return None
 
 
-def cpu_baseline(stage, n_targets: int):
-    """The NumPy restatement of py/util.py:92-147 (map_sum_kernel over a Pool)
-    on a fixed stride subset of stage-1 targets.  Runs before the GPU is
-    initialised, so the worker processes fork from a process without it."""
+CPU_SHARE = 16  # CPUs a one-GPU job gets on the GPU box, whatever os.cpu_count() reports
+
+
+def cpu_baseline(stage, n_targets: int, repeats: int):
+    """The NumPy restatement of py/util.py:92-147 (map_sum_kernel over a Pool,
+    util.py:134-141) on the fixed stride subset of n_targets stage-1 targets:
+    one warm-up on 256 of them, then `repeats` timed runs over the whole
+    subset (median and std of pairs/s).  Runs before the GPU is initialised,
+    so the worker processes fork from a process without it."""
     from oracle import numpy_baseline as nb
-    cores = min(16, nb.cpu_count())
+    cores = min(CPU_SHARE, nb.cpu_count())
     M = stage.v.shape[0]
     sub = (np.arange(n_targets, dtype=np.int64) * M) // n_targets
     x = np.stack([stage.amp, stage.phase], axis=1)
     y = np.empty((n_targets, 2))
-    t0 = time.perf_counter()
-    # CUDA sign s corresponds to the Python direction -s (py/util.py:106)
-    nb.map_sum_kernel(x, y, stage.u, stage.v[sub], direction=-stage.sign, lam=stage.lam,
-                      parallel=cores)
-    dt = time.perf_counter() - t0
     N = stage.u.shape[0]
-    res = {"value": N * n_targets / dt, "unit": "pairs/s", "cores": cores, "kind": "port",
+    # CUDA sign s corresponds to the Python direction -s (py/util.py:106)
+    run = lambda t, out: nb.map_sum_kernel(x, out, stage.u, stage.v[t], direction=-stage.sign,
+                                           lam=stage.lam, parallel=cores)
+    run(sub[:: max(1, n_targets // 256)], np.empty((len(sub[:: max(1, n_targets // 256)]), 2)))
+    rates, walls = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        run(sub, y)
+        dt = time.perf_counter() - t0
+        walls.append(dt)
+        rates.append(N * n_targets / dt)
+    res = {"value": float(np.median(rates)), "unit": "pairs/s", "cores": cores, "kind": "port",
+           "std": float(np.std(rates)), "repeats": repeats,
+           "host_cpus_reported": os.cpu_count(),
            "sample": f"x->y stage, {N} sources x {n_targets} stride-sampled targets "
-                     f"({N * n_targets:.3g} pairs, {dt:.1f} s wall on {cores} processes)"}
+                     f"({N * n_targets:.3g} pairs per run; 1 warm-up + {repeats} runs, "
+                     f"median wall {np.median(walls):.2f} s on {cores} processes)"}
     # the C FP64 restatement, single-threaded and with OpenMP, on the same subset
     import oracle
     c_sub = sub[:: max(1, n_targets // 64)]
@@ -93,8 +108,28 @@ def cpu_baseline(stage, n_targets: int):
     res["c_oracle"] = {"single_thread_pairs_per_s": N * c_sub.size / t1,
                        "openmp_pairs_per_s": N * n_targets / tm, "openmp_threads": cores,
                        "sample": f"{c_sub.size} (1 thread) and {n_targets} ({cores} threads) "
-                                 "of the same stride-sampled targets"}
+                                 "of the same stride-sampled targets, one run each"}
     return res, sub, y[:, 0] * np.exp(1j * y[:, 1])
+
+
+def time_events(fn, repeats: int, warmup: int = 1):
+    """fn() timed with HIP events on torch's current stream (the stream the
+    library launches on): `warmup` untimed calls, then `repeats` timed ones.
+    Returns median / std / mean / min in ms and the repeat count."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(repeats)]
+    for a, b in ev:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    ms = np.array([a.elapsed_time(b) for a, b in ev])
+    return {"median_ms": float(np.median(ms)), "std_ms": float(np.std(ms)),
+            "mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)), "repeats": repeats}
 
 
 def main():
@@ -104,10 +139,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--skip-extras", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-targets", type=int, default=1024)
+    ap.add_argument("--cpu-targets", type=int, default=4096,
+                    help="stride subset of stage-1 targets for the CPU baseline")
+    ap.add_argument("--cpu-repeats", type=int, default=5)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="timed repeats (after one warm-up) of each side-section measurement")
     ap.add_argument("--sections", default="chain,fp32,mc,adjoint",
                     help="comma list of chain, fp32, mc, adjoint (profiling runs pick one)")
-    ap.add_argument("--hologram-steps", type=int, default=3,
+    ap.add_argument("--hologram-steps", type=int, default=20,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
     ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32), default=32,
                     help="tile columns of the FP32-split lattice kernel on uniform lattices "
@@ -122,6 +161,9 @@ def main():
                     help="distance of the 1024^2 target plane of the fp32 and mc sections "
                          "(default 0.35 m); nearer planes exercise the lattice kernel's "
                          "accuracy bounds")
+    ap.add_argument("--target-waves", type=int, default=0,
+                    help="library option target_waves (waves a launch aims for when it splits "
+                         "sources; 0 = the library default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 3 times (HBM counter calibration)")
     a = ap.parse_args()
@@ -139,6 +181,8 @@ def main():
     holo.set_option("grid16", 1 if a.lattice_tiles in (16, 32) else 0)
     holo.set_option("grid32", 1 if a.lattice_tiles == 32 else 0)
     holo.set_option("grid", 0 if a.lattice_tiles == 0 else 1)
+    if a.target_waves:
+        holo.set_option("target_waves", a.target_waves)
     import importlib
     wl = importlib.import_module("holo_amd.workloads")
     w = wl.chain_xyz(0)
@@ -151,7 +195,7 @@ def main():
     cpu = None
     if not a.skip_cpu and not a.calibrate_copy:
         log(f"[bench] cpu baseline: {a.cpu_targets} targets")
-        cpu = cpu_baseline(s1, a.cpu_targets)
+        cpu = cpu_baseline(s1, a.cpu_targets, a.cpu_repeats)
         log(f"[bench] cpu: {cpu[0]}")
 
     dev = torch.device("cuda", 0)
@@ -177,6 +221,7 @@ def main():
 
     # ---------------- FP32-split vs FP64, Monte Carlo, adjoint ---------------
     f64 = None
+    R = max(5, a.repeats)
     if sections & {"fp32", "mc"}:
         p3 = (wl.plane_points(0, a.source_points) if a.source_points else wl.plane_1024(0)).stages[0]
         src_desc = (f"{a.source_points} points of the reference's plane generator"
@@ -186,68 +231,78 @@ def main():
         amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
         plan3 = holo.Plan(ctx, u3)
         y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
+        y32 = torch.empty_like(y64)
+        log(f"[bench] plane {src_desc}: FP64, 1 warm-up + {R} runs")
+        t64 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64,
+                                                  out=y64), R)
+        f64 = torch.view_as_complex(y64).cpu().numpy()
         if "fp32" in sections:
-            y32 = torch.empty_like(y64)
-            log(f"[bench] plane {src_desc}: FP64")
-            torch.cuda.synchronize(dev)
-            t0 = time.perf_counter()
-            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
-            torch.cuda.synchronize(dev)
-            t64 = time.perf_counter() - t0
-            f64 = torch.view_as_complex(y64).cpu().numpy()
-            log(f"[bench] plane {src_desc}: FP32-split")
-            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
-            torch.cuda.synchronize(dev)
-            reps = 3
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32)
-            torch.cuda.synchronize(dev)
-            t32 = (time.perf_counter() - t0) / reps
+            log(f"[bench] plane {src_desc}: FP32-split, 1 warm-up + {R} runs")
+            t32 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam,
+                                                      holo.FP32_SPLIT, out=y32), R)
+            # the same call with its error report (FP64 at 256 stride targets)
+            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32,
+                            report_error=True)
+            err_lib = float(ctx.last_rel_l2.item())
             f32 = torch.view_as_complex(y32).cpu().numpy()
+            pairs = p3.pairs
             out["fp32_split"] = {
                 "workload": f"{src_desc} -> 1024^2 (z 0 -> {p3.v[0, 2]:g} m, 10 / 7.56 um, amp U[0,1), "
                             "phase U[0,2pi)), " + (
                                 f"lattice kernel, {min(a.lattice_tiles, 16)}x{a.lattice_tiles} tiles"
                                 if a.lattice_tiles else "tiled kernel, tiles of 64"),
-                "pairs": p3.pairs, "fp64_pairs_per_s": p3.pairs / t64,
-                "fp32_split_pairs_per_s": p3.pairs / t32, "speedup": t64 / t32,
-                "fp32_split_gflops_29": p3.pairs / t32 * FLOP_PER_PAIR / 1e9,
+                "pairs": pairs,
+                "fp64_pairs_per_s": pairs / (t64["median_ms"] / 1e3),
+                "fp64_ms": t64,
+                "fp32_split_pairs_per_s": pairs / (t32["median_ms"] / 1e3),
+                "fp32_split_ms": t32,
+                "speedup": t64["median_ms"] / t32["median_ms"],
                 "rel_l2_vs_fp64": rel_l2(f32, f64),
-                "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64)))}
+                "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64))),
+                "err_rel_l2_reported": err_lib}
             log(f"[bench] fp32-split: {out['fp32_split']}")
         if "mc" in sections:
             mc = {}
+            ymc = torch.empty_like(y64)
             for shared in (False, True):
                 for K in (256, 1024, 4096):
-                    run = lambda: ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam,
-                                                   holo.FP32_SPLIT, shared=shared)
-                    run()
-                    torch.cuda.synchronize(dev)
-                    t0 = time.perf_counter()
-                    ymc = run()
-                    torch.cuda.synchronize(dev)
-                    tk = time.perf_counter() - t0
                     key = ("batched_" if shared else "") + str(K)
-                    mc[key] = {"pairs_per_s": K * v3.shape[0] / tk, "ms": 1e3 * tk,
-                               "gflops_29": K * v3.shape[0] / tk * FLOP_PER_PAIR / 1e9}
-                    if f64 is not None:
-                        mc[key]["rel_l2_vs_brute_force"] = rel_l2(ymc.cpu().numpy(), f64)
+                    log(f"[bench] monte carlo {key}: 1 warm-up + {R} runs")
+
+                    def run(report=False):
+                        return ctx.transform_mc(amp3, ph3, u3, v3, K, 0, p3.sign, p3.lam,
+                                                holo.FP32_SPLIT, shared=shared,
+                                                report_error=report)
+                    tk = time_events(run, R)
+                    ymc = run(True)
+                    err_lib = float(ctx.last_rel_l2.item())
+                    samples = K * v3.shape[0]
+                    mc[key] = {"pairs_per_s": samples / (tk["median_ms"] / 1e3), "ms": tk,
+                               "rel_l2_vs_brute_force": rel_l2(ymc.cpu().numpy(), f64),
+                               "expected_sqrt_N_over_K": float(np.sqrt(u3.shape[0] / K)),
+                               "err_rel_l2_reported": err_lib}
             out["monte_carlo"] = {"workload": f"same {src_desc} -> 1024^2, FP32-split, seed 0; K draws "
                                               "per target, or per group of 1024 targets (batched); "
-                                              "expected rel. error sqrt(N/K) for a random-phase field",
+                                              "rel_l2_vs_brute_force is the estimator's statistical "
+                                              "error, expected sqrt(N/K) for this random-phase "
+                                              "field; err_rel_l2_reported is the FP32-split "
+                                              "arithmetic against the FP64 estimator with the "
+                                              "same indices",
                                   "K": mc}
             log(f"[bench] monte carlo: {mc}")
 
     if "adjoint" in sections:
         hol = importlib.import_module("holo_amd.hologram")
-        n = a.hologram_steps
-        r = hol.optimise(steps=n, warmup=1, log_every=max(1, n // 20) if n > 10 else 0)
+        n = max(20, a.hologram_steps)
+        r = hol.optimise(steps=n, warmup=1, log_every=max(1, n // 20) if n > 20 else 0,
+                         step_events=True, report_error=True)
         out["adjoint"] = {"workload": "phase-only hologram 1024^2 (7.56 um, z=0) -> 512^2 "
                                       "(10 um, z=0.35 m), FP32-split forward + adjoint, Adam lr 0.05",
-                          "ms_per_grad_step": r.ms_per_step, "target_ms": 50.0,
+                          "ms_per_grad_step": r.step_ms["median_ms"],
+                          "step_ms": r.step_ms, "target_ms": 50.0,
                           "pairs_per_step": 2 * 1024 * 1024 * 512 * 512,
-                          "steps": n, "losses": r.losses}
+                          "steps": n, "losses": r.losses,
+                          "forward_err_rel_l2_reported": r.forward_rel_l2}
         log(f"[bench] adjoint: {out['adjoint']}")
 
     ctx.close()
@@ -303,7 +358,8 @@ def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
     achieved = FLOP_PER_PAIR * pairs_launch / avg_launch_s
     roofline = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
-                "traffic": read_traffic("k_superpose_f64"),
+                "traffic": read_profile("k_superpose_f64", "traffic_bytes_per_launch"),
+                "valu_busy_profiled": read_profile("k_superpose_f64", "valu_busy"),
                 "kernel": "k_superpose_f64 (+ partial-sum reduce)",
                 "flop_per_pair": FLOP_PER_PAIR, "pairs_per_launch": pairs_launch,
                 "counted_flop_per_pair": COUNTED_FLOP_PER_PAIR,
@@ -325,10 +381,14 @@ def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):
     log("[bench] stage 1 from host memory")
     ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam,
                        holo.FP64)  # first call sizes the staging buffers
-    t0 = time.perf_counter()
-    yh_host = ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam, holo.FP64)
-    th = time.perf_counter() - t0
+    ths = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        yh_host = ctx.transform_full(s1.amp, s1.phase, s1.u, s1.v, s1.sign, s1.lam, holo.FP64)
+        ths.append(time.perf_counter() - t0)
+    th = float(np.median(ths))
     res["host_pointer_stage1"] = {"pairs_per_s": s1.pairs / th, "ms": 1e3 * th,
+                                  "ms_std": 1e3 * float(np.std(ths)), "repeats": 3,
                                   "kernel_ms": 1e3 * avg_launch_s,
                                   "rel_l2_vs_device_path": rel_l2(
                                       yh_host, torch.view_as_complex(y).cpu().numpy())}

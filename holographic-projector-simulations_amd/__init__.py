@@ -89,8 +89,9 @@ def lib() -> ctypes.CDLL:
     L.holo_set_option.argtypes = [ctypes.c_char_p, ctypes.c_long]
     L.holo_get_option.argtypes = [ctypes.c_char_p]
     L.holo_get_option.restype = ctypes.c_long
-    L.holo_transform_full.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, vp, ctypes.c_uint]
-    L.holo_transform_mc.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, u32, u64, vp,
+    L.holo_transform_full.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, vp, vp,
+                                      ctypes.c_uint]
+    L.holo_transform_mc.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, u32, u64, vp, vp,
                                     ctypes.c_uint]
     L.holo_mc_indices.argtypes = [vp, u64, u64, sz, sz, u32, vp, ctypes.c_uint]
     L.holo_normalize_polar.argtypes = [vp, vp, sz, vp, vp, vp, ctypes.c_uint]
@@ -100,15 +101,20 @@ def lib() -> ctypes.CDLL:
     L.holo_plan_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
                                  ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(d)]
     L.holo_plan_lattice.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
-    L.holo_plan_transform.argtypes = [vp, vp, i, i, d, vp, vp, vp, sz, vp, ctypes.c_uint]
-    L.holo_plan_transform_complex.argtypes = [vp, vp, i, i, d, vp, vp, sz, vp, ctypes.c_uint]
-    L.holo_plan_adjoint.argtypes = [vp, vp, i, i, d, vp, vp, vp, vp, sz, vp, vp, ctypes.c_uint]
+    L.holo_plan_transform.argtypes = [vp, vp, i, i, d, vp, vp, vp, sz, vp, vp, ctypes.c_uint]
+    L.holo_plan_transform_complex.argtypes = [vp, vp, i, i, d, vp, vp, sz, vp, vp,
+                                              ctypes.c_uint]
+    L.holo_plan_adjoint.argtypes = [vp, vp, i, i, d, vp, vp, vp, vp, sz, vp, vp, vp,
+                                    ctypes.c_uint]
     L.holo_read_dat.argtypes = [ctypes.c_char_p, vp, sz, ctypes.POINTER(sz)]
     L.holo_write_dat.argtypes = [ctypes.c_char_p, vp, sz]
     dp = ctypes.POINTER(d)
     L.holo_plan_layout.argtypes = [vp, sz, ctypes.c_uint, vp, sz, ctypes.POINTER(sz), dp,
                                    ctypes.POINTER(ctypes.c_int)]
     L.holo_plan_layout.restype = i
+    L.holo_lattice_layout.argtypes = [vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                      ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)]
+    L.holo_lattice_layout.restype = i
     L.holo_host_sincos.argtypes = [d, dp, dp]
     L.holo_host_sincos.restype = None
     L.holo_host_sqrt_inv.argtypes = [d, dp, dp]
@@ -201,6 +207,11 @@ class Context:
         s = None if stream is None else int(stream)
         check(lib().holo_create(ctypes.byref(self._h), int(device), s))
         self.device = device
+        # error report of the last call made with report_error=True: the
+        # relative L2 distance of an FP32-split result from the FP64
+        # formulation at 256 stride targets (0 for FP64); a float, or a
+        # one-element device tensor for calls on device tensors (no sync)
+        self.last_rel_l2 = None
 
     def close(self):
         if self._h:
@@ -234,11 +245,28 @@ class Context:
         import torch
         self.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
 
+    def _err_slot(self, report: bool, device: bool, like=None):
+        """(holder, pointer) for the err_rel_l2 argument, or (None, None)."""
+        if not report:
+            return None, None
+        if device:
+            import torch
+            t = torch.zeros(1, dtype=torch.float64, device=like.device)
+            return t, t.data_ptr()
+        c = ctypes.c_double(0.0)
+        return c, ctypes.addressof(c)
+
+    def _keep_err(self, holder, device: bool) -> None:
+        if holder is not None:
+            self.last_rel_l2 = holder if device else holder.value
+
     # -- brute force ---------------------------------------------------------
     def transform_full(self, amp, phase, u, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                       precision: int = FP64, obliquity: bool = False):
+                       precision: int = FP64, obliquity: bool = False,
+                       report_error: bool = False):
         """y[m] = sum_n amp_n/d exp(i(phase_n + sign 2 pi d/lam)); complex (M,).
-        obliquity: each term also carries the factor |dz|/d."""
+        obliquity: each term also carries the factor |dz|/d.  report_error:
+        the call also writes its error report to self.last_rel_l2."""
         extra = OBLIQUITY if obliquity else 0
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)
@@ -248,21 +276,27 @@ class Context:
         if device:
             import torch
             self._torch_stream()
-            y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
+            y = torch.empty((M, 2), dtype=torch.float64, device=v.device)
+            eh, ep = self._err_slot(report_error, True, v)
             check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                            _ptr(u), N, _ptr(v), M, _ptr(y),
+                                            _ptr(u), N, _ptr(v), M, _ptr(y), ep,
                                             DEVICE_POINTERS | extra))
+            self._keep_err(eh, True)
             return torch.view_as_complex(y)
         y = np.empty(2 * M)
+        eh, ep = self._err_slot(report_error, False)
         check(lib().holo_transform_full(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                        _ptr(u), N, _ptr(v), M, _ptr(y), extra))
+                                        _ptr(u), N, _ptr(v), M, _ptr(y), ep, extra))
+        self._keep_err(eh, False)
         return y.view(np.complex128)
 
     # -- Monte Carlo -----------------------------------------------------------
     def transform_mc(self, amp, phase, u, v, K: int, seed: int = 0, sign: int = FORWARDS,
                      lam: float = LAMBDA, precision: int = FP32_SPLIT, shared: bool = False,
-                     obliquity: bool = False):
-        """Monte Carlo estimate with K draws per target (shared: per group of 1024)."""
+                     obliquity: bool = False, report_error: bool = False):
+        """Monte Carlo estimate with K draws per target (shared: per group of 1024).
+        report_error: self.last_rel_l2 receives the distance from the FP64
+        estimator with the same indices at 256 stride targets."""
         extra = (MC_SHARED if shared else 0) | (OBLIQUITY if obliquity else 0)
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)
@@ -272,14 +306,19 @@ class Context:
         if device:
             import torch
             self._torch_stream()
-            y = torch.empty((M, 2), dtype=torch.float64, device=amp.device)
+            y = torch.empty((M, 2), dtype=torch.float64, device=v.device)
+            eh, ep = self._err_slot(report_error, True, v)
             check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                          _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y),
+                                          _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), ep,
                                           DEVICE_POINTERS | extra))
+            self._keep_err(eh, True)
             return torch.view_as_complex(y)
         y = np.empty(2 * M)
+        eh, ep = self._err_slot(report_error, False)
         check(lib().holo_transform_mc(self._h, precision, sign, lam, _ptr(amp), _ptr(phase),
-                                      _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), extra))
+                                      _ptr(u), N, _ptr(v), M, int(K), int(seed), _ptr(y), ep,
+                                      extra))
+        self._keep_err(eh, False)
         return y.view(np.complex128)
 
     def mc_indices(self, seed: int, N: int, m0: int, count: int, K: int,
@@ -360,7 +399,8 @@ class Plan:
         return np.empty(2 * M)
 
     def transform(self, amp, phase, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                  precision: int = FP64, out=None, obliquity: bool = False):
+                  precision: int = FP64, out=None, obliquity: bool = False,
+                  report_error: bool = False):
         device = _is_torch(v) and v.is_cuda
         amp, phase, v = _prep([amp, phase, v], device)
         M = _n(v, 3)
@@ -370,15 +410,18 @@ class Plan:
         if device:
             self.ctx._torch_stream()
         flags = (DEVICE_POINTERS if device else 0) | (OBLIQUITY if obliquity else 0)
+        eh, ep = self.ctx._err_slot(report_error, device, v)
         check(lib().holo_plan_transform(self.ctx.handle, self._h, precision, sign, lam,
-                                        _ptr(amp), _ptr(phase), _ptr(v), M, _ptr(y), flags))
+                                        _ptr(amp), _ptr(phase), _ptr(v), M, _ptr(y), ep, flags))
+        self.ctx._keep_err(eh, device)
         if device:
             import torch
             return torch.view_as_complex(y) if y.dim() == 2 else y
         return y.view(np.complex128)
 
     def transform_complex(self, w, v, sign: int = FORWARDS, lam: float = LAMBDA,
-                          precision: int = FP64, obliquity: bool = False):
+                          precision: int = FP64, obliquity: bool = False,
+                          report_error: bool = False):
         device = _is_torch(v) and v.is_cuda
         w, v = _prep([w, v], device)
         M = _n(v, 3)
@@ -388,16 +431,20 @@ class Plan:
         if device:
             self.ctx._torch_stream()
         flags = (DEVICE_POINTERS if device else 0) | (OBLIQUITY if obliquity else 0)
+        eh, ep = self.ctx._err_slot(report_error, device, v)
         check(lib().holo_plan_transform_complex(self.ctx.handle, self._h, precision, sign, lam,
-                                                _ptr(w), _ptr(v), M, _ptr(y), flags))
+                                                _ptr(w), _ptr(v), M, _ptr(y), ep, flags))
+        self.ctx._keep_err(eh, device)
         if device:
             import torch
             return torch.view_as_complex(y)
         return y.view(np.complex128)
 
     def adjoint(self, g, amp, theta, u, sign: int = FORWARDS, lam: float = LAMBDA,
-                precision: int = FP64, obliquity: bool = False):
-        """(dL/damp, dL/dtheta) for z = transform(amp, theta; u -> plan points)."""
+                precision: int = FP64, obliquity: bool = False, report_error: bool = False):
+        """(dL/damp, dL/dtheta) for z = transform(amp, theta; u -> plan points).
+        report_error: ctx.last_rel_l2 receives the report of the field
+        b = sum_m g_m conj(G_mn) at the points u."""
         device = _is_torch(u) and u.is_cuda
         g, amp, theta, u = _prep([g, amp, theta, u], device)
         N = _n(u, 3)
@@ -410,10 +457,12 @@ class Plan:
             gt = torch.empty(N, dtype=torch.float64, device=u.device)
         else:
             ga, gt = np.empty(N), np.empty(N)
+        eh, ep = self.ctx._err_slot(report_error, device, u)
         check(lib().holo_plan_adjoint(self.ctx.handle, self._h, precision, sign, lam, _ptr(g),
-                                      _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt),
+                                      _ptr(amp), _ptr(theta), _ptr(u), N, _ptr(ga), _ptr(gt), ep,
                                       (DEVICE_POINTERS if device else 0)
                                       | (OBLIQUITY if obliquity else 0)))
+        self.ctx._keep_err(eh, device)
         return ga, gt
 
 
@@ -428,6 +477,17 @@ def plan_layout(u, tile: int = 64):
     check(lib().holo_plan_layout(_ptr(u), N, int(tile), _ptr(perm), perm.size, ctypes.byref(nt),
                                  ctypes.byref(off), ctypes.byref(pl)))
     return perm[:nt.value * (tile or 64)], nt.value, off.value, bool(pl.value)
+
+
+def lattice_layout(u):
+    """Host-side lattice detection of a plan over positions u (no device):
+    (nx, ny, tail, tiles16) -- the lattice (0, 0 if none), the sources after
+    it summed exactly, and whether 16-row tiles are built."""
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    nx, ny, tail, t16 = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int()
+    check(lib().holo_lattice_layout(_ptr(u), u.size // 3, ctypes.byref(nx), ctypes.byref(ny),
+                                    ctypes.byref(tail), ctypes.byref(t16)))
+    return nx.value, ny.value, tail.value, bool(t16.value)
 
 
 def host_sincos(x: float):

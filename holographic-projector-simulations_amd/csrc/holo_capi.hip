@@ -82,6 +82,9 @@ struct holo_ctx {
   // staging buffers for host-pointer calls and scratch
   DevBuf amp, phase, u, v, y, w, partial, small, b, pos, ph;
   DevBuf mc_wc, mc_xy, mc_info;  // product-grid Monte Carlo (McLattice)
+  DevBuf layout;                 // source layout of device-pointer FP64 calls
+  DevBuf errb;                   // error report: subset, FP64 values, partials
+  DevBuf pos64, ph64;            // FP64 Monte Carlo records for the error report
   // recorded on the previous stream when holo_set_stream switches streams
   hipEvent_t switched = nullptr;
 };
@@ -107,8 +110,9 @@ struct holo_plan {
   double grid_cx0 = 0, grid_cx1 = 0, grid_cy0 = 0, grid_cy1 = 0, grid_z = 0;
   float grid_rho_x = 0, grid_rho_y = 0;
   DevBuf g_centre, g_goff, g_perm, g_wc, g_wh, g_wl, g_ts;
-  // 16 x 16 tiles of the same lattice when its pitches are uniform and nx, ny
-  // are multiples of 16 (g16_tiles = 0 otherwise)
+  // 16 x 16 tiles of the same lattice when its pitches are uniform, over the
+  // lattice padded with weight-0 slots to multiples of 16 (g16_tiles = 0
+  // otherwise)
   size_t g16_tiles = 0;
   // sources after a padded lattice, summed exactly: their slots follow the
   // tiles' in g16_perm and g16_wc
@@ -118,7 +122,7 @@ struct holo_plan {
   float g16_rho_x = 0, g16_rho_y = 0;
   DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_ts;
   // the 16 x 32 tiles formed by pairs of 16 x 16 tiles (16x16 tiles 2T, 2T+1)
-  // when ny is a multiple of 32 (g32_tiles = 0 otherwise)
+  // when the padded count of tile columns is even (g32_tiles = 0 otherwise)
   size_t g32_tiles = 0;
   double g32_cx0 = 0, g32_cx1 = 0, g32_cy0 = 0, g32_cy1 = 0;
   float g32_rho_y = 0;
@@ -205,6 +209,66 @@ holo_status finish_out(holo_ctx* ctx, T* host, const T* devp, size_t count,
     HOLO_HIP(hipMemcpyAsync(host, devp, count * sizeof(T), hipMemcpyDeviceToHost,
                             ctx->stream));
   if (!dev) HOLO_HIP(hipStreamSynchronize(ctx->stream));
+  return HOLO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Error report of a result r at M targets (holo.h): the FP64 formulation at
+// count = min(M, kErrTargets) stride targets, then sqrt(sum |r - r64|^2 /
+// sum |r64|^2) on the device.  The scratch is one context buffer.
+struct ErrScratch {
+  double* err = nullptr;      // the report, when the caller's pointer is host memory
+  uint32_t* tmap = nullptr;   // stride subset m_i
+  double* vsub = nullptr;     // target positions at m_i
+  double2* ref = nullptr;     // FP64 values at m_i
+  double2* partial = nullptr; // split partials of the FP64 launch
+  size_t partial_cap = 0;
+};
+
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+holo_status err_scratch(holo_ctx* ctx, ErrScratch* e) {
+  const size_t n = holo::kErrTargets, np = 64 * n;
+  const size_t o_tmap = 256, o_vsub = o_tmap + align256(n * sizeof(uint32_t)),
+               o_ref = o_vsub + align256(3 * n * sizeof(double)),
+               o_part = o_ref + align256(n * sizeof(double2)),
+               total = o_part + np * sizeof(double2);
+  HOLO_TRY(ensure(ctx->errb, total, ctx->device));
+  char* b = static_cast<char*>(ctx->errb.ptr);
+  e->err = reinterpret_cast<double*>(b);
+  e->tmap = reinterpret_cast<uint32_t*>(b + o_tmap);
+  e->vsub = reinterpret_cast<double*>(b + o_vsub);
+  e->ref = reinterpret_cast<double2*>(b + o_ref);
+  e->partial = reinterpret_cast<double2*>(b + o_part);
+  e->partial_cap = np;
+  return HOLO_OK;
+}
+
+// Writes the report of result y (device, M values at targets v) to err_out
+// (host or device pointer as `dev`; nothing when NULL): 0 when fp64, else
+// run(e, count) fills e.ref with the FP64 values at e.vsub (count of them)
+// first.  A host err_out is written by a queued copy: the caller waits.
+holo_status report_zero(holo_ctx* ctx, double* err_out, bool dev) {
+  if (!err_out) return HOLO_OK;
+  if (dev) HOLO_HIP(holo::launch_zero(err_out, 1, ctx->stream));
+  else *err_out = 0.0;
+  return HOLO_OK;
+}
+
+template <typename F>
+holo_status report_error(holo_ctx* ctx, bool fp64, const double* v, size_t M, const double* y,
+                         double* err_out, bool dev, F&& run) {
+  if (!err_out) return HOLO_OK;
+  if (fp64 || M == 0) return report_zero(ctx, err_out, dev);
+  ErrScratch e;
+  HOLO_TRY(err_scratch(ctx, &e));
+  const size_t count = std::min(M, holo::kErrTargets);
+  HOLO_HIP(holo::launch_subset(v, M, count, e.tmap, e.vsub, ctx->stream));
+  HOLO_TRY(run(e, count));
+  HOLO_HIP(holo::launch_rel_l2(reinterpret_cast<const double2*>(y), e.tmap, e.ref, count,
+                               dev ? err_out : e.err, ctx->stream));
+  if (!dev)
+    HOLO_HIP(hipMemcpyAsync(err_out, e.err, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   return HOLO_OK;
 }
 
@@ -525,6 +589,23 @@ bool build_grid(const double* u, size_t N, HostGrid& g) {
   return true;
 }
 
+// Number of leading complete rows of an x-slow product grid with rows of ny:
+// row i (sources i*ny .. i*ny + ny - 1) is complete when all of its sources
+// have the x of its first source, the y of row 0's source in the same column
+// and the z of source 0.
+size_t complete_rows(const double* u, size_t N, size_t ny) {
+  const double z = u[2];
+  size_t i = 0;
+  for (; (i + 1) * ny <= N; ++i) {
+    const double x = u[3 * i * ny];
+    for (size_t j = 0; j < ny; ++j) {
+      const double* p = &u[3 * (i * ny + j)];
+      if (p[0] != x || p[1] != u[3 * j + 1] || p[2] != z) return i;
+    }
+  }
+  return i;
+}
+
 // A uniform lattice that build_grid does not take: sides of at least 16, not
 // both multiples of 8, or followed by fewer than max(nx, ny) other sources at
 // any positions (the reference's plane generator fills a point count that is
@@ -537,17 +618,15 @@ bool build_grid16_padded(const double* u, size_t N, HostGrid& g) {
   size_t ny = 1;
   while (ny < N && u[3 * ny] == u[0]) ++ny;
   if (ny < 16 || ny == N) return false;
-  const size_t nx = N / ny, body = nx * ny;
+  // nx = the number of leading complete rows: row i holds sources
+  // i*ny .. i*ny + ny - 1 with one x and the y of row 0's columns.  With an
+  // aspect ratio above 1 the generator's copies of point 0 can number ny or
+  // more; they repeat row 0's y and so never complete a row.
+  const size_t nx = complete_rows(u, N, ny), body = nx * ny;
   if (nx < 16 || N - body >= std::max(nx, ny)) return false;
-  const double z = u[2];
-  for (size_t n = 0; n < body; ++n) {
-    const size_t i = n / ny, j = n % ny;
-    if (u[3 * n] != u[3 * i * ny] || u[3 * n + 1] != u[3 * j + 1] || u[3 * n + 2] != z)
-      return false;
-  }
   g.nx = nx;
   g.ny = ny;
-  g.z = z;
+  g.z = u[2];
   build_grid16(u, g);
   if (g.centre16.empty()) return false;
   for (size_t n = body; n < N; ++n) g.tail.push_back((int32_t)n);
@@ -876,6 +955,18 @@ holo_status plan_run(holo_ctx* ctx, const holo_plan* p, int precision, int sign,
   return s;
 }
 
+// The FP64 formulation over the plan's sources (weights w0, w1 in form wf) at
+// the error report's subset: e.ref[i] for the count targets e.vsub.
+holo_status plan_fp64_at(holo_ctx* ctx, const holo_plan* p, int sign, bool obliquity,
+                         double lambda, holo::WeightForm wf, const double* w0, const double* w1,
+                         const ErrScratch& e, size_t count) {
+  const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
+  HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, p->uniform_z, p->row, w0, w1,
+                                      static_cast<const double*>(p->u.ptr), p->N, e.vsub, count,
+                                      K, e.ref, e.partial, e.partial_cap, ctx->stream));
+  return HOLO_OK;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1016,7 +1107,7 @@ long holo_get_option(const char* name) {
 holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sign,
                                 double lambda, const double* amp, const double* phase,
                                 const double* u, size_t N, const double* v, size_t M,
-                                double* y, unsigned flags) {
+                                double* y, double* err_rel_l2, unsigned flags) {
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
   HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
@@ -1031,27 +1122,41 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
   HOLO_TRY(stage_in(ctx, ctx->v, v, 3 * M, dev, &dv));
   if (N == 0) {
     HOLO_HIP(holo::launch_zero(dy, 2 * M, ctx->stream));
+    HOLO_TRY(report_zero(ctx, err_rel_l2, dev));
     return finish_out(ctx, y, dy, 2 * M, dev);
   }
   if (precision == HOLO_FP64) {
+    HOLO_TRY(report_zero(ctx, err_rel_l2, dev));
     const double* du;
     HOLO_TRY(stage_in(ctx, ctx->u, u, 3 * N, dev, &du));
     const size_t pe = split_elems_f64(M, N);
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
     const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
-    // device positions are not scanned: the general path gives the same bits
-    const bool uz = !dev && uniform_z(u, N);
-    const size_t row = uz ? row_length(u, N) : 0;
-    HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, uz, row, da, dp, du, N, dv, M,
-                                        K,
-                                        reinterpret_cast<double2*>(dy),
-                                        static_cast<double2*>(ctx->partial.ptr), pe,
-                                        ctx->stream));
+    if (dev && !holo::tuning().f64_libm) {
+      // device positions: their layout is found on the device
+      HOLO_TRY(ensure(ctx->layout, 4 * sizeof(uint32_t), ctx->device));
+      HOLO_HIP(holo::launch_superpose_f64_auto(
+          sign, holo::kPolar, obl, da, dp, du, N, dv, M, K, reinterpret_cast<double2*>(dy),
+          static_cast<double2*>(ctx->partial.ptr), pe, static_cast<uint32_t*>(ctx->layout.ptr),
+          ctx->stream));
+    } else {
+      const bool uz = !dev && uniform_z(u, N);
+      const size_t row = uz ? row_length(u, N) : 0;
+      HOLO_HIP(holo::launch_superpose_f64(sign, holo::kPolar, obl, uz, row, da, dp, du, N, dv,
+                                          M, K, reinterpret_cast<double2*>(dy),
+                                          static_cast<double2*>(ctx->partial.ptr), pe,
+                                          ctx->stream));
+    }
   } else {
     holo_plan plan;
     HOLO_TRY(plan_build(ctx, &plan, u, N, 0, dev));
     HOLO_TRY(plan_run(ctx, &plan, precision, sign, obl, lambda, holo::kPolar, da, dp, dv, M,
                       dy));
+    HOLO_TRY(report_error(ctx, false, dv, M, dy, err_rel_l2, dev,
+                          [&](const ErrScratch& e, size_t count) {
+                            return plan_fp64_at(ctx, &plan, sign, obl, lambda, holo::kPolar, da,
+                                                dp, e, count);
+                          }));
     HOLO_HIP(hipStreamSynchronize(ctx->stream));  // plan buffers die here
   }
   return finish_out(ctx, y, dy, 2 * M, dev);
@@ -1060,7 +1165,8 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
 holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
                               double lambda, const double* amp, const double* phase,
                               const double* u, size_t N, const double* v, size_t M,
-                              uint32_t K, uint64_t seed, double* y, unsigned flags) {
+                              uint32_t K, uint64_t seed, double* y, double* err_rel_l2,
+                              unsigned flags) {
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, M));
   HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_MC_SHARED | HOLO_OBLIQUITY));
   HOLO_TRY(use_device(ctx));
@@ -1098,10 +1204,24 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
     HOLO_HIP(holo::launch_mc_lattice_probe(du, N, static_cast<uint32_t*>(ctx->mc_info.ptr),
                                            static_cast<double*>(ctx->mc_xy.ptr), ctx->stream));
   const double Kw = (2 * M_PI) / lambda;
-  HOLO_HIP(holo::launch_mc(sign, fp64, flags & HOLO_MC_SHARED, flags & HOLO_OBLIQUITY,
-                           static_cast<const double4*>(ctx->pos.ptr),
+  const bool shared = flags & HOLO_MC_SHARED, obl = flags & HOLO_OBLIQUITY;
+  HOLO_HIP(holo::launch_mc(sign, fp64, shared, obl, static_cast<const double4*>(ctx->pos.ptr),
                            static_cast<const double*>(ctx->ph.ptr), N, dv, M, K, seed, Kw,
                            lambda, lat, reinterpret_cast<double2*>(dy), ctx->stream));
+  // the FP64 estimator with the same indices at the report's targets
+  HOLO_TRY(report_error(ctx, fp64, dv, M, dy, err_rel_l2, dev,
+                        [&](const ErrScratch& e, size_t count) {
+                          HOLO_TRY(ensure(ctx->pos64, N * sizeof(double4), ctx->device));
+                          HOLO_TRY(ensure(ctx->ph64, N * sizeof(double), ctx->device));
+                          double4* p64 = static_cast<double4*>(ctx->pos64.ptr);
+                          double* ph64 = static_cast<double*>(ctx->ph64.ptr);
+                          HOLO_HIP(holo::launch_mc_pack(true, da, dp, du, N, p64, ph64, nullptr,
+                                                        ctx->stream));
+                          HOLO_HIP(holo::launch_mc_subset(sign, shared, obl, p64, ph64, N,
+                                                          e.vsub, count, e.tmap, K, seed, Kw,
+                                                          lambda, e.ref, ctx->stream));
+                          return HOLO_OK;
+                        }));
   return finish_out(ctx, y, dy, 2 * M, dev);
 }
 
@@ -1191,7 +1311,7 @@ holo_status holo_plan_info(const holo_plan* plan, size_t* n_sources, size_t* n_t
 static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int precision,
                                       int sign, double lambda, holo::WeightForm wf,
                                       const double* w0, const double* w1, const double* v,
-                                      size_t M, double* y, unsigned flags) {
+                                      size_t M, double* y, double* err, unsigned flags) {
   if (!plan) return fail(HOLO_EINVAL, "null plan");
   HOLO_TRY(check_common(ctx, precision, sign, lambda, plan->N, M));
   HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
@@ -1206,6 +1326,7 @@ static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int 
   const size_t N = plan->N;
   if (N == 0) {
     HOLO_HIP(holo::launch_zero(dy, 2 * M, ctx->stream));
+    HOLO_TRY(report_zero(ctx, err, dev));
     return finish_out(ctx, y, dy, 2 * M, dev);
   }
   const double *d0, *d1 = nullptr;
@@ -1215,32 +1336,37 @@ static holo_status plan_transform_any(holo_ctx* ctx, const holo_plan* plan, int 
   } else {
     HOLO_TRY(stage_in(ctx, ctx->amp, w0, 2 * N, dev, &d0));
   }
-  HOLO_TRY(plan_run(ctx, plan, precision, sign, flags & HOLO_OBLIQUITY, lambda, wf, d0, d1, dv,
-                    M, dy));
+  const bool obl = flags & HOLO_OBLIQUITY;
+  HOLO_TRY(plan_run(ctx, plan, precision, sign, obl, lambda, wf, d0, d1, dv, M, dy));
+  HOLO_TRY(report_error(ctx, precision == HOLO_FP64, dv, M, dy, err, dev,
+                        [&](const ErrScratch& e, size_t count) {
+                          return plan_fp64_at(ctx, plan, sign, obl, lambda, wf, d0, d1, e,
+                                              count);
+                        }));
   return finish_out(ctx, y, dy, 2 * M, dev);
 }
 
 holo_status holo_plan_transform(holo_ctx* ctx, const holo_plan* plan, holo_precision precision,
                                 int sign, double lambda, const double* amp,
                                 const double* phase, const double* v, size_t M, double* y,
-                                unsigned flags) {
+                                double* err_rel_l2, unsigned flags) {
   return plan_transform_any(ctx, plan, precision, sign, lambda, holo::kPolar, amp, phase, v, M,
-                            y, flags);
+                            y, err_rel_l2, flags);
 }
 
 holo_status holo_plan_transform_complex(holo_ctx* ctx, const holo_plan* plan,
                                         holo_precision precision, int sign, double lambda,
                                         const double* w, const double* v, size_t M, double* y,
-                                        unsigned flags) {
+                                        double* err_rel_l2, unsigned flags) {
   return plan_transform_any(ctx, plan, precision, sign, lambda, holo::kCartesian, w, nullptr,
-                            v, M, y, flags);
+                            v, M, y, err_rel_l2, flags);
 }
 
 holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
                               holo_precision precision, int sign, double lambda,
                               const double* g, const double* amp, const double* theta,
                               const double* u, size_t N, double* grad_amp, double* grad_theta,
-                              unsigned flags) {
+                              double* err_rel_l2, unsigned flags) {
   if (!plan_targets) return fail(HOLO_EINVAL, "null plan");
   HOLO_TRY(check_common(ctx, precision, sign, lambda, N, plan_targets->N));
   HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
@@ -1248,7 +1374,7 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
   if (plan_targets->device != ctx->device)
     return fail(HOLO_EINVAL, "plan belongs to another device");
   const bool dev = flags & HOLO_DEVICE_POINTERS;
-  if (N == 0) return HOLO_OK;
+  if (N == 0) return report_zero(ctx, err_rel_l2, dev);
   const size_t M = plan_targets->N;
   const double *dg = nullptr, *da, *dt, *du;
   HOLO_TRY(stage_in(ctx, ctx->w, g, 2 * M, dev, &dg));
@@ -1260,14 +1386,20 @@ holo_status holo_plan_adjoint(holo_ctx* ctx, const holo_plan* plan_targets,
   HOLO_TRY(out_buf(ctx, ctx->ph, grad_theta, N, dev, &dgt));
   HOLO_TRY(ensure(ctx->b, 2 * N * sizeof(double), ctx->device));
   double* db = static_cast<double*>(ctx->b.ptr);
+  const bool obl = flags & HOLO_OBLIQUITY;
   if (M == 0) {
     HOLO_HIP(holo::launch_zero(db, 2 * N, ctx->stream));
   } else {
     // b_n: the forward targets carry g and radiate back with the sign flipped
     // (the obliquity factor |dz| / d is symmetric in the pair)
-    HOLO_TRY(plan_run(ctx, plan_targets, precision, -sign, flags & HOLO_OBLIQUITY, lambda,
-                      holo::kCartesian, dg, nullptr, du, N, db));
+    HOLO_TRY(plan_run(ctx, plan_targets, precision, -sign, obl, lambda, holo::kCartesian, dg,
+                      nullptr, du, N, db));
   }
+  HOLO_TRY(report_error(ctx, precision == HOLO_FP64 || M == 0, du, N, db, err_rel_l2, dev,
+                        [&](const ErrScratch& e, size_t count) {
+                          return plan_fp64_at(ctx, plan_targets, -sign, obl, lambda,
+                                              holo::kCartesian, dg, nullptr, e, count);
+                        }));
   HOLO_HIP(holo::launch_adjoint_epilogue(reinterpret_cast<const double2*>(db), da, dt, N, dga,
                                          dgt, ctx->stream));
   if (!dev) {
@@ -1297,6 +1429,28 @@ holo_status holo_plan_layout(const double* u, size_t N, unsigned tile, int32_t* 
     if (perm_capacity < t.perm.size()) return fail(HOLO_EINVAL, "perm buffer too small");
     std::copy(t.perm.begin(), t.perm.end(), perm);
   }
+  return HOLO_OK;
+}
+
+holo_status holo_lattice_layout(const double* u, size_t N, size_t* nx, size_t* ny,
+                                size_t* tail, int* tiles16) {
+  if (N > kMaxCount) return fail(HOLO_EINVAL, "N must be below 2^31");
+  if (N && !u) return fail(HOLO_EINVAL, "null source positions");
+  HostGrid g;
+  size_t t = 0;
+  bool t16 = false;
+  if (N && build_grid(u, N, g)) {
+    t16 = !g.centre16.empty();
+  } else if (N && build_grid16_padded(u, N, g)) {
+    t = g.tail.size();
+    t16 = true;
+  } else {
+    g = HostGrid();
+  }
+  if (nx) *nx = g.nx;
+  if (ny) *ny = g.ny;
+  if (tail) *tail = t;
+  if (tiles16) *tiles16 = t16 ? 1 : 0;
   return HOLO_OK;
 }
 

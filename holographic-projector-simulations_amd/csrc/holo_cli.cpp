@@ -296,9 +296,12 @@ std::vector<double> read_dat(const std::string& path) {  // cuda/util.h:248-283
   return buf;
 }
 
-// cuda/util.h:358-444: out.json line + <k1>_amp.dat, <k1>_phase.dat, <k2>.dat
+// cuda/util.h:358-444: out.json line + <k1>_amp.dat, <k1>_phase.dat, <k2>.dat.  A
+// field computed in FP32-split (computed = true) also records its error report
+// as "rel_l2_fp64" (holo.h); FP64 runs write the reference's keys only.
 void write_arrays(const std::vector<Polar>& x, const std::vector<double>& u, const std::string& k1,
-                  const std::string& k2, const Plane& p, double dt, double fl, const std::string& dir) {
+                  const std::string& k2, const Plane& p, double dt, double fl, const std::string& dir,
+                  bool computed = false) {
   static bool overwrite = true;
   std::ofstream out(dir + "out.json", overwrite ? std::ofstream::binary : std::ofstream::app);
   if (!out) die("cannot write " + dir + "out.json");
@@ -317,7 +320,10 @@ void write_arrays(const std::vector<Polar>& x, const std::vector<double>& u, con
       << ", " << q("z_offset") << ": " << p.offset.z << ", " << q("width") << ": " << p.width
       << ", " << q("randomized") << ": " << p.randomize << ", " << q("aspect_ratio") << ": "
       << p.aspect_ratio << ", " << q("timestamp") << ": " << t.tv_nsec << ", " << q("runtime")
-      << ": " << dt << ", " << q("flops") << ": " << fl << "}\n";
+      << ": " << dt << ", " << q("flops") << ": " << fl;
+  if (computed && holo::settings().precision == HOLO_FP32_SPLIT)
+    out << ", " << q("rel_l2_fp64") << ": " << holo::settings().last_rel_l2;
+  out << "}\n";
   out.close();
   if (!std::isfinite(amp) || !std::isfinite(phase)) die("non-finite result in " + k1);
   std::vector<double> a(x.size()), ph(x.size());
@@ -394,7 +400,7 @@ int body(Params& params) {
       const auto y = run<Direction::Backwards, add_ref>(x, u, v, projector, x_plane, y_plane,
                                                         &dt[0], false);
       write_arrays(y, v, "y" + xs, "v" + xs, y_plane, dt[0], holo::flops(dt[0], x.size(), y.size()),
-                   "../tmp/");
+                   "../tmp/", true);
       for (size_t j = 0; j < params.n_planes.projection; ++j) {
         const double r2 = j == 0 ? 0 : j / ((double)params.n_planes.projection - 1.);
         const double width = gerp(params.projection_width, r2),
@@ -410,7 +416,7 @@ int body(Params& params) {
                                                        &dt[j], false);
         const std::string zs = xs + "_" + std::to_string(j);
         write_arrays(z, w, "z" + zs, "w" + zs, z_plane, dt[j],
-                     holo::flops(dt[j], y.size(), z.size()), "../tmp/");
+                     holo::flops(dt[j], y.size(), z.size()), "../tmp/", true);
       }
     }
   } else {
@@ -419,12 +425,12 @@ int body(Params& params) {
     const auto y = run<Direction::Backwards, add_ref>(x, u, v, projector, x_plane, y_plane, &dt[0],
                                                       true);
     write_arrays(y, v, "y0", "v0", y_plane, dt[0], holo::flops(dt[0], x.size(), y.size()),
-                 params.files);
+                 params.files, true);
     if (params.projection) {
       const auto z = run<Direction::Forwards, false>(x, u, v, projector, x_plane, y_plane, &dt[0],
                                                      true);
       write_arrays(z, w, "z0_0", "w0_0", z_plane, dt[0], holo::flops(dt[0], y.size(), z.size()),
-                   params.files);
+                   params.files, true);
     }
   }
   printf("done\n");

@@ -112,6 +112,22 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool un
                                 double2* y, double2* partial,
                                 size_t partial_cap, hipStream_t stream);
 
+// The same sum for sources whose layout the host has not scanned (device
+// pointers): launch_source_layout writes it to `layout` (4 words of device
+// memory) and the kernel reads it there, taking the uniform-z and rows paths
+// of launch_superpose_f64 where they apply (the same bits).
+hipError_t launch_superpose_f64_auto(int sign, WeightForm wf, bool obliquity,
+                                     const double* w0, const double* w1, const double* u,
+                                     size_t N, const double* v, size_t M, double K, double2* y,
+                                     double2* partial, size_t partial_cap, uint32_t* layout,
+                                     hipStream_t stream);
+// layout[0] bit 0: every source has the z of source 0; bit 1: rows of
+// layout[1] (>= 4) consecutive sources that share one x, the last row
+// possibly shorter; the same rules as the host scan.  Bits not in `allow`
+// stay clear.
+hipError_t launch_source_layout(const double* u, size_t N, uint32_t* layout,
+                                hipStream_t stream, unsigned allow = 3u);
+
 // y[m] = sum_s partial[s*M + m] in fixed order (nothing to do for S <= 1).
 hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, double2* y,
                                   hipStream_t stream);
@@ -169,7 +185,9 @@ constexpr unsigned kMcGroup = 1024;  // targets sharing one subset (batched MC)
 
 // Sources on an x-slow product grid (x depends on n / ny only, y on n % ny
 // only, one z), found on the device by launch_mc_lattice_probe:
-//   info[0] = ny (0: not found), info[1] != 0: some source is off the grid;
+//   info[0] = ny (0: not found), info[1] = the number of leading complete
+//   rows (sources n < info[1] * ny lie on the grid; the rest are read from
+//   the packed records);
 //   xy[j] = y of column j (j < ny <= kLatticeMaxRow), xy[kLatticeMaxRow] = z,
 //   xy[kLatticeMaxRow + 1 + i] = x of row i;  wc[n] = the FP32 weight.
 // The per-target FP32-split kernel then gathers 8 bytes per sample instead
@@ -185,6 +203,14 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
                      const McLattice& lat, double2* y, hipStream_t stream);
+
+// The FP64 estimator at `count` targets vsub whose indices (Philox counters)
+// are tmap[i]: per target, or (shared) of the target's group; pos/ph packed
+// for FP64 by launch_mc_pack.  For the error report of an FP32-split call.
+hipError_t launch_mc_subset(int sign, bool shared, bool obliquity, const double4* pos,
+                            const double* ph, size_t N, const double* vsub, size_t count,
+                            const uint32_t* tmap, uint32_t K, uint64_t seed, double K_wave,
+                            double lambda, double2* y, hipStream_t stream);
 
 // wc may be NULL; otherwise it also receives the FP32 weights (FP32-split).
 hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
@@ -208,6 +234,16 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
                                   double* scratch, hipStream_t stream);
 
 hipError_t launch_zero(double* x, size_t n, hipStream_t stream);
+
+// Error report of an FP32-split result y (M targets): tmap[i] = floor(i M /
+// count) and vsub[i] = v[tmap[i]] for i < count; then *err (device) =
+// sqrt(sum_i |y[tmap[i]] - ref[i]|^2 / sum_i |ref[i]|^2), summed in a fixed
+// order, with ref the FP64 sum at vsub.
+constexpr size_t kErrTargets = 256;
+hipError_t launch_subset(const double* v, size_t M, size_t count, uint32_t* tmap, double* vsub,
+                         hipStream_t stream);
+hipError_t launch_rel_l2(const double2* y, const uint32_t* tmap, const double2* ref,
+                         size_t count, double* err, hipStream_t stream);
 
 // Adjoint epilogue: grad_amp = Re(e^{-i theta} b), grad_theta = A Im(...).
 hipError_t launch_adjoint_epilogue(const double2* b, const double* amp,

@@ -45,20 +45,14 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units) {
 // ROWS (with PZ): the sources come in rows of `row` consecutive sources that
 // share one x (a lattice in x-slow order, cuda/init.h:89-158), so dx^2 is
 // formed once per (target, row): again the same bits as per pair.
+// s_tab: the sin/cos table, already in LDS.
 template <int SIGN, int R, int WF, int MATH, bool OBL, bool PZ, bool ROWS>
-__global__ __launch_bounds__(kBlock) void k_superpose_f64(
+__device__ __forceinline__ void superpose_f64_body(
     const double* __restrict__ w0, const double* __restrict__ w1,
     const double* __restrict__ u, uint32_t N, uint32_t chunk, uint32_t row,
     const double* __restrict__ v, uint32_t M, double K,
-    double2* __restrict__ out) {
+    double2* __restrict__ out, const double* s_tab) {
   static_assert(!ROWS || PZ, "rows need sources in one plane");
-  // the sin/cos table, 32 KB, in LDS (5 workgroups per CU, as the VGPRs allow)
-  constexpr int kTab = sizeof(kSinCosTab) / sizeof(double);
-  static_assert(kTab % kBlock == 0, "table copy");
-  __shared__ double s_tab[kTab];
-#pragma unroll
-  for (int q = 0; q < kTab / kBlock; ++q) s_tab[threadIdx.x + q * kBlock] = kSinCosTab[threadIdx.x + q * kBlock];
-  __syncthreads();
   const uint32_t n0 = blockIdx.y * chunk;
   const uint32_t n1 = min(N, n0 + chunk);
   const uint32_t base = blockIdx.x * (kBlock * R) + threadIdx.x;
@@ -145,6 +139,103 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f64(
   }
 }
 
+// The sin/cos table, 32 KB, into LDS (5 workgroups per CU, as the VGPRs allow).
+constexpr int kTab = sizeof(kSinCosTab) / sizeof(double);
+static_assert(kTab % kBlock == 0, "table copy");
+
+__device__ __forceinline__ void load_sincos_table(double* s_tab) {
+#pragma unroll
+  for (int q = 0; q < kTab / kBlock; ++q)
+    s_tab[threadIdx.x + q * kBlock] = kSinCosTab[threadIdx.x + q * kBlock];
+  __syncthreads();
+}
+
+template <int SIGN, int R, int WF, int MATH, bool OBL, bool PZ, bool ROWS>
+__global__ __launch_bounds__(kBlock) void k_superpose_f64(
+    const double* __restrict__ w0, const double* __restrict__ w1,
+    const double* __restrict__ u, uint32_t N, uint32_t chunk, uint32_t row,
+    const double* __restrict__ v, uint32_t M, double K,
+    double2* __restrict__ out) {
+  __shared__ double s_tab[kTab];
+  load_sincos_table(s_tab);
+  superpose_f64_body<SIGN, R, WF, MATH, OBL, PZ, ROWS>(w0, w1, u, N, chunk, row, v, M, K, out,
+                                                        s_tab);
+}
+
+// The same sum with the source layout read from device memory (written by
+// launch_source_layout): layout[0] bit 0 = every source has the z of source
+// 0, bit 1 = rows of layout[1] sources with one x.  The branch is uniform over
+// the launch, and every branch gives the same bits.
+template <int SIGN, int R, int WF, bool OBL>
+__global__ __launch_bounds__(kBlock) void k_superpose_f64_auto(
+    const double* __restrict__ w0, const double* __restrict__ w1,
+    const double* __restrict__ u, uint32_t N, uint32_t chunk,
+    const uint32_t* __restrict__ layout, const double* __restrict__ v, uint32_t M, double K,
+    double2* __restrict__ out) {
+  __shared__ double s_tab[kTab];
+  load_sincos_table(s_tab);
+  const uint32_t mode = layout[0], row = layout[1];
+  if ((mode & 3u) == 3u)
+    superpose_f64_body<SIGN, R, WF, 0, OBL, true, true>(w0, w1, u, N, chunk, row, v, M, K, out,
+                                                        s_tab);
+  else if (mode & 1u)
+    superpose_f64_body<SIGN, R, WF, 0, OBL, true, false>(w0, w1, u, N, chunk, 0u, v, M, K, out,
+                                                         s_tab);
+  else
+    superpose_f64_body<SIGN, R, WF, 0, OBL, false, false>(w0, w1, u, N, chunk, 0u, v, M, K,
+                                                          out, s_tab);
+}
+
+// Source layout on the device, for calls on device pointers (the host-pointer
+// and plan paths scan the positions on the host with the same rules):
+//   layout[0] bit 0: every z equals source 0's (== on values);
+//   layout[0] bit 1: the sources form rows of L = layout[1] consecutive
+//   sources that share the row's first x (L = the run of source 0's x, at
+//   least 4; the last row may be shorter);
+//   layout[1]: L.
+__global__ __launch_bounds__(kBlock) void k_layout_init(uint32_t N, uint32_t allow,
+                                                        uint32_t* __restrict__ layout) {
+  if (threadIdx.x == 0) {
+    layout[0] = allow;
+    layout[1] = N;
+  }
+}
+
+// L = the first n with x_n != x_0 (N if none): an atomic minimum, order-free.
+__global__ __launch_bounds__(kBlock) void k_layout_run(const double* __restrict__ u, uint32_t N,
+                                                       uint32_t* __restrict__ layout) {
+  const double x0 = u[0];
+  for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < N; n += gridDim.x * kBlock)
+    if (u[3 * (size_t)n] != x0) {  // this thread's first hit is its smallest
+      atomicMin(&layout[1], n);
+      break;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_layout_check(const double* __restrict__ u,
+                                                         uint32_t N,
+                                                         uint32_t* __restrict__ layout) {
+  const uint32_t L = layout[1];
+  const double z0 = u[2];
+  bool planar = true, rows = L >= 4;
+  for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < N; n += gridDim.x * kBlock) {
+    planar = planar && u[3 * (size_t)n + 2] == z0;
+    rows = rows && u[3 * (size_t)n] == u[3 * (size_t)(n - n % L)];
+  }
+  const uint32_t clear = (planar ? 0u : 3u) | (rows ? 0u : 2u);
+  if (clear) atomicAnd(&layout[0], ~clear);
+}
+
+hipError_t launch_source_layout(const double* u, size_t N, uint32_t* layout,
+                                hipStream_t stream, unsigned allow) {
+  k_layout_init<<<1, kBlock, 0, stream>>>((uint32_t)N, allow, layout);
+  if (!N) return hipGetLastError();
+  const unsigned blocks = (unsigned)std::min<size_t>((N + kBlock - 1) / kBlock, 2048);
+  k_layout_run<<<blocks, kBlock, 0, stream>>>(u, (uint32_t)N, layout);
+  k_layout_check<<<blocks, kBlock, 0, stream>>>(u, (uint32_t)N, layout);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(
     const double2* __restrict__ partial, uint32_t S, uint32_t M,
     double2* __restrict__ y) {
@@ -208,6 +299,41 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool un
     });
   });
   hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_reduce_partials(S, M, partial, y, stream);
+}
+
+hipError_t launch_superpose_f64_auto(int sign, WeightForm wf, bool obliquity,
+                                     const double* w0, const double* w1, const double* u,
+                                     size_t N, const double* v, size_t M, double K, double2* y,
+                                     double2* partial, size_t partial_cap, uint32_t* layout,
+                                     hipStream_t stream) {
+  if (M == 0) return hipSuccess;
+  // the options switch the paths off as they do for host-scanned sources
+  const unsigned allow = tuning().f64_planar ? (tuning().f64_rows ? 3u : 1u) : 0u;
+  hipError_t e = launch_source_layout(u, N, layout, stream, allow);
+  if (e != hipSuccess) return e;
+  int R = tuning().r_f64;
+  if (R != 1 && R != 2 && R != 4) R = 2;
+  int S = N ? choose_splits(M, R, N, 1024) : 1;
+  while (S > 1 && (size_t)S * M > partial_cap) --S;
+  const uint32_t chunk = N ? (uint32_t)((N + S - 1) / S) : 0;
+  const dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
+                  (unsigned)S);
+  double2* out = S > 1 ? partial : y;
+  with_sign(sign, [&](auto sg) {
+    with_r(R, [&](auto rr) {
+      with_bool(wf == kCartesian, [&](auto cart) {
+        with_bool(obliquity, [&](auto obl) {
+          constexpr int wfv = cart.value ? kCartesian : kPolar;
+          k_superpose_f64_auto<sg.value, rr.value, wfv, (bool)obl.value>
+              <<<grid, kBlock, 0, stream>>>(w0, w1, u, (uint32_t)N, chunk, layout, v,
+                                            (uint32_t)M, K, out);
+        });
+      });
+    });
+  });
+  e = hipGetLastError();
   if (e != hipSuccess) return e;
   return launch_reduce_partials(S, M, partial, y, stream);
 }
@@ -444,26 +570,43 @@ hipError_t launch_tile_weights(WeightForm wf, const double* w0,
 
 constexpr uint32_t kMcSharedTag = 0x80000000u;
 
-template <int SIGN, bool FP64, bool OBL>
+// The Philox block of draws k, k + 1 (k even) of target m: per target, or
+// (SHARED_IDX) of m's group's shared subset.
+template <bool SHARED_IDX>
+__device__ __forceinline__ U4 mc_draws(uint32_t k, uint32_t m, uint32_t slo, uint32_t shi) {
+  if constexpr (SHARED_IDX)
+    return philox4x32_10(U4{k >> 1, m / kMcGroup, kMcSharedTag, kMcStream}, slo, shi);
+  else
+    return philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, slo, shi);
+}
+
+// Per-target estimator; thread i evaluates the target at v[i], whose index
+// (the Philox counter) is tmap[i], or i without a map.
+template <int SIGN, bool FP64, bool OBL, bool SHARED_IDX>
 __global__ __launch_bounds__(kBlock) void k_mc(
     const double4* __restrict__ pos, const double* __restrict__ ph,
-    uint32_t N, const double* __restrict__ v, uint32_t M, uint32_t K,
-    uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
+    uint32_t N, const double* __restrict__ v, uint32_t M, const uint32_t* __restrict__ tmap,
+    uint32_t K, uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
     double scale, McLattice lat, double2* __restrict__ y) {
-  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
-  if (m >= M) return;
-  const double vx = v[3 * (size_t)m], vy = v[3 * (size_t)m + 1],
-               vz = v[3 * (size_t)m + 2];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= M) return;
+  const uint32_t m = tmap ? tmap[i] : i;
+  const double vx = v[3 * (size_t)i], vy = v[3 * (size_t)i + 1],
+               vz = v[3 * (size_t)i + 2];
   // product-grid sources: (x, y, z, w) from the row/column tables and the
   // weight array, the same values as the packed record
-  uint32_t ny = 0;
+  uint32_t ny = 0, rows = 0;
   if constexpr (!FP64) {
-    if (lat.info && !lat.info[1]) ny = lat.info[0];
+    if (lat.info) {
+      ny = lat.info[0];
+      rows = ny ? lat.info[1] : 0u;
+      if (!rows) ny = 0;
+    }
   }
   const double inv_ny = ny ? 1.0 / (double)ny : 0.0;
-  // sources after the last whole row (a padded plane's copies of source 0)
-  // come from the packed records
-  const uint32_t body = ny ? (N / ny) * ny : 0u;
+  // sources after the leading complete rows (a padded plane's copies of
+  // source 0) come from the packed records
+  const uint32_t body = rows * ny;
   const auto source = [&](uint32_t n) -> double4 {
     if (n >= body) return pos[n];
     uint32_t i = (uint32_t)((double)n * inv_ny);
@@ -481,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
     uint32_t idx[8];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
-      const U4 o = philox4x32_10(U4{(k >> 1) + h, m, 0u, kMcStream}, seed_lo, seed_hi);
+      const U4 o = mc_draws<SHARED_IDX>(k + 2 * h, m, seed_lo, seed_hi);
       idx[2 * h] = lemire_index(o.x, o.y, N);
       idx[2 * h + 1] = lemire_index(o.z, o.w, N);
     }
@@ -510,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
     }
   }
   for (uint32_t k = K8; k < K; k += 2) {
-    const U4 o = philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, seed_lo, seed_hi);
+    const U4 o = mc_draws<SHARED_IDX>(k, m, seed_lo, seed_hi);
     const uint32_t idx[2] = {lemire_index(o.x, o.y, N), lemire_index(o.z, o.w, N)};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -526,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
   }
   ar += fr;
   ai += fi;
-  y[m] = make_double2(ar * scale, ai * scale);
+  y[i] = make_double2(ar * scale, ai * scale);
 }
 
 // Batched Monte Carlo: one workgroup = one group of kMcGroup targets
@@ -607,8 +750,29 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
           k_mc_shared<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
               pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
         else
-          k_mc<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
-              pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, lat, y);
+          k_mc<sg.value, (bool)f64.value, (bool)obl.value, false>
+              <<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, nullptr, K,
+                                              slo, shi, K_wave, il, scale, lat, y);
+      });
+    });
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_mc_subset(int sign, bool shared, bool obliquity, const double4* pos,
+                            const double* ph, size_t N, const double* vsub, size_t count,
+                            const uint32_t* tmap, uint32_t K, uint64_t seed, double K_wave,
+                            double lambda, double2* y, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  const double scale = (double)N / (double)K;
+  const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
+  const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+  with_sign(sign, [&](auto sg) {
+    with_bool(shared, [&](auto sh) {
+      with_bool(obliquity, [&](auto obl) {
+        k_mc<sg.value, true, (bool)obl.value, (bool)sh.value><<<blocks, kBlock, 0, stream>>>(
+            pos, ph, (uint32_t)N, vsub, (uint32_t)count, tmap, K, slo, shi, K_wave,
+            1.0 / lambda, scale, McLattice{}, y);
       });
     });
   });
@@ -650,9 +814,9 @@ hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,
 
 // Row length of an x-slow product grid: the first n with x_n != x_0, among
 // n < min(N, kLatticeMaxRow); N when all x are equal and N <= kLatticeMaxRow;
-// 0 (not a grid) otherwise.  The sources after the last whole row (a padded
-// plane's copies of source 0, cuda/init.h:153-157) are not part of the grid.
-// Clears info[1].
+// 0 (not a grid) otherwise.  info[0] = that length ny, info[1] = N / ny, the
+// number of whole rows, which k_lattice_check lowers to the number of leading
+// complete rows.
 __global__ __launch_bounds__(kBlock) void k_lattice_row(const double* __restrict__ u,
                                                         uint32_t N, uint32_t* __restrict__ info) {
   __shared__ uint32_t s_min[kBlock / 64];
@@ -670,14 +834,17 @@ __global__ __launch_bounds__(kBlock) void k_lattice_row(const double* __restrict
   if (threadIdx.x == 0) {
     uint32_t f = lim;
     for (int w = 0; w < kBlock / 64; ++w) f = min(f, s_min[w]);
-    info[0] = f < lim ? f : (N <= kLatticeMaxRow ? N : 0u);
-    info[1] = 0;
+    const uint32_t ny = f < lim ? f : (N <= kLatticeMaxRow ? N : 0u);
+    info[0] = ny;
+    info[1] = ny ? N / ny : 0u;
   }
 }
 
 // Checks u[n] == (x of row n / ny, y of column n % ny, z_0) exactly for the
-// sources of the whole rows, writing 1 to info[1] where it fails (plain stores
-// of the same value), and fills the tables described with McLattice.
+// sources of the whole rows and lowers info[1] to the first row where it fails
+// (an atomic minimum: order-free), so that info[1] rows lead complete (a
+// padded plane's copies of source 0 repeat row 0's y and never complete a
+// row); fills the tables described with McLattice.
 __global__ __launch_bounds__(kBlock) void k_lattice_check(const double* __restrict__ u,
                                                           uint32_t N, uint32_t* __restrict__ info,
                                                           double* __restrict__ xy) {
@@ -686,7 +853,8 @@ __global__ __launch_bounds__(kBlock) void k_lattice_check(const double* __restri
   if (!ny || n >= (N / ny) * ny) return;
   const uint32_t i = n / ny, j = n - i * ny;
   const double x = u[3 * (size_t)n], y = u[3 * (size_t)n + 1], z = u[3 * (size_t)n + 2];
-  if (!(x == u[3 * (size_t)i * ny] && y == u[3 * (size_t)j + 1] && z == u[2])) info[1] = 1;
+  if (!(x == u[3 * (size_t)i * ny] && y == u[3 * (size_t)j + 1] && z == u[2]))
+    atomicMin(&info[1], i);
   if (j == 0) xy[kLatticeMaxRow + 1 + i] = x;
   if (i == 0) xy[j] = y;
   if (n == 0) xy[kLatticeMaxRow] = z;
@@ -796,6 +964,73 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
   k_final_max<<<1, kBlock, 0, stream>>>(scratch, nb, max_out);
   const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
   k_to_polar<<<blocks, kBlock, 0, stream>>>(y, (uint32_t)M, max_out, amp, phase);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Error report of an FP32-split result: the FP64 sum at a stride subset of
+// its targets (tmap[i] = floor(i M / count)), then the relative L2 distance
+// in a fixed order.
+
+__global__ __launch_bounds__(kBlock) void k_subset(const double* __restrict__ v, uint32_t M,
+                                                   uint32_t count, uint32_t* __restrict__ tmap,
+                                                   double* __restrict__ vsub) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t m = (uint32_t)(((uint64_t)i * M) / count);
+  tmap[i] = m;
+  vsub[3 * (size_t)i] = v[3 * (size_t)m];
+  vsub[3 * (size_t)i + 1] = v[3 * (size_t)m + 1];
+  vsub[3 * (size_t)i + 2] = v[3 * (size_t)m + 2];
+}
+
+hipError_t launch_subset(const double* v, size_t M, size_t count, uint32_t* tmap, double* vsub,
+                         hipStream_t stream) {
+  if (!count) return hipSuccess;
+  const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+  k_subset<<<blocks, kBlock, 0, stream>>>(v, (uint32_t)M, (uint32_t)count, tmap, vsub);
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// One workgroup: sqrt(sum_i |y[tmap[i]] - ref[i]|^2 / sum_i |ref[i]|^2).
+__global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y,
+                                                   const uint32_t* __restrict__ tmap,
+                                                   const double2* __restrict__ ref,
+                                                   uint32_t count, double* __restrict__ err) {
+  __shared__ double red[2][kBlock / 64];
+  double num = 0.0, den = 0.0;
+  for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
+    const double2 a = y[tmap[i]], b = ref[i];
+    const double dr = a.x - b.x, di = a.y - b.y;
+    num += dr * dr + di * di;
+    den += b.x * b.x + b.y * b.y;
+  }
+  num = wave_sum(num);
+  den = wave_sum(den);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = num;
+    red[1][threadIdx.x >> 6] = den;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double n = 0.0, d = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+      n += red[0][w];
+      d += red[1][w];
+    }
+    err[0] = d > 0.0 ? __builtin_sqrt(n / d) : (n > 0.0 ? INFINITY : 0.0);
+  }
+}
+
+hipError_t launch_rel_l2(const double2* y, const uint32_t* tmap, const double2* ref,
+                         size_t count, double* err, hipStream_t stream) {
+  k_rel_l2<<<1, kBlock, 0, stream>>>(y, tmap, ref, (uint32_t)count, err);
   return hipGetLastError();
 }
 

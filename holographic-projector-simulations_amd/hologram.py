@@ -34,11 +34,18 @@ class HologramResult:
     losses: list
     ms_per_step: float
     theta: torch.Tensor
+    step_ms: dict = None         # per-step HIP-event times (median, std, ...), step_events
+    forward_rel_l2: float = None  # error report of the final forward field, report_error
 
 
 def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 1024,
              n_img: int = 512, precision: int = FP32_SPLIT, device: int = 0,
-             warmup: int = 1, log_every: int = 0, z: float = 0.35) -> HologramResult:
+             warmup: int = 1, log_every: int = 0, z: float = 0.35,
+             step_events: bool = False, report_error: bool = False) -> HologramResult:
+    """Adam on theta.  step_events: every step is bracketed by HIP events on
+    torch's current stream (the stream the library launches on) and the result
+    carries their median / std; report_error: the FP32-split forward field of
+    the final theta is evaluated once more with its error report."""
     wl = hologram_workload(seed, n_proj, n_img, z)
     st = wl.stages[0]
     dev = torch.device("cuda", device)
@@ -61,9 +68,15 @@ def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 10
     for _ in range(warmup):
         step()
     torch.cuda.synchronize(dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps if step_events else 0)]
     t0 = time.perf_counter()
     for i in range(steps):
+        if step_events:
+            ev[i][0].record()
         loss = step()
+        if step_events:
+            ev[i][1].record()
         if log_every and i % log_every == 0:
             losses.append(float(loss.detach()))
             print(f"step {i}: loss {losses[-1]:.4e}", file=sys.stderr, flush=True)
@@ -71,7 +84,18 @@ def optimise(steps: int = 200, lr: float = 0.05, seed: int = 0, n_proj: int = 10
     dt = time.perf_counter() - t0
     if not log_every:
         losses.append(float(loss.detach()))
-    return HologramResult(losses, 1e3 * dt / max(steps, 1), theta.detach())
+    res = HologramResult(losses, 1e3 * dt / max(steps, 1), theta.detach())
+    if step_events and steps:
+        ms = np.array([a.elapsed_time(b) for a, b in ev])
+        res.step_ms = {"median_ms": float(np.median(ms)), "std_ms": float(np.std(ms)),
+                       "mean_ms": float(np.mean(ms)), "min_ms": float(np.min(ms)),
+                       "repeats": int(steps)}
+    if report_error:
+        with torch.no_grad():
+            prop.plan_u.transform(amp, theta.detach(), prop.v, prop.sign, prop.lam,
+                                  prop.precision, report_error=True)
+            res.forward_rel_l2 = float(prop.ctx.last_rel_l2.item())
+    return res
 
 
 if __name__ == "__main__":

@@ -26,6 +26,14 @@
  *  - N and M must be below 2^31.  A source and a target at the same point
  *    (d = 0) yield a non-finite term, as in the reference.
  *  - One context per host thread; calls on one context are not reentrant.
+ *  - Error report (err_rel_l2, nullable, of every call that returns a field
+ *    or gradients): for HOLO_FP32_SPLIT the call also evaluates the FP64
+ *    formulation at count = min(M, 256) stride targets m_i = floor(i M /
+ *    count) and writes sqrt(sum_i |r[m_i] - r64_i|^2 / sum_i |r64_i|^2) of
+ *    its result r; for HOLO_FP64 it writes 0 (the result is the FP64
+ *    formulation).  The extra work is 256 N FP64 pairs (256 K samples for
+ *    Monte Carlo).  err_rel_l2 is a host pointer, or a device pointer with
+ *    HOLO_DEVICE_POINTERS (the call then stays asynchronous).
  */
 #ifndef HOLO_H
 #define HOLO_H
@@ -37,7 +45,7 @@
 extern "C" {
 #endif
 
-#define HOLO_ABI_VERSION 1
+#define HOLO_ABI_VERSION 2
 
 typedef enum holo_status {
   HOLO_OK = 0,
@@ -99,10 +107,10 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * "grid" (0/1: HOLO_FP32_SPLIT on lattice sources uses the matrix-core
  * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice that
  * kernel uses 16 x 16 tiles, padded to multiples of 16; default 1), "grid32"
- * (0/1: with "grid16", and ny padded to a multiple of 32, tiles of 16 x 32; default
- * 1), "mc_lattice" (0/1: per-target FP32-split Monte Carlo on product-grid
- * sources reads positions from row and column tables; same results; default
- * 1).  Results do not depend on them beyond summation order and, for "grid",
+ * (0/1: with "grid16", where the padded count of 16-column tile columns is
+ * even, tiles of 16 x 32; default 1), "mc_lattice" (0/1: per-target
+ * FP32-split Monte Carlo on product-grid sources reads positions from row
+ * and column tables; same results; default 1).  Results do not depend on them beyond summation order and, for "grid",
  * "grid16" and "grid32", the FP32-split rounding (all paths stay within
  * ~1e-6).  Replaces the blockDim/gridDim/thread_size/n_streams hints of
  * cuda/input.h:116-122. */
@@ -112,12 +120,13 @@ long holo_get_option(const char *name);
 /* ---- brute force: transform_full ----------------------------------------
  * Replaces transform_full<Direction, Algorithm, shared_memory>(x, u, v,
  * Geometry) (cuda/transform.cu:147-340): y[m] for m < M, unnormalised.
- * amp, phase: N doubles; u: 3N; v: 3M; y: 2M (interleaved complex). */
+ * amp, phase: N doubles; u: 3N; v: 3M; y: 2M (interleaved complex);
+ * err_rel_l2: the error report above (may be NULL). */
 holo_status holo_transform_full(holo_ctx *ctx, holo_precision precision,
                                 int sign, double lambda, const double *amp,
                                 const double *phase, const double *u,
                                 size_t N, const double *v, size_t M,
-                                double *y, unsigned flags);
+                                double *y, double *err_rel_l2, unsigned flags);
 
 /* ---- Monte Carlo: transform ---------------------------------------------
  * The reference's transform() (named in README.md:127, absent from the
@@ -131,12 +140,16 @@ holo_status holo_transform_full(holo_ctx *ctx, holo_precision precision,
  * With HOLO_MC_SHARED ("Batched MC", py/plot.py:787-804): the targets of
  * group g = m / 1024 share the subset n_{g,k} drawn with counter
  * (k/2, g, 0x80000000, 0x4d43); each target's estimate is still unbiased
- * with the same variance, but errors are correlated within a group. */
+ * with the same variance, but errors are correlated within a group.
+ * err_rel_l2 (may be NULL): the error report above against the FP64
+ * estimator with the same indices (not the statistical error, which is
+ * that of the estimator itself). */
 holo_status holo_transform_mc(holo_ctx *ctx, holo_precision precision,
                               int sign, double lambda, const double *amp,
                               const double *phase, const double *u, size_t N,
                               const double *v, size_t M, uint32_t K,
-                              uint64_t seed, double *y, unsigned flags);
+                              uint64_t seed, double *y, double *err_rel_l2,
+                              unsigned flags);
 
 /* Sample indices n_{m,k} for targets m0 .. m0+count-1 (row-major count x K,
  * int64), as drawn on the device by holo_transform_mc; with HOLO_MC_SHARED,
@@ -192,19 +205,27 @@ holo_status holo_plan_layout(const double *u, size_t N, unsigned tile,
                              int32_t *perm, size_t perm_capacity,
                              size_t *n_tiles, double *max_offset, int *planar);
 
+/* Host-only preview of the lattice a plan with the default tile detects (no
+ * device needed): nx, ny as holo_plan_lattice reports them (0, 0: not a
+ * lattice), the number of sources after the lattice that every target sums
+ * exactly (tail), and whether the 16-row tiles are built (tiles16).  Any
+ * output pointer may be NULL.  No reference counterpart. */
+holo_status holo_lattice_layout(const double *u, size_t N, size_t *nx, size_t *ny,
+                                size_t *tail, int *tiles16);
+
 /* Polar weights (amp, phase: N each), as holo_transform_full. */
 holo_status holo_plan_transform(holo_ctx *ctx, const holo_plan *plan,
                                 holo_precision precision, int sign,
                                 double lambda, const double *amp,
                                 const double *phase, const double *v, size_t M,
-                                double *y, unsigned flags);
+                                double *y, double *err_rel_l2, unsigned flags);
 
 /* Complex weights w (interleaved, N): y[m] = sum_n w_n exp(i sign d K)/d. */
 holo_status holo_plan_transform_complex(holo_ctx *ctx, const holo_plan *plan,
                                         holo_precision precision, int sign,
                                         double lambda, const double *w,
                                         const double *v, size_t M, double *y,
-                                        unsigned flags);
+                                        double *err_rel_l2, unsigned flags);
 
 /* Gradient of a real loss L through z = transform(amp, theta) with sources
  * u (N) and targets the plan's points (M): given
@@ -213,13 +234,15 @@ holo_status holo_plan_transform_complex(holo_ctx *ctx, const holo_plan *plan,
  *     grad_amp[n]    = Re(exp(-i theta_n) b_n)
  *     grad_theta[n]  = amp_n Im(exp(-i theta_n) b_n)
  * `plan_targets` tiles the forward targets (they are the adjoint's sources);
- * `sign` is the forward sign.  No reference counterpart. */
+ * `sign` is the forward sign.  err_rel_l2 (may be NULL): the error report
+ * above of the field b at the N points u.  No reference counterpart. */
 holo_status holo_plan_adjoint(holo_ctx *ctx, const holo_plan *plan_targets,
                               holo_precision precision, int sign,
                               double lambda, const double *g,
                               const double *amp, const double *theta,
                               const double *u, size_t N, double *grad_amp,
-                              double *grad_theta, unsigned flags);
+                              double *grad_theta, double *err_rel_l2,
+                              unsigned flags);
 
 /* ---- .dat files -----------------------------------------------------------
  * Headerless little-endian float64 arrays (cuda/util.h:248-283 read_bytes,

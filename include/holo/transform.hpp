@@ -128,6 +128,10 @@ struct Settings {
   // Cosine (obliquity) factor |dz| / d on every pair, as in the MATLAB
   // light-field model (matlab/holofunctions.m:226-227); the CUDA path has none.
   bool obliquity = false;
+  // Error report of the last transform (holo.h): relative L2 distance of an
+  // HOLO_FP32_SPLIT field from the FP64 formulation at 256 stride targets;
+  // 0 after an HOLO_FP64 transform.
+  double last_rel_l2 = 0.0;
 };
 inline Settings& settings() {
   static Settings s;
@@ -152,6 +156,7 @@ inline std::vector<Wave> superpose(Direction dir, const std::vector<Polar>& x,
   std::vector<Wave> y(M_out);
   check(holo_transform_full(context(), precision, sign_of(dir), lambda, amp.data(), phase.data(),
                             u.data(), N, vv.data(), M_out, reinterpret_cast<double*>(y.data()),
+                            &settings().last_rel_l2,
                             settings().obliquity ? HOLO_OBLIQUITY : 0u),
         "holo_transform_full");
   return y;
@@ -205,6 +210,7 @@ inline std::vector<Polar> time_transform(const std::vector<Polar>& x, const std:
   clock_gettime(CLOCK_MONOTONIC, t1);
   if (p.algorithm < 1 || p.algorithm > 3) throw Error(HOLO_EINVAL, "algorithm is incorrect");
   std::vector<Wave> y = transform_full<direction, Algorithm::Reduced>(x, u, v, p);
+  const double rel_l2 = settings().last_rel_l2;  // of this field, not the reference wave's
   normalize_amp<false>(y, 1.0);
   if (add_reference_wave) {
     // single far-away source behind the v plane, unit amplitude, phase 0
@@ -222,9 +228,13 @@ inline std::vector<Polar> time_transform(const std::vector<Polar>& x, const std:
   }
   clock_gettime(CLOCK_MONOTONIC, t2);
   *dt = diff(*t1, *t2);
-  if (verbose)
+  settings().last_rel_l2 = rel_l2;
+  if (verbose) {
     std::printf("TFLOPS:   \t%0.5f \t (%i FLOP_PER_POINT) \t E[runtime]: %f s\n",
                 flops(*dt, x.size(), y.size()) * 1e-12, kFlopPerPoint, *dt);
+    if (settings().precision == HOLO_FP32_SPLIT)
+      std::printf("FP32-split rel. L2 vs FP64 (256 targets): %.3e\n", rel_l2);
+  }
   std::vector<Polar> out(std::min(p.n.y, y.size()));
   for (size_t i = 0; i < out.size(); ++i)
     out[i] = {std::hypot(y[i].x, y[i].y), std::atan2(y[i].y, y[i].x)};

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol(pkg):
 
 def test_abi_version_and_status_strings(pkg):
     L = pkg.lib()
-    assert L.holo_abi_version() == 1
+    assert L.holo_abi_version() == 2
     assert L.holo_status_string(0) == b"ok"
     assert L.holo_status_string(1) == b"invalid argument"
     assert L.holo_status_string(99) == b"unknown status"
@@ -143,6 +143,7 @@ def test_create_without_device_fails_cleanly(pkg):
 
 def test_null_and_bad_arguments_need_no_device(pkg):
     L = pkg.lib()
-    assert L.holo_transform_full(None, 0, 1, 1e-6, None, None, None, 0, None, 0, None, 0) == 1
+    assert L.holo_transform_full(None, 0, 1, 1e-6, None, None, None, 0, None, 0, None, None,
+                                 0) == 1
     assert L.holo_plan_info(None, None, None, None, None) == 1
     assert L.holo_set_option(None, 0) == 1

@@ -302,3 +302,142 @@ def test_matlab_projector_fixture(tmp_path):
         ep = np.fromfile(os.path.join(GOLDEN, f"matlab_projector_{k}_phase.dat"), "<f8")
         assert np.max(np.abs(a - ea)) < 1e-12
         assert np.linalg.norm(a * np.exp(1j * p) - ea * np.exp(1j * ep)) / np.linalg.norm(ea) < 1e-12
+
+
+def generated_run(tmp_path, args, env=None):
+    """Run the generated mode in tmp_path/run (it writes to ../tmp/); return
+    the output directory and the parsed out.json records by phasor name."""
+    work = tmp_path / "run"
+    (tmp_path / "tmp").mkdir(parents=True)
+    work.mkdir()
+    r = run(args, cwd=str(work), env=env)
+    assert r.returncode == 0, r.stderr
+    out = tmp_path / "tmp"
+    recs = {l["phasor"]: l for l in (json.loads(x) for x in open(out / "out.json"))}
+    return out, recs
+
+
+def read_polar(out, k):
+    a = np.fromfile(out / f"{k}_amp.dat", "<f8")
+    p = np.fromfile(out / f"{k}_phase.dat", "<f8")
+    return a, p
+
+
+def lattice_sites(n, width, z, aspect, offset):
+    """The reference generator's unjittered sites (cuda/init.h:96-157) and pitches."""
+    x = int(np.sqrt(n * aspect))
+    y = n // x
+    height = width / aspect
+    dx, dy = width / x, height / y
+    i, j = np.meshgrid(np.arange(x), np.arange(y), indexing="ij")
+    s = np.zeros((x * y, 3))
+    s[:, 0] = (i * dx - (width - dx) / 2 + offset[0]).ravel()
+    s[:, 1] = (j * dy - (height - dy) / 2 + offset[1]).ravel()
+    s[:, 2] = z
+    return s, dx, dy
+
+
+@pytest.mark.gpu
+def test_randomized_projector_pixels(tmp_path, oracle, pkg):
+    """-r (cuda/init.h:116-147): every pixel of the projector and projection
+    lattices moves by up to (1 - 0.025) pitch / 2 per axis from its site, the
+    points left over copy the (jittered) point 0, the jittered plane is no
+    lattice, and the fields over the written positions match the oracle (FP64
+    <= 1e-12; FP32-split <= 1e-6 on the tiled kernel, with the error report
+    recorded in out.json)."""
+    n_y, n_z = 4100, 1024  # 64 x 64 + 4 copies of point 0
+    out, recs = generated_run(tmp_path, ["-x", "16", "-y", str(n_y), "-z", str(n_z), "-r"])
+    v = np.fromfile(out / "v0.dat", "<f8").reshape(-1, 3)
+    w = np.fromfile(out / "w0_0.dat", "<f8").reshape(-1, 3)
+    for pts, n, width in ((v, n_y, 512 * 7e-6), (w, n_z, 1.2e-3)):
+        z = pts[0, 2]  # 0, and the object plane's z = 10^log10(0.35) for w
+        sites, dx, dy = lattice_sites(n, width, z, 1.0, (0.0, 0.0))
+        body = len(sites)
+        d = pts[:body, :2] - sites[:, :2]
+        assert np.all(np.abs(d[:, 0]) <= 0.4875 * dx * (1 + 1e-12))
+        assert np.all(np.abs(d[:, 1]) <= 0.4875 * dy * (1 + 1e-12))
+        assert np.abs(d[:, 0]).max() > 0.4 * dx and np.abs(d[:, 1]).max() > 0.4 * dy
+        # roughly uniform offsets: mean near 0, spread near that of U(-0.4875, 0.4875)
+        assert abs(d[:, 0].mean() / dx) < 0.02 and abs(d[:, 0].std() / dx - 0.975 / 12 ** 0.5) < 0.02
+        assert np.all(pts[:, 2] == z)
+        assert np.array_equal(pts[body:], np.repeat(pts[:1], n - body, 0))
+        assert pkg.lattice_layout(pts)[:2] == (0, 0)
+    assert abs(w[0, 2] - 0.35) <= 1e-15
+    assert recs["y0"]["randomized"] == 1 and recs["z0_0"]["randomized"] == 1
+    u = np.fromfile(out / "u0.dat", "<f8")
+    xa, xp = read_polar(out, "x0")
+    ya, yp = read_polar(out, "y0")
+    ry = oracle.transform_full(-1, LAM_REF, xa, xp, u, v)
+    ra, rp, _ = oracle.normalize_polar(ry)
+    assert np.max(np.abs(ya - ra)) < 1e-12
+    assert np.linalg.norm(ya * np.exp(1j * yp) - ra * np.exp(1j * rp)) / np.linalg.norm(ra) < 1e-12
+    za, zp = read_polar(out, "z0_0")
+    rz = oracle.transform_full(+1, LAM_REF, ya, yp, v, w)
+    rza, rzp, _ = oracle.normalize_polar(rz)
+    assert np.linalg.norm(za * np.exp(1j * zp) - rza * np.exp(1j * rzp)) / np.linalg.norm(rza) < 1e-12
+    assert "rel_l2_fp64" not in recs["z0_0"]
+    # FP32-split over the same jittered projector (its own run: the jitter
+    # stream restarts with the process, so the positions are the same)
+    out2, recs2 = generated_run(tmp_path / "fp32", ["-x", "16", "-y", str(n_y), "-z", str(n_z),
+                                                    "-r"], env={"HOLO_PRECISION": "fp32split"})
+    assert np.array_equal(np.fromfile(out2 / "v0.dat", "<f8").reshape(-1, 3), v)
+    ya2, yp2 = read_polar(out2, "y0")
+    za2, zp2 = read_polar(out2, "z0_0")
+    z2 = za2 * np.exp(1j * zp2)
+    rz2 = oracle.transform_full(+1, LAM_REF, ya2, yp2, v, w)
+    rza2, rzp2, _ = oracle.normalize_polar(rz2)
+    err = np.linalg.norm(z2 - rza2 * np.exp(1j * rzp2)) / np.linalg.norm(rza2)
+    assert err <= 1e-6
+    assert 0 < recs2["z0_0"]["rel_l2_fp64"] <= 1e-6
+
+
+@pytest.mark.gpu
+def test_generated_planes_interpolated(tmp_path, oracle, workloads):
+    """-X 2 -Z 2 with ranges (cuda/main.cu:125-199): object plane i sits at
+    z = gerp(0.35 -> 0.25, i) (geometric interpolation, cuda/algebra.h:140-150)
+    with lerp'd width and offsets; projection plane j at z offset
+    lerp(0 -> 0.1, j) from it, width gerp(1.2 -> 2.4 mm, j), shifted into one
+    quadrant by -q.  Every written position set is the generator's, every
+    out.json record carries the plane's offsets and width, and every y<i> and
+    z<i>_<j> is the oracle's normalised field over the written inputs."""
+    args = ["-X", "2", "-Z", "2", "-x", "12", "-y", "1024", "-z", "256",
+            "-w", "0.35", "-W", "0.25", "-u", "1e-4", "-U", "3e-4", "-v", "-2e-4", "-V", "0",
+            "-o", "1e-3", "-O", "2e-3", "-n", "1.2e-3", "-N", "2.4e-3", "-m", "0", "-M", "0.1",
+            "-q"]
+    out, recs = generated_run(tmp_path, args)
+    assert sorted(recs) == ["x0", "x1", "y0", "y1", "z0_0", "z0_1", "z1_0", "z1_1"]
+    rec = lambda r, k: pytest.approx(r, rel=1e-5, abs=1e-12)  # out.json keeps 6 digits
+    gerp = lambda a, b, r: 10 ** ((1 - r) * np.log10(a) + r * np.log10(b))
+    for i in range(2):
+        zo = gerp(0.35, 0.25, i)
+        xo, yo = 1e-4 + i * 2e-4, -2e-4 + i * 2e-4
+        xr = recs[f"x{i}"]
+        assert xr["z_offset"] == rec(zo, 0) and xr["x_offset"] == rec(xo, 0)
+        assert xr["y_offset"] == rec(yo, 0) and xr["width"] == rec(1e-3 * (1 + i), 0)
+        u = np.fromfile(out / f"u{i}.dat", "<f8").reshape(-1, 3)
+        assert np.allclose(np.hypot(u[:, 0] - xo, u[:, 1] - yo), 0.5e-3 * (1 + i), rtol=0,
+                           atol=1e-15)
+        assert np.all(u[:, 2] == u[0, 2]) and u[0, 2] == pytest.approx(zo, rel=1e-15)
+        v = np.fromfile(out / f"v{i}.dat", "<f8").reshape(-1, 3)
+        np.testing.assert_allclose(v, workloads.reference_plane(1024, 512 * 7e-6, 0.0),
+                                   rtol=0, atol=1e-16)
+        xa, xp = read_polar(out, f"x{i}")
+        assert np.allclose(xa, np.linspace(1, 0, 12)) and np.all(xp == 0)
+        ya, yp = read_polar(out, f"y{i}")
+        ra, rp, _ = oracle.normalize_polar(oracle.transform_full(-1, LAM_REF, xa, xp, u, v))
+        assert np.max(np.abs(ya - ra)) < 1e-12
+        assert np.linalg.norm(ya * np.exp(1j * yp) - ra * np.exp(1j * rp)) / np.linalg.norm(ra) < 1e-12
+        for j in range(2):
+            width = gerp(1.2e-3, 2.4e-3, j)
+            zr = recs[f"z{i}_{j}"]
+            xz, yz, zz = xo + width / 2, yo + width / 2, zo + 0.1 * j
+            assert zr["width"] == rec(width, 0) and zr["x_offset"] == rec(xz, 0)
+            assert zr["y_offset"] == rec(yz, 0) and zr["z_offset"] == rec(zz, 0)
+            w = np.fromfile(out / f"w{i}_{j}.dat", "<f8").reshape(-1, 3)
+            np.testing.assert_allclose(
+                w, workloads.reference_plane(256, width, zz, 1.0, (xz, yz)), rtol=0, atol=1e-15)
+            za, zp = read_polar(out, f"z{i}_{j}")
+            rza, rzp, _ = oracle.normalize_polar(oracle.transform_full(+1, LAM_REF, ya, yp, v, w))
+            assert np.max(np.abs(za - rza)) < 1e-12
+            assert (np.linalg.norm(za * np.exp(1j * zp) - rza * np.exp(1j * rzp))
+                    / np.linalg.norm(rza)) < 1e-12

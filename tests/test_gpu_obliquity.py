@@ -147,7 +147,8 @@ def test_unknown_flag_rejected(pkg, ctx):
     pos = np.array([0.0, 0.0, 0.0])
     tgt = np.array([0.0, 0.0, 0.1])
     st = L.holo_transform_full(ctx.handle, pkg.FP64, 1, LAM, one.ctypes.data, one.ctypes.data,
-                               pos.ctypes.data, 1, tgt.ctypes.data, 1, y.ctypes.data, 0x40)
+                               pos.ctypes.data, 1, tgt.ctypes.data, 1, y.ctypes.data, None,
+                               0x40)
     assert st == 1 and b"flag" in L.holo_last_error()
     st = L.holo_mc_indices(ctx.handle, 0, 10, 0, 1, 1, y.ctypes.data, pkg.OBLIQUITY)
     assert st == 1

@@ -137,6 +137,44 @@ def test_fp64_rows_path_same_bits(pkg, ctx, oracle, workloads, shape, r_f64, spl
     assert rel_l2(y2, oracle.transform_full(+1, LAM, amp, phase, u2, v)) <= 1e-12
 
 
+@pytest.mark.parametrize("case", ["rows", "rows_padded", "planar", "general", "line3"])
+@pytest.mark.parametrize("obliquity", [False, True])
+def test_fp64_device_layout_same_bits(pkg, ctx, oracle, workloads, case, obliquity):
+    """transform_full on device pointers finds the source layout on the device
+    (one z; rows of one x, the last possibly shorter) and takes the same
+    kernel paths as the host scan of a host-pointer call: identical bits,
+    also with the options that switch the paths off; within 1e-12 of the
+    oracle."""
+    import torch
+    if case.startswith("rows"):
+        u = workloads.grid(40, 52, 10e-6, 0.0)
+        if case == "rows_padded":
+            u = np.concatenate([u, np.repeat(u[:1], 67, 0)])  # more than one row of copies
+    elif case == "planar":
+        g = np.random.default_rng(5)
+        u = np.zeros((3000, 3))
+        u[:, :2] = (g.random((3000, 2)) - 0.5) * 3e-3
+    elif case == "line3":  # a run of 3 sources with one x: not rows (too short)
+        u = workloads.grid(700, 3, 10e-6, 0.0)
+    else:
+        g = np.random.default_rng(6)
+        u = (g.random((3000, 3)) - 0.5) * np.array([3e-3, 3e-3, 1e-4])
+    g = np.random.default_rng(len(u))
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    v = workloads.grid(30, 41, 7.56e-6, 0.3)
+    dev = torch.device("cuda", 0)
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    for planar, rows in ((1, 1), (1, 0), (0, 0)):
+        pkg.set_option("f64_planar", planar)
+        pkg.set_option("f64_rows", rows)
+        yh = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64, obliquity=obliquity)
+        yd = ctx.transform_full(T(amp), T(phase), T(u), T(v), -1, LAM, pkg.FP64,
+                                obliquity=obliquity).cpu().numpy()
+        assert np.array_equal(yh, yd), (planar, rows)
+    ref = oracle.transform_full(-1, LAM, amp, phase, u, v, obliquity=obliquity)
+    assert rel_l2(yd, ref) <= 1e-12
+
+
 def test_fp64_golden_fixture(pkg, ctx):
     """Committed golden vectors (tests/golden/plain_small_*.dat, written by the
     oracle) of the 32x32 -> 64x64 case, sign +1."""

@@ -119,7 +119,7 @@ def test_plan_shared_by_two_contexts(pkg, workloads):
             pkg.check(lib.holo_plan_transform(c.handle, plan._h, pkg.FP32_SPLIT, +1, LAM,
                                               w[0].data_ptr(), w[1].data_ptr(),
                                               dv.data_ptr(), ctypes.c_size_t(v.shape[0]),
-                                              y.data_ptr(), pkg.DEVICE_POINTERS))
+                                              y.data_ptr(), None, pkg.DEVICE_POINTERS))
 
         ys = [torch.empty((v.shape[0], 2), dtype=torch.float64, device="cuda")
               for _ in range(4)]
