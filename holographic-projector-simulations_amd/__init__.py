@@ -91,6 +91,8 @@ def lib() -> ctypes.CDLL:
     L.holo_get_option.restype = ctypes.c_long
     L.holo_transform_full.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, vp, vp,
                                       ctypes.c_uint]
+    L.holo_transform_full_noalloc.argtypes = [i, d, vp, vp, vp, sz, vp, sz, vp, vp,
+                                              ctypes.c_uint]
     L.holo_transform_mc.argtypes = [vp, i, i, d, vp, vp, vp, sz, vp, sz, u32, u64, vp, vp,
                                     ctypes.c_uint]
     L.holo_mc_indices.argtypes = [vp, u64, u64, sz, sz, u32, vp, ctypes.c_uint]
@@ -120,7 +122,8 @@ def lib() -> ctypes.CDLL:
     L.holo_host_sqrt_inv.argtypes = [d, dp, dp]
     L.holo_host_sqrt_inv.restype = None
     for name in ("holo_create", "holo_set_stream", "holo_synchronize", "holo_set_option",
-                 "holo_transform_full", "holo_transform_mc", "holo_mc_indices",
+                 "holo_transform_full", "holo_transform_full_noalloc", "holo_transform_mc",
+                 "holo_mc_indices",
                  "holo_normalize_polar", "holo_plan_create", "holo_plan_info", "holo_plan_lattice",
                  "holo_plan_transform", "holo_plan_transform_complex", "holo_plan_adjoint",
                  "holo_read_dat", "holo_write_dat"):

@@ -144,6 +144,39 @@ struct holo_plan {
 
 namespace {
 
+// ---- scratch of holo_transform_full_noalloc --------------------------------
+// The split partials and source-layout words of the no-allocation entry point
+// live in this file-scope device array: one copy per device, allocated when
+// the code object loads on that device and held until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: kernels
+// and copies get the address hipGetSymbolAddress returns for the current
+// device, kept per device in g_scratch.
+constexpr size_t kNoallocScratchBytes = HOLO_NOALLOC_SCRATCH_BYTES;
+constexpr size_t kNoallocLayoutBytes = 256;  // 4 layout words, padded
+constexpr int kMaxDevices = 64;
+static_assert(kNoallocScratchBytes % 256 == 0 && kNoallocScratchBytes > kNoallocLayoutBytes,
+              "scratch size");
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current.  The
+// first call on a device looks it up; no call allocates.
+holo_status noalloc_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(HOLO_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    HOLO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(HOLO_EDEVICE, "hipGetSymbolAddress returned no address for scratch");
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return HOLO_OK;
+}
+
 holo_status use_device(holo_ctx* ctx) {
   if (!ctx) return fail(HOLO_EINVAL, "null context");
   HOLO_HIP(hipSetDevice(ctx->device));
@@ -1001,6 +1034,8 @@ holo_status holo_create(holo_ctx** out, int device, void* stream) {
     return fail(HOLO_EDEVICE, std::string("libholo is built for gfx950, device is ") +
                                   prop.gcnArchName);
   HOLO_HIP(hipSetDevice(device));
+  void* scratch_ptr = nullptr;  // the no-allocation entry point's scratch, looked up once
+  HOLO_TRY(noalloc_scratch(device, &scratch_ptr));
   holo_ctx* c = new holo_ctx();
   c->device = device;
   if (stream) {
@@ -1160,6 +1195,38 @@ holo_status holo_transform_full(holo_ctx* ctx, holo_precision precision, int sig
     HOLO_HIP(hipStreamSynchronize(ctx->stream));  // plan buffers die here
   }
   return finish_out(ctx, y, dy, 2 * M, dev);
+}
+
+holo_status holo_transform_full_noalloc(int sign, double lambda, const double* amp,
+                                        const double* phase, const double* u, size_t N,
+                                        const double* v, size_t M, double* y, void* stream,
+                                        unsigned flags) {
+  HOLO_TRY(check_flags(flags, HOLO_DEVICE_POINTERS | HOLO_OBLIQUITY));
+  if (sign != 1 && sign != -1) return fail(HOLO_EINVAL, "sign must be +1 or -1");
+  if (!(lambda > 0.0) || !std::isfinite(lambda))
+    return fail(HOLO_EINVAL, "lambda must be positive and finite");
+  if (N > kMaxCount || M > kMaxCount) return fail(HOLO_EINVAL, "N and M must be below 2^31");
+  if (M == 0) return HOLO_OK;
+  if (!v || !y) return fail(HOLO_EINVAL, "null target or output array");
+  if (N && (!amp || !phase || !u)) return fail(HOLO_EINVAL, "null source array");
+  int device = 0;
+  HOLO_HIP(hipGetDevice(&device));
+  void* base = nullptr;
+  HOLO_TRY(noalloc_scratch(device, &base));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (N == 0) {
+    HOLO_HIP(holo::launch_zero(y, 2 * M, st));
+    return HOLO_OK;
+  }
+  uint32_t* layout = static_cast<uint32_t*>(base);
+  double2* partial =
+      reinterpret_cast<double2*>(static_cast<unsigned char*>(base) + kNoallocLayoutBytes);
+  const size_t cap = (kNoallocScratchBytes - kNoallocLayoutBytes) / sizeof(double2);
+  const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
+  HOLO_HIP(holo::launch_superpose_f64_auto(sign, holo::kPolar, flags & HOLO_OBLIQUITY, amp,
+                                           phase, u, N, v, M, K, reinterpret_cast<double2*>(y),
+                                           partial, cap, layout, st));
+  return HOLO_OK;
 }
 
 holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,

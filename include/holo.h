@@ -128,6 +128,30 @@ holo_status holo_transform_full(holo_ctx *ctx, holo_precision precision,
                                 size_t N, const double *v, size_t M,
                                 double *y, double *err_rel_l2, unsigned flags);
 
+/* ---- no-allocation variant of holo_transform_full -------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops and harnesses that
+ * forbid hipMalloc.  Computes what holo_transform_full computes with
+ * HOLO_FP64 and device pointers (all arrays are device memory on the current
+ * device; HOLO_DEVICE_POINTERS may be given and changes nothing;
+ * HOLO_OBLIQUITY is accepted), ordered on `stream` (a hipStream_t of the
+ * current device, NULL = its null stream) without synchronising.  It needs no
+ * context and calls no allocating routine: its scratch (the source-split
+ * partial fields and four words of source layout) is a static device array of
+ * HOLO_NOALLOC_SCRATCH_BYTES per device, held from the library's first use
+ * on a device (holo_create, or the first call of this function there) until
+ * the process ends.  The call uses at most (HOLO_NOALLOC_SCRATCH_BYTES - 256)
+ * / (16 M) source splits, so every M below 2^31 fits (one split writes y
+ * directly).  Every call on a device shares that device's array: calls that
+ * overlap on one device (other streams, other threads) race and must be
+ * ordered by the caller.  Results equal holo_transform_full's to FP64
+ * rounding of the split sums (bit for bit when both use the same splits). */
+#define HOLO_NOALLOC_SCRATCH_BYTES (32u << 20)
+holo_status holo_transform_full_noalloc(int sign, double lambda, const double *amp,
+                                        const double *phase, const double *u, size_t N,
+                                        const double *v, size_t M, double *y,
+                                        void *stream, unsigned flags);
+
 /* ---- Monte Carlo: transform ---------------------------------------------
  * The reference's transform() (named in README.md:127, absent from the
  * snapshot; remnants cuda/kernel.cu:101-106,197-212): each target m draws K
