@@ -161,6 +161,12 @@ def main():
                     help="distance of the 1024^2 target plane of the fp32 and mc sections "
                          "(default 0.35 m); nearer planes exercise the lattice kernel's "
                          "accuracy bounds")
+    ap.add_argument("--mc-passes", type=int, default=0,
+                    help="library option mc_passes: source ranges of per-target Monte Carlo, "
+                         "one launch each (0 = the library's choice, 1 = one direct launch)")
+    ap.add_argument("--no-series", action="store_true",
+                    help="library option grid_series = 0: near-field pairs of the lattice "
+                         "kernels take the exact path instead of the q-series")
     ap.add_argument("--target-waves", type=int, default=0,
                     help="library option target_waves (waves a launch aims for when it splits "
                          "sources; 0 = the library default)")
@@ -183,6 +189,10 @@ def main():
     holo.set_option("grid", 0 if a.lattice_tiles == 0 else 1)
     if a.target_waves:
         holo.set_option("target_waves", a.target_waves)
+    if a.mc_passes:
+        holo.set_option("mc_passes", a.mc_passes)
+    if a.no_series:
+        holo.set_option("grid_series", 0)
     import importlib
     wl = importlib.import_module("holo_amd.workloads")
     w = wl.chain_xyz(0)
@@ -232,9 +242,12 @@ def main():
         plan3 = holo.Plan(ctx, u3)
         y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
         y32 = torch.empty_like(y64)
-        log(f"[bench] plane {src_desc}: FP64, 1 warm-up + {R} runs")
-        t64 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64,
-                                                  out=y64), R)
+        if "fp32" in sections:
+            log(f"[bench] plane {src_desc}: FP64, 1 warm-up + {R} runs")
+            t64 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64,
+                                                      out=y64), R)
+        else:  # the brute-force field the Monte Carlo estimates are compared with
+            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP64, out=y64)
         f64 = torch.view_as_complex(y64).cpu().numpy()
         if "fp32" in sections:
             log(f"[bench] plane {src_desc}: FP32-split, 1 warm-up + {R} runs")

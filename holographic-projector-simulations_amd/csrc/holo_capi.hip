@@ -1113,8 +1113,13 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid16 = value != 0;
   } else if (n == "grid32") {
     t.grid32 = value != 0;
+  } else if (n == "grid_series") {
+    t.grid_series = value != 0;
   } else if (n == "mc_lattice") {
     t.mc_lattice = value != 0;
+  } else if (n == "mc_passes") {
+    if (value < 0 || value > 64) return fail(HOLO_EINVAL, "mc_passes must be in [0, 64]");
+    t.mc_passes = (int)value;
   } else {
     return fail(HOLO_EINVAL, "unknown option " + n);
   }
@@ -1135,7 +1140,9 @@ long holo_get_option(const char* name) {
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
   if (n == "grid32") return t.grid32 ? 1 : 0;
+  if (n == "grid_series") return t.grid_series ? 1 : 0;
   if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;
+  if (n == "mc_passes") return t.mc_passes;
   return -1;
 }
 

@@ -159,7 +159,10 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
   sincos_rev_f32(f, sn, cs);
   if (SIGN < 0) sn = -sn;
   const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
-  const float2 w = __builtin_bit_cast(float2, p.w);
+  // the FP32 weight packed in .w, unpacked by integer ops (no trip through memory)
+  const uint64_t wb = __builtin_bit_cast(uint64_t, p.w);
+  const float2 w = make_float2(__builtin_bit_cast(float, (uint32_t)wb),
+                               __builtin_bit_cast(float, (uint32_t)(wb >> 32)));
   const float pr = idf * cs, pi = idf * sn;
   fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
   fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));

@@ -62,6 +62,7 @@ constexpr float kFourPi = 12.566370614359172f;
 struct GridBounds {
   double cx0, cx1, cy0, cy1, z;
   float rho_x, rho_y;
+  int series = 0;  // 16-row kernels: near pairs take the series path where it holds
 };
 
 // The accuracy bounds of one (target, tile), or of a target against every
@@ -420,6 +421,97 @@ __device__ __forceinline__ float hi_bits(double x) {
   return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, (float)x) & mask);
 }
 
+// Near-field path of the 16-row kernels.  A (target, tile) pair that fails
+// grid16_pairs_ok (targets within ~0.2 m of a 1024^2 plane: the q-series
+// truncation and the first-order cross term exceed their bounds) is still far
+// enough for the q-series per pair.  With r = v - c, D = |r|, the source
+// (p_x k_a, p_y k_b) of the tile has q = P_a + Q_b,
+//   P_a = p_x^2 k_a^2 - 2 r_x p_x k_a,  Q_b = p_y^2 k_b^2 - 2 r_y p_y k_b,
+//   (d - D) / lambda = (q / (2D) - q^2 / (8 D^3) + q^3 / (16 D^5) - ...) / lambda
+//   1 / d = (1 / D)(1 - q / (2 D^2) + 3 q^2 / (8 D^4) - ...)
+//   |dz| / d^2 = (|r_z| / D^2)(1 - q / D^2 + q^2 / D^4 - ...)
+// The linear phase s_x k_a (s_x = -2 alpha r_x p_x cycles per pitch, alpha =
+// 1 / (2 D lambda), several cycles across a tile) is split as in the GEMM path:
+// s_hi k_a and its fraction are exact in FP32, the rest of the phase is below
+// a cycle.  Per row a: R_a = fract(sx_hi k_a) + sx_lo k_a + alpha p_x^2 k_a^2;
+// per column b: C_b = fract(sy_hi k_b) + sy_lo k_b + alpha p_y^2 k_b^2 + f
+// (f = frac(D / lambda)); per pair, in FP32,
+//   q = P_a + Q_b,  t = R_a + C_b + q^2 (c2 + c3 q),  am = 1 + q (k1 + k2 q),
+//   S_a += w_ab am e^{i s 2 pi t},
+// each row's S_a added in FP64, the tile's sum scaled by A = 1 / D (or
+// |r_z| / D^2).  Dropped: the q^4 phase term and the amplitude's third order.
+
+// The series path's bounds for one (target, tile): the FP32 rounding of the
+// phase (as grid16_pairs_ok), the q^4 term, 5 q^4 / (128 D^7 lambda) cycles,
+// and the amplitude's third order, (q / D^2)^3 (5/16, or 1 with obliquity).
+// NaN fails.
+template <bool OBL>
+__device__ __forceinline__ bool series_pairs_ok(float qmax, float rx2, float ry2, float ia,
+                                                float idf) {
+#pragma clang fp contract(off)
+  const float sq = qmax * idf * idf;  // q / D^2
+  const float round_err = 6e-8f * (3.f + 0.5f * ia * (rx2 + ry2));
+  const float trunc4 = 0.0390625f * qmax * sq * sq * sq * ia;
+  const float amp3 = (OBL ? 1.f : 0.3125f) * sq * sq * sq;
+  return round_err <= kGuardCycles && trunc4 <= 0.05f * kGuardCycles && amp3 <= 1e-7f;
+}
+
+template <int BITS>
+__device__ __forceinline__ float hi_bits(double x);
+
+// The 256 NKC pairs of one tile (weights w: 256 slots per 16 x 16 part, slot
+// 16 a + b) for one target, added into (ar, ai).
+template <int SIGN, bool OBL, int NKC>
+__device__ __forceinline__ void series_tile_pairs(const float2* __restrict__ w, double rx,
+                                                  double ry, double rz, double dref, double id,
+                                                  double inv_lambda, double px, double py,
+                                                  double& ar, double& ai) {
+#pragma clang fp contract(off)
+  const double al = 0.5 * id * inv_lambda;
+  const double sx = -2.0 * al * rx * px, sy = -2.0 * al * ry * py;
+  const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
+  const float sxl = (float)(sx - (double)sxh), syl = (float)(sy - (double)syh);
+  const float ax = (float)(al * px * px), ay = (float)(al * py * py);
+  const float mx = (float)(-2.0 * rx * px), my = (float)(-2.0 * ry * py);
+  const float px2 = (float)(px * px), py2 = (float)(py * py);
+  const float idf = (float)id, id2 = idf * idf, ia = (float)(id * inv_lambda);
+  const float c2 = -0.125f * ia * id2, c3 = 0.0625f * ia * id2 * id2;
+  const float k1 = OBL ? -id2 : -0.5f * id2, k2 = OBL ? id2 * id2 : 0.375f * id2 * id2;
+  const float f = (float)frac_cycle(dref * inv_lambda);
+  const double A = OBL ? __builtin_fabs(rz) * id * id : id;
+  for (int c = 0; c < NKC; ++c) {
+    float Cb[16], Qb[16];
+#pragma unroll
+    for (int b = 0; b < 16; ++b) {
+      const float k = (float)(16 * c + b) - (8.f * NKC - 0.5f), kk = k * k;
+      Qb[b] = __builtin_fmaf(my, k, py2 * kk);
+      Cb[b] = __builtin_amdgcn_fractf(syh * k) + __builtin_fmaf(syl, k, ay * kk) + f;
+    }
+    for (int a = 0; a < 16; ++a) {
+      const float k = (float)a - 7.5f, kk = k * k;
+      const float Pa = __builtin_fmaf(mx, k, px2 * kk);
+      const float Ra = __builtin_amdgcn_fractf(sxh * k) + __builtin_fmaf(sxl, k, ax * kk);
+      const float2* wr = w + 256 * c + 16 * a;
+      float sr = 0.f, si = 0.f;
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const float2 wb = wr[b];
+        const float q = Pa + Qb[b];
+        const float t = __builtin_fmaf(q * q, __builtin_fmaf(c3, q, c2), Ra + Cb[b]);
+        const float am = __builtin_fmaf(q, __builtin_fmaf(k2, q, k1), 1.f);
+        float sn, cs;
+        sincos_rev_f32(t, sn, cs);
+        if (SIGN < 0) sn = -sn;
+        const float pr = am * cs, pi = am * sn;
+        sr = __builtin_fmaf(wb.x, pr, __builtin_fmaf(-wb.y, pi, sr));
+        si = __builtin_fmaf(wb.x, pi, __builtin_fmaf(wb.y, pr, si));
+      }
+      ar += A * sr;
+      ai += A * si;
+    }
+  }
+}
+
 // MFMA operands of the 16 x 16 tiles' weights, one wave per tile: for row
 // block rb, wq[(6t + 3rb + q) * 64 + lane] holds, for lane l = 16 g + i,
 // row i = 2a' + c (a = 8 rb + a') and columns 4g .. 4g + 3 (re, im) of the real
@@ -603,9 +695,19 @@ __device__ __forceinline__ void grid16_tiles(
       lds.k[1][wv][lane] = k1;
       lds.k[2][wv][lane] = k2;
       lds.k[3][wv][lane] = k3;
-      if (near)
-        exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC, wc + (size_t)t * 256 * NKC,
-                                    256 * NKC, vx, vy, vz, inv_lambda, ex_r, ex_i);
+      if (near) {
+        const float Pm = 2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
+                    Qm = 2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2;
+        if (gb.series && series_pairs_ok<OBL>(Pm + Qm, rx2, ry2, 2.f * alf, idf))
+          series_tile_pairs<SIGN, OBL, NKC>(wc + (size_t)t * 256 * NKC, rx, ry, rz, dref, id,
+                                            inv_lambda, px, py, ex_r, ex_i);
+        else
+          exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC,
+                                      wc + (size_t)t * 256 * NKC, 256 * NKC, vx, vy, vz,
+                                      inv_lambda, ex_r, ex_i);
+      }
+      // a wave whose targets are all near adds nothing through the GEMM
+      if (__builtin_amdgcn_ballot_w64(!near) == 0) continue;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -762,9 +864,10 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
   const dim3 grid((unsigned)((M + kBlock - 1) / kBlock), (unsigned)S);
   double2* out = S > 1 ? partial : y;
   const double il = 1.0 / lambda;
-  const GridBounds gb{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y};
+  const int series = tuning().grid_series ? 1 : 0;
+  const GridBounds gb{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y, series};
   const GridBounds gb16{halves.cx0, halves.cx1, halves.cy0, halves.cy1, halves.z, halves.rho_x,
-                        halves.rho_y};
+                        halves.rho_y, series};
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
       if (nkc == 2)

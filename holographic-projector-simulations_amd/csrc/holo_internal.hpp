@@ -61,7 +61,9 @@ struct Tuning {
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
   bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
   bool grid32 = true;     // ... as 16 x 32 tiles where the columns pair up
+  bool grid_series = true; // 16-row kernels: near-field pairs by the q-series, not exactly
   bool mc_lattice = true; // per-target FP32-split MC: table gather on product grids
+  int mc_passes = 0;      // per-target MC: source ranges, one launch each (0 = automatic)
 };
 
 Tuning& tuning();
@@ -198,6 +200,13 @@ struct McLattice {
   const float2* wc = nullptr;
   const double* xy = nullptr;
 };
+
+// Per-target Monte Carlo runs as one launch per source range of about
+// kMcPassBytes of gathered data, which one XCD's 4 MB L2 holds beside the
+// streamed targets (on the 1024^2 lattice, K = 4096: 39.1 ms in one launch,
+// 23.0 / 22.9 / 25.9 / 32.9 ms in 2 / 3 / 4 / 6 ranges): the number of ranges.
+constexpr size_t kMcPassBytes = size_t(3) << 20;
+uint32_t mc_passes(size_t N, bool fp64);
 
 hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,

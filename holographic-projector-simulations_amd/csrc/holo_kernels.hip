@@ -580,6 +580,42 @@ __device__ __forceinline__ U4 mc_draws(uint32_t k, uint32_t m, uint32_t slo, uin
     return philox4x32_10(U4{k >> 1, m, 0u, kMcStream}, slo, shi);
 }
 
+// A drawn source n as (x, y, z, w): w = amp (FP64, the phase in ph[n]) or the
+// bit-cast FP32 weight (FP32-split).  On a product grid (FP32-split only) the
+// sources of the leading complete rows come from the row and column tables
+// and the weight array, the same values as the packed records.
+struct McSourceView {
+  const double4* pos;
+  McLattice lat;
+  uint32_t ny = 0, body = 0;
+  double inv_ny = 0.0;
+  __device__ McSourceView(const double4* p, const McLattice& l, bool fp64, uint32_t N)
+      : pos(p), lat(l) {
+    uint32_t rows = 0;
+    if (!fp64 && lat.info) {
+      ny = lat.info[0];
+      rows = ny ? lat.info[1] : 0u;
+      if (!rows) ny = 0;
+    }
+    body = min(rows * ny, N);
+    inv_ny = ny ? 1.0 / (double)ny : 0.0;
+  }
+  // branch-free: each component is one load from an address chosen by select
+  __device__ __forceinline__ double4 operator()(uint32_t n) const {
+    const bool tab = n < body;
+    uint32_t i = (uint32_t)((double)n * inv_ny);  // within one of n / ny
+    i -= (uint32_t)(i * ny > n);
+    i += (uint32_t)((i + 1) * ny <= n);
+    const uint32_t j = n - i * ny;
+    const double* rec = &pos[n].x;
+    const double* px = tab ? lat.xy + kLatticeMaxRow + 1 + i : rec;
+    const double* py = tab ? lat.xy + j : rec + 1;
+    const double* pz = tab ? lat.xy + kLatticeMaxRow : rec + 2;
+    const double* pw = tab ? reinterpret_cast<const double*>(lat.wc + n) : rec + 3;
+    return make_double4(*px, *py, *pz, *pw);
+  }
+};
+
 // Per-target estimator; thread i evaluates the target at v[i], whose index
 // (the Philox counter) is tmap[i], or i without a map.
 template <int SIGN, bool FP64, bool OBL, bool SHARED_IDX>
@@ -593,29 +629,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
   const uint32_t m = tmap ? tmap[i] : i;
   const double vx = v[3 * (size_t)i], vy = v[3 * (size_t)i + 1],
                vz = v[3 * (size_t)i + 2];
-  // product-grid sources: (x, y, z, w) from the row/column tables and the
-  // weight array, the same values as the packed record
-  uint32_t ny = 0, rows = 0;
-  if constexpr (!FP64) {
-    if (lat.info) {
-      ny = lat.info[0];
-      rows = ny ? lat.info[1] : 0u;
-      if (!rows) ny = 0;
-    }
-  }
-  const double inv_ny = ny ? 1.0 / (double)ny : 0.0;
-  // sources after the leading complete rows (a padded plane's copies of
-  // source 0) come from the packed records
-  const uint32_t body = rows * ny;
-  const auto source = [&](uint32_t n) -> double4 {
-    if (n >= body) return pos[n];
-    uint32_t i = (uint32_t)((double)n * inv_ny);
-    if (i * ny > n) --i;
-    else if ((i + 1) * ny <= n) ++i;
-    const uint32_t j = n - i * ny;
-    return make_double4(lat.xy[kLatticeMaxRow + 1 + i], lat.xy[j], lat.xy[kLatticeMaxRow],
-                        __builtin_bit_cast(double, lat.wc[n]));
-  };
+  const McSourceView source(pos, lat, FP64, N);
   double ar = 0.0, ai = 0.0;
   float fr = 0.f, fi = 0.f;
   // 8 samples per step: 4 Philox blocks, then 8 independent gathers in flight
@@ -670,6 +684,118 @@ __global__ __launch_bounds__(kBlock) void k_mc(
   ar += fr;
   ai += fi;
   y[i] = make_double2(ar * scale, ai * scale);
+}
+
+// Per-target Monte Carlo over source ranges.  A per-target draw gathers one
+// random source; from a 1024^2 plane the gathered data (8 MB of weights on a
+// lattice, 32 MB of records otherwise) does not fit the 4 MB L2 of an XCD, and
+// the misses bound k_mc.  k_mc_pass takes only the draws whose source index
+// lies in [lo, hi), a range whose data fits one L2, and the host launches it
+// once per range: every launch regenerates its targets' Philox streams (the
+// indices are k_mc's), queues each lane's in-range draws in that lane's LDS
+// ring and evaluates them 8 at a time, every lane of the wave taking 8 once
+// each active lane holds 8; a ring that could overflow in the next round
+// forces a partial batch (lanes with fewer entries idle in it).  Each launch
+// adds its sums into y[m] (FP64): the first writes, the last scales by N / K.
+// The summation order of a target is fixed by its indices and the ranges.
+constexpr uint32_t kMcQueue = 32;  // ring slots per lane, a power of two
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
+  return x;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, off, 64));
+  return x;
+}
+
+template <int SIGN, bool FP64, bool OBL>
+__global__ __launch_bounds__(kBlock) void k_mc_pass(
+    const double4* __restrict__ pos, const double* __restrict__ ph, uint32_t N,
+    const double* __restrict__ v, uint32_t M, uint32_t K, uint32_t seed_lo, uint32_t seed_hi,
+    double Kw, double inv_lambda, McLattice lat, uint32_t lo, uint32_t hi, int first,
+    double scale, double2* __restrict__ y) {
+  __shared__ uint32_t s_q[kMcQueue][kBlock];  // [slot][lane]: one bank per lane
+  const uint32_t t = threadIdx.x, m = blockIdx.x * kBlock + t;
+  const bool active = m < M;
+  if (__ballot(active) == 0) return;  // the whole wave is past M (no barriers below)
+  const uint32_t mm = active ? m : M - 1;
+  const double vx = v[3 * (size_t)mm], vy = v[3 * (size_t)mm + 1], vz = v[3 * (size_t)mm + 2];
+  const McSourceView source(pos, lat, FP64, N);
+  double ar = 0.0, ai = 0.0;
+  uint32_t head = 0, tail = 0;
+  // the next `take` (<= 8) queued draws of this lane; idle slots read source
+  // lo and their terms are dropped by a select (never multiplied by zero: a
+  // target at that source would turn 0 * inf into NaN)
+  const auto batch = [&](uint32_t take) __attribute__((always_inline)) {
+    uint32_t idx[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const uint32_t q = s_q[(head + h) & (kMcQueue - 1)][t];
+      idx[h] = (active && (uint32_t)h < take) ? q : lo;
+    }
+    double4 p[8];
+    double phv[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      p[h] = source(idx[h]);
+      if constexpr (FP64) phv[h] = ph[idx[h]];
+    }
+    float fr = 0.f, fi = 0.f;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      const bool use = (uint32_t)h < take;
+      if constexpr (FP64) {
+        const PairF64 q =
+            pair_f64<SIGN, 0, OBL>(p[h].w, phv[h], p[h].x, p[h].y, p[h].z, vx, vy, vz, Kw);
+        double tr = 0.0, ti = 0.0;
+        accumulate_phasor<0>(q.r, q.arg, tr, ti);
+        ar += use ? tr : 0.0;
+        ai += use ? ti : 0.0;
+      } else {
+        float tr = 0.f, ti = 0.f;
+        mc_pair_f32s<SIGN, OBL>(p[h], vx, vy, vz, inv_lambda, tr, ti);
+        fr += use ? tr : 0.f;
+        fi += use ? ti : 0.f;
+      }
+    }
+    if constexpr (!FP64) {
+      ar += fr;
+      ai += fi;
+    }
+    head += take;
+  };
+  for (uint32_t k = 0; k < K; k += 8) {
+    uint32_t idx[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const U4 o = mc_draws<false>(k + 2 * h, mm, seed_lo, seed_hi);
+      idx[2 * h] = lemire_index(o.x, o.y, N);
+      idx[2 * h + 1] = lemire_index(o.z, o.w, N);
+    }
+#pragma unroll
+    for (int h = 0; h < 8; ++h)
+      if (active && k + h < K && idx[h] - lo < hi - lo) {
+        s_q[tail & (kMcQueue - 1)][t] = idx[h];
+        ++tail;
+      }
+    // full batches while every active lane holds 8 (idle lanes count as full)
+    while (wave_min_u32(active ? tail - head : 0xffffffffu) >= 8) batch(8);
+    // every ring holds at most kMcQueue - 8 entries before the next round
+    if (wave_max_u32(active ? tail - head : 0u) > kMcQueue - 8) batch(min(8u, tail - head));
+  }
+  while (wave_max_u32(active ? tail - head : 0u) > 0) batch(min(8u, tail - head));
+  if (active) {
+    if (!first) {
+      const double2 old = y[m];
+      ar += old.x;
+      ai += old.y;
+    }
+    y[m] = make_double2(ar * scale, ai * scale);
+  }
 }
 
 // Batched Monte Carlo: one workgroup = one group of kMcGroup targets
@@ -733,6 +859,17 @@ __global__ __launch_bounds__(kBlock) void k_mc_shared(
   }
 }
 
+uint32_t mc_passes(size_t N, bool fp64) {
+  const int opt = tuning().mc_passes;
+  if (opt > 0) return (uint32_t)std::min<size_t>((size_t)opt, std::max<size_t>(N, 1));
+  // the gathered bytes per source: the FP32 weight on a lattice (the records
+  // of other point sets are 4x larger and get 4x fuller ranges), or the FP64
+  // record and phase
+  const size_t bytes = N * (fp64 ? 40 : 8);
+  const size_t p = (bytes + kMcPassBytes - 1) / kMcPassBytes;
+  return (uint32_t)std::min<size_t>(std::max<size_t>(p, 1), 8);
+}
+
 hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const double4* pos,
                      const double* ph, size_t N, const double* v, size_t M,
                      uint32_t K, uint64_t seed, double K_wave, double lambda,
@@ -743,16 +880,25 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
   const double il = 1.0 / lambda;
   const unsigned blocks = shared ? (unsigned)((M + kMcGroup - 1) / kMcGroup)
                                  : (unsigned)((M + kBlock - 1) / kBlock);
+  const uint32_t passes = shared ? 1u : mc_passes(N, fp64);
   with_sign(sign, [&](auto sg) {
     with_bool(fp64, [&](auto f64) {
       with_bool(obliquity, [&](auto obl) {
         if (shared)
           k_mc_shared<sg.value, (bool)f64.value, (bool)obl.value><<<blocks, kBlock, 0, stream>>>(
               pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, scale, y);
-        else
+        else if (passes <= 1)
           k_mc<sg.value, (bool)f64.value, (bool)obl.value, false>
               <<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, nullptr, K,
                                               slo, shi, K_wave, il, scale, lat, y);
+        else
+          for (uint32_t b = 0; b < passes; ++b)
+            k_mc_pass<sg.value, (bool)f64.value, (bool)obl.value>
+                <<<blocks, kBlock, 0, stream>>>(
+                    pos, ph, (uint32_t)N, v, (uint32_t)M, K, slo, shi, K_wave, il, lat,
+                    (uint32_t)((uint64_t)b * N / passes),
+                    (uint32_t)((uint64_t)(b + 1) * N / passes), b == 0 ? 1 : 0,
+                    b + 1 == passes ? scale : 1.0, y);
       });
     });
   });

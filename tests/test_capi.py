@@ -46,7 +46,14 @@ def test_options(pkg):
     with pytest.raises(pkg.HoloError):
         pkg.set_option("no_such_option", 1)
     assert pkg.get_option("no_such_option") == -1
-    for name in ("f64_libm", "f64_planar", "f64_rows", "grid", "grid16", "grid32", "mc_lattice"):
+    assert pkg.get_option("mc_passes") == 0
+    pkg.set_option("mc_passes", 3)
+    assert pkg.get_option("mc_passes") == 3
+    with pytest.raises(pkg.HoloError):
+        pkg.set_option("mc_passes", 65)
+    pkg.set_option("mc_passes", 0)
+    for name in ("f64_libm", "f64_planar", "f64_rows", "grid", "grid16", "grid32", "grid_series",
+                 "mc_lattice"):
         old = pkg.get_option(name)
         assert old in (0, 1)
         pkg.set_option(name, 1 - old)

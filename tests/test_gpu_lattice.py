@@ -362,3 +362,79 @@ def test_lattice_padded_planes(pkg, ctx, oracle, workloads, n, aspect, sign, til
     y2 = plan.transform(amp2, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=(sign < 0))
     if n > x * y:
         assert rel_l2(y2, y32) > 1e-4
+
+
+@pytest.mark.parametrize("z", [0.1, 0.15, 0.25])
+@pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
+def test_lattice_near_field_series(pkg, ctx, oracle, workloads, z, sign, obliquity, tiles):
+    """Targets 0.1 - 0.25 m from a 10 um lattice, up to 8 mm off axis: the
+    GEMM factorisation's bounds fail for most (target, tile) pairs there; the
+    16-row kernels sum them by the q-series to third order per pair (option
+    grid_series) and stay within 1e-6 of the oracle, as does the exact path
+    (grid_series = 0)."""
+    u = workloads.grid(256, 192, 10e-6, 0.0)
+    g = np.random.default_rng(int(z * 100))
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    v = targets(int(z * 1000), 900, z, half=8e-3)
+    sub = np.arange(0, 900, 3)
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v[sub], obliquity=obliquity)
+    plan = ctx.plan(u)
+    for series in (1, 0):
+        pkg.set_option("grid_series", series)
+        y = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+        err = rel_l2(y[sub], ref)
+        print(f"z={z} series={series}: rel L2 {err:.3e}")
+        assert err <= 1e-6
+    pkg.set_option("grid_series", 1)
+
+
+def test_lattice_near_field_full_plane(pkg, ctx, oracle, workloads, tiles):
+    """The benchmark's 1024^2 lattice with the 1024^2 target plane moved to
+    z = 0.1 m: FP32-split within 1e-6 of the FP64 kernel over every target and
+    of the oracle on a stride subset; the library's error report agrees."""
+    import torch
+    st = workloads.plane_1024(0).stages[0]
+    v = st.v.copy()
+    v[:, 2] = 0.1
+    dev = [torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0") for a in (st.amp, st.phase,
+                                                                           st.u, v)]
+    plan = ctx.plan(dev[2])
+    y64 = plan.transform(dev[0], dev[1], dev[3], st.sign, st.lam, pkg.FP64).cpu().numpy()
+    y32 = plan.transform(dev[0], dev[1], dev[3], st.sign, st.lam, pkg.FP32_SPLIT,
+                         report_error=True).cpu().numpy()
+    err_report = float(ctx.last_rel_l2.item())
+    err = rel_l2(y32, y64)
+    sub = workloads.stride_subset(len(v), 48)
+    ref = oracle.transform_full(st.sign, st.lam, st.amp, st.phase, st.u, v, targets=sub)
+    print(f"1024^2 at 0.1 m, tiles {tiles}: rel L2 {err:.3e} (report {err_report:.3e})")
+    assert err <= 1e-6 and rel_l2(y32[sub], ref) <= 1e-6 and rel_l2(y64[sub], ref) <= 1e-12
+    assert err_report <= 1e-6
+
+
+@pytest.mark.parametrize("splits", [1, 3, 7])
+def test_lattice_tail_of_arbitrary_points(pkg, ctx, oracle, workloads, splits, tiles):
+    """A uniform lattice followed by fewer than max(nx, ny) other sources at
+    any positions (here: off the plane, off the lattice, and one 2 mm from some
+    targets), summed exactly by every target in the first source split; with
+    source splits 1, 3 and 7 and targets near and far enough that some waves
+    of the 16 x 32 kernel fall back to 16 x 16 tiles."""
+    lat = workloads.grid(64, 96, 10e-6, 0.0)
+    g = np.random.default_rng(splits)
+    tail = np.zeros((40, 3))
+    tail[:, :2] = (g.random((40, 2)) - 0.5) * 2e-3
+    tail[:, 2] = (g.random(40) - 0.5) * 2e-4
+    tail[0] = [1e-3, -1e-3, 0.248]  # 2 mm below the target plane
+    u = np.concatenate([lat, tail])
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    v = np.concatenate([targets(7, 400, 0.25, half=2e-3), targets(8, 400, 0.25, half=6e-3)])
+    plan = ctx.plan(u)
+    assert plan.lattice() == (64, 96)
+    pkg.set_option("splits", splits)
+    y = plan.transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    ref = oracle.transform_full(-1, LAM, amp, phase, u, v)
+    assert rel_l2(y, ref) <= 1e-6
+    # the tail is summed: without it the field differs
+    amp2 = amp.copy()
+    amp2[len(lat):] = 0.0
+    y2 = plan.transform(amp2, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y2, y) > 1e-3

@@ -121,3 +121,62 @@ def test_mc_lattice_matches_oracle(pkg, ctx, oracle, workloads):
     sub = np.arange(0, 2000, 9)
     ref = oracle.transform_mc(+1, LAM, amp, phase, u, v, 513, 11, targets=sub)
     assert rel_l2(y[sub], ref) <= 1e-6
+
+
+# --------------------------------------------------------------------------
+# per-target Monte Carlo in passes over source ranges (option mc_passes)
+
+
+@pytest.mark.parametrize("passes", [2, 3, 7])
+@pytest.mark.parametrize("case", ["lattice", "padded", "cloud"])
+def test_mc_passes_match_oracle(pkg, ctx, oracle, workloads, passes, case):
+    """Each pass takes the draws of one source range; together they are the
+    estimator with the same indices: FP32-split within 1e-6 and FP64 within
+    1e-12 of the oracle, and the table gather bit-identical to the record
+    gather.  M = 700 leaves a partial wave; K = 1001 is odd and long enough to
+    force partial batches of the per-lane queues."""
+    if case == "lattice":
+        u = workloads.grid(96, 80, 10e-6, 0.0)
+    elif case == "padded":
+        u = workloads.reference_plane(7000, 1.0e-3, 0.0, 16 / 9)
+    else:
+        g = np.random.default_rng(31)
+        u = np.zeros((6000, 3))
+        u[:, :2] = (g.random((6000, 2)) - 0.5) * 2e-3
+    amp, phase = weights(passes, len(u))
+    v = targets(32, 700)
+    pkg.set_option("mc_passes", passes)
+    K = 1001
+    y1, y0 = both(pkg, ctx, amp, phase, u, v, K, -1)
+    assert np.array_equal(y1, y0)
+    sub = np.arange(0, 700, 7)
+    ref = oracle.transform_mc(-1, LAM, amp, phase, u, v, K, 7, targets=sub)
+    assert rel_l2(y1[sub], ref) <= 1e-6
+    y64 = ctx.transform_mc(amp, phase, u, v, K, 7, -1, LAM, pkg.FP64)
+    assert rel_l2(y64[sub], ref) <= 1e-12
+
+
+@pytest.mark.parametrize("K", [1, 7, 8, 9, 64])
+def test_mc_passes_ragged_k(pkg, ctx, oracle, workloads, K):
+    """Few draws per target: the queues drain with partial batches only."""
+    u = workloads.grid(64, 48, 10e-6, 0.0)
+    amp, phase = weights(K, len(u))
+    v = targets(33, 333)
+    pkg.set_option("mc_passes", 4)
+    y = ctx.transform_mc(amp, phase, u, v, K, 5, +1, LAM, pkg.FP32_SPLIT, obliquity=True)
+    ref = oracle.transform_mc(+1, LAM, amp, phase, u, v, K, 5, obliquity=True)
+    assert rel_l2(y, ref) <= 1e-6
+
+
+def test_mc_passes_independent_of_count(pkg, ctx, workloads):
+    """The estimate does not depend on the number of passes beyond FP32
+    summation order (1 pass is the direct kernel)."""
+    u = workloads.grid(128, 96, 10e-6, 0.0)
+    amp, phase = weights(8, len(u))
+    v = targets(34, 1500)
+    ys = []
+    for passes in (1, 2, 5, 16):
+        pkg.set_option("mc_passes", passes)
+        ys.append(ctx.transform_mc(amp, phase, u, v, 2048, 3, +1, LAM, pkg.FP32_SPLIT))
+    for y in ys[1:]:
+        assert rel_l2(y, ys[0]) <= 5e-7
