@@ -145,7 +145,9 @@ def main():
     ap.add_argument("--repeats", type=int, default=5,
                     help="timed repeats (after one warm-up) of each side-section measurement")
     ap.add_argument("--sections", default="chain,fp32,mc,adjoint",
-                    help="comma list of chain, fp32, mc, adjoint (profiling runs pick one)")
+                    help="comma list of chain, fp32, mc, adjoint (profiling runs pick one), "
+                         "and lattices (not run by default: the FP32-split lattice kernel's "
+                         "tile shapes on narrow and square lattices)")
     ap.add_argument("--hologram-steps", type=int, default=20,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
     ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32), default=32,
@@ -304,6 +306,9 @@ def main():
                                   "K": mc}
             log(f"[bench] monte carlo: {mc}")
 
+    if "lattices" in sections:
+        out["lattices"] = run_lattices(holo, ctx, wl, T, max(5, a.repeats))
+
     if "adjoint" in sections:
         hol = importlib.import_module("holo_amd.hologram")
         n = max(20, a.hologram_steps)
@@ -320,6 +325,35 @@ def main():
 
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def run_lattices(holo, ctx, wl, T, R):
+    """FP32-split on uniform lattices of 8, 24, 40 and 1024 rows (10 um pitch,
+    about 65k or 1M sources) -> 1024^2 targets at 0.35 m, with 16 x 32, 16 x 16
+    and 8 x 8 tiles: narrow lattices pad their rows to 16 (8 rows: half the
+    slots are padding)."""
+    import torch
+    res = {}
+    v = T(wl.grid(1024, 1024, 7.56e-6, 0.35))
+    for nx, ny in ((8, 8192), (24, 2720), (40, 1632), (1024, 1024)):
+        u = wl.grid(nx, ny, 10e-6, 0.0)
+        g = np.random.default_rng(0)
+        amp, ph, uu = T(g.random(len(u))), T(g.random(len(u)) * 2 * np.pi), T(u)
+        plan = holo.Plan(ctx, uu)
+        y = torch.empty((v.shape[0], 2), dtype=torch.float64, device=v.device)
+        row = {}
+        for name, g16, g32 in (("16x32", 1, 1), ("16x16", 1, 0), ("8x8", 0, 1)):
+            holo.set_option("grid16", g16)
+            holo.set_option("grid32", g32)
+            t = time_events(lambda: plan.transform(amp, ph, v, +1, 632.8e-9, holo.FP32_SPLIT,
+                                                   out=y), R)
+            row[name] = {"pairs_per_s": len(u) * v.shape[0] / (t["median_ms"] / 1e3), "ms": t}
+        holo.set_option("grid16", 1)
+        holo.set_option("grid32", 1)
+        res[f"{nx}x{ny}"] = row
+        log(f"[bench] lattice {nx}x{ny}: " + ", ".join(
+            f"{k} {r['pairs_per_s']:.3g}" for k, r in row.items()))
+    return res
 
 
 def run_chain(a, holo, ctx, s1, s2, T, dev, cpu, name):

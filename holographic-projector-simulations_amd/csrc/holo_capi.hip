@@ -261,7 +261,7 @@ struct ErrScratch {
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 holo_status err_scratch(holo_ctx* ctx, ErrScratch* e) {
-  const size_t n = holo::kErrTargets, np = 64 * n;
+  const size_t n = holo::kErrTargets, np = holo::kFewTargetSplits * n;
   const size_t o_tmap = 256, o_vsub = o_tmap + align256(n * sizeof(uint32_t)),
                o_ref = o_vsub + align256(3 * n * sizeof(double)),
                o_part = o_ref + align256(n * sizeof(double2)),
@@ -996,7 +996,7 @@ holo_status plan_fp64_at(holo_ctx* ctx, const holo_plan* p, int sign, bool obliq
   const double K = (2 * M_PI) / lambda;  // TWO_PI_OVER_LAMBDA, cuda/macros.h:29-31
   HOLO_HIP(holo::launch_superpose_f64(sign, wf, obliquity, p->uniform_z, p->row, w0, w1,
                                       static_cast<const double*>(p->u.ptr), p->N, e.vsub, count,
-                                      K, e.ref, e.partial, e.partial_cap, ctx->stream));
+                                      K, e.ref, e.partial, e.partial_cap, ctx->stream, true));
   return HOLO_OK;
 }
 
@@ -1293,7 +1293,8 @@ holo_status holo_transform_mc(holo_ctx* ctx, holo_precision precision, int sign,
                                                         ctx->stream));
                           HOLO_HIP(holo::launch_mc_subset(sign, shared, obl, p64, ph64, N,
                                                           e.vsub, count, e.tmap, K, seed, Kw,
-                                                          lambda, e.ref, ctx->stream));
+                                                          lambda, e.ref, e.partial,
+                                                          e.partial_cap, ctx->stream));
                           return HOLO_OK;
                         }));
   return finish_out(ctx, y, dy, 2 * M, dev);

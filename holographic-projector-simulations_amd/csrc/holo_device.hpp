@@ -123,11 +123,13 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 
 constexpr uint32_t kMcStream = 0x4d43u;  // third counter word of MC draws
 
-// Lemire multiply-shift of a 64-bit uniform word into [0, N).
-__device__ __forceinline__ uint32_t lemire_index(uint32_t lo, uint32_t hi,
-                                                 uint64_t N) {
-  const uint64_t r = ((uint64_t)hi << 32) | lo;
-  return (uint32_t)__umul64hi(r, N);
+// Lemire multiply-shift of the 64-bit uniform word r = hi 2^32 + lo into
+// [0, N): floor(r N / 2^64).  With N < 2^32 that is
+// floor((hi N + floor(lo N / 2^32)) / 2^32) exactly (the dropped fraction is
+// below one and the sum below 2^64), two 32 x 32 -> 64-bit products.
+__device__ __forceinline__ uint32_t lemire_index(uint32_t lo, uint32_t hi, uint32_t N) {
+  const uint64_t t = (uint64_t)hi * N + (((uint64_t)lo * N) >> 32);
+  return (uint32_t)(t >> 32);
 }
 
 // Fractional cycle in [-0.5, 0.5]: x - rint(x), exact in FP64 for |x| < 2^52.

@@ -96,8 +96,14 @@ inline void with_r(int R, F&& f) {
 
 
 // Number of source splits S chosen for a launch over M targets with R
-// targets per lane and `units` source units (sources or tiles).
-int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
+// targets per lane and `units` source units (sources or tiles), at most
+// max_splits.
+int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split,
+                  size_t max_splits = 64);
+
+// Source splits of a launch over a few hundred targets (the error report's
+// subset): enough workgroups to fill the 256 CUs.
+constexpr size_t kFewTargetSplits = 1024;
 
 // y (interleaved complex, M) = sum_n pair(n, m).  Polar weights: w0 = amp,
 // w1 = phase.  Cartesian: w0 = interleaved complex weights, w1 unused.
@@ -106,13 +112,15 @@ int choose_splits(size_t M, int R, size_t units, size_t min_units_per_split);
 // fewer per pair).  `partial` holds partial_cap double2; S source splits need
 // S*M of them.
 // row > 0: the sources form rows of `row` consecutive sources with one x
-// (used with uniform_z).
+// (used with uniform_z).  few_targets: one target per lane and up to
+// kFewTargetSplits source splits, for M of a few hundred.
 hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool uniform_z,
                                 size_t row, const double* w0, const double* w1, const double* u,
                                 size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
-                                size_t partial_cap, hipStream_t stream);
+                                size_t partial_cap, hipStream_t stream,
+                                bool few_targets = false);
 
 // The same sum for sources whose layout the host has not scanned (device
 // pointers): launch_source_layout writes it to `layout` (4 words of device
@@ -219,7 +227,8 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
 hipError_t launch_mc_subset(int sign, bool shared, bool obliquity, const double4* pos,
                             const double* ph, size_t N, const double* vsub, size_t count,
                             const uint32_t* tmap, uint32_t K, uint64_t seed, double K_wave,
-                            double lambda, double2* y, hipStream_t stream);
+                            double lambda, double2* y, double2* partial, size_t partial_cap,
+                            hipStream_t stream);
 
 // wc may be NULL; otherwise it also receives the FP32 weights (FP32-split).
 hipError_t launch_mc_pack(bool fp64, const double* amp, const double* phase,

@@ -26,14 +26,14 @@ Tuning& tuning() {
   return t;
 }
 
-int choose_splits(size_t M, int R, size_t units, size_t min_units) {
+int choose_splits(size_t M, int R, size_t units, size_t min_units, size_t max_splits) {
   const Tuning& tu = tuning();
   if (tu.splits > 0) return (int)std::min<size_t>(tu.splits, std::max<size_t>(units, 1));
   const size_t waves = (M + 64 * R - 1) / (64 * R);
   size_t S = (tu.target_waves + waves - 1) / std::max<size_t>(waves, 1);
   const size_t max_by_units = std::max<size_t>(units / std::max<size_t>(min_units, 1), 1);
   S = std::min(S, max_by_units);
-  S = std::min<size_t>(S, 64);
+  S = std::min<size_t>(S, max_splits);
   return (int)std::max<size_t>(S, 1);
 }
 
@@ -264,11 +264,11 @@ hipError_t launch_superpose_f64(int sign, WeightForm wf, bool obliquity, bool un
                                 size_t N,
                                 const double* v, size_t M, double K,
                                 double2* y, double2* partial,
-                                size_t partial_cap, hipStream_t stream) {
+                                size_t partial_cap, hipStream_t stream, bool few_targets) {
   if (M == 0) return hipSuccess;
-  int R = tuning().r_f64;
+  int R = few_targets ? 1 : tuning().r_f64;
   if (R != 1 && R != 2 && R != 4) R = 2;
-  int S = N ? choose_splits(M, R, N, 1024) : 1;
+  int S = N ? choose_splits(M, R, N, 1024, few_targets ? kFewTargetSplits : 64) : 1;
   while (S > 1 && (size_t)S * M > partial_cap) --S;
   const uint32_t chunk = N ? (uint32_t)((N + S - 1) / S) : 0;
   const dim3 grid((unsigned)((M + (size_t)kBlock * R - 1) / ((size_t)kBlock * R)),
@@ -617,15 +617,18 @@ struct McSourceView {
 };
 
 // Per-target estimator; thread i evaluates the target at v[i], whose index
-// (the Philox counter) is tmap[i], or i without a map.
+// (the Philox counter) is tmap[i], or i without a map.  Block row blockIdx.y
+// takes the draws k in [blockIdx.y kchunk, +kchunk) (kchunk even) of the K and
+// writes its share to y[blockIdx.y M + i].
 template <int SIGN, bool FP64, bool OBL, bool SHARED_IDX>
 __global__ __launch_bounds__(kBlock) void k_mc(
     const double4* __restrict__ pos, const double* __restrict__ ph,
     uint32_t N, const double* __restrict__ v, uint32_t M, const uint32_t* __restrict__ tmap,
-    uint32_t K, uint32_t seed_lo, uint32_t seed_hi, double Kw, double inv_lambda,
-    double scale, McLattice lat, double2* __restrict__ y) {
+    uint32_t K, uint32_t kchunk, uint32_t seed_lo, uint32_t seed_hi, double Kw,
+    double inv_lambda, double scale, McLattice lat, double2* __restrict__ y) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= M) return;
+  const uint32_t kb = blockIdx.y * kchunk, ke = min(K, kb + kchunk);
   const uint32_t m = tmap ? tmap[i] : i;
   const double vx = v[3 * (size_t)i], vy = v[3 * (size_t)i + 1],
                vz = v[3 * (size_t)i + 2];
@@ -633,8 +636,8 @@ __global__ __launch_bounds__(kBlock) void k_mc(
   double ar = 0.0, ai = 0.0;
   float fr = 0.f, fi = 0.f;
   // 8 samples per step: 4 Philox blocks, then 8 independent gathers in flight
-  const uint32_t K8 = K & ~7u;
-  for (uint32_t k = 0; k < K8; k += 8) {
+  const uint32_t K8 = kb + ((ke - kb) & ~7u);
+  for (uint32_t k = kb; k < K8; k += 8) {
     uint32_t idx[8];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
@@ -666,12 +669,12 @@ __global__ __launch_bounds__(kBlock) void k_mc(
       fi = 0.f;
     }
   }
-  for (uint32_t k = K8; k < K; k += 2) {
+  for (uint32_t k = K8; k < ke; k += 2) {
     const U4 o = mc_draws<SHARED_IDX>(k, m, seed_lo, seed_hi);
     const uint32_t idx[2] = {lemire_index(o.x, o.y, N), lemire_index(o.z, o.w, N)};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      if (h == 1 && k + 1 >= K) break;
+      if (h == 1 && k + 1 >= ke) break;
       const double4 p = source(idx[h]);
       if constexpr (FP64) {
         const PairF64 q = pair_f64<SIGN, 0, OBL>(p.w, ph[idx[h]], p.x, p.y, p.z, vx, vy, vz, Kw);
@@ -683,7 +686,7 @@ __global__ __launch_bounds__(kBlock) void k_mc(
   }
   ar += fr;
   ai += fi;
-  y[i] = make_double2(ar * scale, ai * scale);
+  y[(size_t)blockIdx.y * M + i] = make_double2(ar * scale, ai * scale);
 }
 
 // Per-target Monte Carlo over source ranges.  A per-target draw gathers one
@@ -699,18 +702,6 @@ __global__ __launch_bounds__(kBlock) void k_mc(
 // adds its sums into y[m] (FP64): the first writes, the last scales by N / K.
 // The summation order of a target is fixed by its indices and the ranges.
 constexpr uint32_t kMcQueue = 32;  // ring slots per lane, a power of two
-
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = min(x, (uint32_t)__shfl_xor((int)x, off, 64));
-  return x;
-}
-
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, off, 64));
-  return x;
-}
 
 template <int SIGN, bool FP64, bool OBL>
 __global__ __launch_bounds__(kBlock) void k_mc_pass(
@@ -783,11 +774,12 @@ __global__ __launch_bounds__(kBlock) void k_mc_pass(
         ++tail;
       }
     // full batches while every active lane holds 8 (idle lanes count as full)
-    while (wave_min_u32(active ? tail - head : 0xffffffffu) >= 8) batch(8);
+    while (__builtin_amdgcn_ballot_w64(active && tail - head < 8) == 0) batch(8);
     // every ring holds at most kMcQueue - 8 entries before the next round
-    if (wave_max_u32(active ? tail - head : 0u) > kMcQueue - 8) batch(min(8u, tail - head));
+    if (__builtin_amdgcn_ballot_w64(active && tail - head > kMcQueue - 8))
+      batch(min(8u, tail - head));
   }
-  while (wave_max_u32(active ? tail - head : 0u) > 0) batch(min(8u, tail - head));
+  while (__builtin_amdgcn_ballot_w64(active && tail != head)) batch(min(8u, tail - head));
   if (active) {
     if (!first) {
       const double2 old = y[m];
@@ -890,7 +882,7 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
         else if (passes <= 1)
           k_mc<sg.value, (bool)f64.value, (bool)obl.value, false>
               <<<blocks, kBlock, 0, stream>>>(pos, ph, (uint32_t)N, v, (uint32_t)M, nullptr, K,
-                                              slo, shi, K_wave, il, scale, lat, y);
+                                              K + (K & 1u), slo, shi, K_wave, il, scale, lat, y);
         else
           for (uint32_t b = 0; b < passes; ++b)
             k_mc_pass<sg.value, (bool)f64.value, (bool)obl.value>
@@ -908,21 +900,35 @@ hipError_t launch_mc(int sign, bool fp64, bool shared, bool obliquity, const dou
 hipError_t launch_mc_subset(int sign, bool shared, bool obliquity, const double4* pos,
                             const double* ph, size_t N, const double* vsub, size_t count,
                             const uint32_t* tmap, uint32_t K, uint64_t seed, double K_wave,
-                            double lambda, double2* y, hipStream_t stream) {
+                            double lambda, double2* y, double2* partial, size_t partial_cap,
+                            hipStream_t stream) {
   if (!count) return hipSuccess;
   const double scale = (double)N / (double)K;
   const uint32_t slo = (uint32_t)seed, shi = (uint32_t)(seed >> 32);
   const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
+  // a few hundred targets: the draws are split into chunks of at least 64
+  // (a multiple of 8) over enough blocks to fill the GPU, summed in order
+  uint32_t S = (uint32_t)std::min<size_t>({(K + 63) / 64, kFewTargetSplits,
+                                          partial_cap / count});
+  S = std::max(S, 1u);
+  uint32_t chunk = (K + S - 1) / S;
+  chunk = (chunk + 7u) & ~7u;
+  S = (K + chunk - 1) / chunk;
+  double2* out = S > 1 ? partial : y;
   with_sign(sign, [&](auto sg) {
     with_bool(shared, [&](auto sh) {
       with_bool(obliquity, [&](auto obl) {
-        k_mc<sg.value, true, (bool)obl.value, (bool)sh.value><<<blocks, kBlock, 0, stream>>>(
-            pos, ph, (uint32_t)N, vsub, (uint32_t)count, tmap, K, slo, shi, K_wave,
-            1.0 / lambda, scale, McLattice{}, y);
+        k_mc<sg.value, true, (bool)obl.value, (bool)sh.value>
+            <<<dim3(blocks, S), kBlock, 0, stream>>>(pos, ph, (uint32_t)N, vsub,
+                                                     (uint32_t)count, tmap, K, chunk, slo, shi,
+                                                     K_wave, 1.0 / lambda, scale, McLattice{},
+                                                     out);
       });
     });
   });
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_reduce_partials((int)S, count, partial, y, stream);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mc_pack(
@@ -1023,8 +1029,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_indices(
   const uint32_t m = m0 + j;
   for (uint32_t k = 0; k < K; k += 2) {
     const U4 o = philox4x32_10(U4{k >> 1, m, tag, kMcStream}, seed_lo, seed_hi);
-    idx[(size_t)j * K + k] = lemire_index(o.x, o.y, N);
-    if (k + 1 < K) idx[(size_t)j * K + k + 1] = lemire_index(o.z, o.w, N);
+    idx[(size_t)j * K + k] = lemire_index(o.x, o.y, (uint32_t)N);
+    if (k + 1 < K) idx[(size_t)j * K + k + 1] = lemire_index(o.z, o.w, (uint32_t)N);
   }
 }
 
