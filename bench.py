@@ -256,8 +256,9 @@ def main():
             t32 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam,
                                                       holo.FP32_SPLIT, out=y32), R)
             # the same call with its error report (FP64 at 256 stride targets)
-            plan3.transform(amp3, ph3, v3, p3.sign, p3.lam, holo.FP32_SPLIT, out=y32,
-                            report_error=True)
+            t32r = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam,
+                                                       holo.FP32_SPLIT, out=y32,
+                                                       report_error=True), R)
             err_lib = float(ctx.last_rel_l2.item())
             f32 = torch.view_as_complex(y32).cpu().numpy()
             pairs = p3.pairs
@@ -271,6 +272,7 @@ def main():
                 "fp64_ms": t64,
                 "fp32_split_pairs_per_s": pairs / (t32["median_ms"] / 1e3),
                 "fp32_split_ms": t32,
+                "fp32_split_with_error_report_ms": t32r,
                 "speedup": t64["median_ms"] / t32["median_ms"],
                 "rel_l2_vs_fp64": rel_l2(f32, f64),
                 "max_abs_err_rel_to_max": float(np.max(np.abs(f32 - f64)) / np.max(np.abs(f64))),

@@ -250,9 +250,42 @@ __global__ __launch_bounds__(kBlock) void k_reduce_partials(
   y[m] = make_double2(re, im);
 }
 
+// Few targets over many splits (the error report's 256 targets over up to
+// 1024 splits): one workgroup per target, lane t summing the splits t, t + 256,
+// ... in order, then a fixed tree over the lanes, so the order is set by S
+// alone.
+__global__ __launch_bounds__(kBlock) void k_reduce_partials_wide(
+    const double2* __restrict__ partial, uint32_t S, uint32_t M,
+    double2* __restrict__ y) {
+  __shared__ double s_re[kBlock], s_im[kBlock];
+  const uint32_t m = blockIdx.x, t = threadIdx.x;
+  double re = 0.0, im = 0.0;
+  for (uint32_t s = t; s < S; s += kBlock) {
+    const double2 p = partial[(size_t)s * M + m];
+    re += p.x;
+    im += p.y;
+  }
+  s_re[t] = re;
+  s_im[t] = im;
+  __syncthreads();
+  for (uint32_t h = kBlock / 2; h > 0; h >>= 1) {
+    if (t < h) {
+      s_re[t] += s_re[t + h];
+      s_im[t] += s_im[t + h];
+    }
+    __syncthreads();
+  }
+  if (t == 0) y[m] = make_double2(s_re[0], s_im[0]);
+}
+
 hipError_t launch_reduce_partials(int S, size_t M, const double2* partial, double2* y,
                                   hipStream_t stream) {
-  if (S <= 1) return hipSuccess;
+  if (S <= 1 || M == 0) return hipSuccess;
+  if (M <= 4096 && S >= 64) {
+    k_reduce_partials_wide<<<(unsigned)M, kBlock, 0, stream>>>(partial, (uint32_t)S,
+                                                               (uint32_t)M, y);
+    return hipGetLastError();
+  }
   const unsigned blocks = (unsigned)((M + kBlock - 1) / kBlock);
   k_reduce_partials<<<blocks, kBlock, 0, stream>>>(partial, (uint32_t)S,
                                                    (uint32_t)M, y);

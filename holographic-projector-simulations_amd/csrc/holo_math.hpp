@@ -6,12 +6,18 @@
 // and the angle-addition formulas (the table-driven scheme of P. T. P. Tang,
 // "Table-driven implementation of the exponential function", 1989, applied
 // to sin/cos):
-//   k = rint(x 1024/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r);  j = k mod 2048
+//   k = RN_int(x 1024/pi);  r = fma(-k, P1, x);  r = fma(-k, P2, r);  j = k mod 2048
 //   sin x = S_j cos r + C_j sin r,   cos x = C_j cos r - S_j sin r
 // P1 = RN(pi/1024), P2 = RN(pi/1024 - P1); the neglected third term is below
 // 2e-35, so for |x| < 2^40 the reduced argument is within ~2e-19 of
-// x - k pi/1024.  |r| <= pi/2048, where sin r = r - r^3/6 and cos r = 1 -
-// r^2/2 + r^4/24 are within 7.1e-17 and 1.8e-20.  The table
+// x - k pi/1024.  |r| <= pi/2048, where sin r = r - r^3/6 is within 7.1e-17
+// and cos r = 1 + C2 r^2 within 3.96e-14: C2 = -1/2 + (sqrt 2 - 1) R^2 / 12,
+// R = pi/2048, is the minimax choice for that one term (the error
+// r^4/24 - (C2 + 1/2) r^2 takes +-3.96e-14 at r = R and r^2 = 6 (C2 + 1/2);
+// the Taylor term -1/2 alone would leave up to 2.3e-13, and r^4/24 one more
+// FMA per pair).  The error only scales the phasor (S_j, C_j) cos r by 1 +
+// O(4e-14): 10^-13-level against the 1e-12 relative L2 the FP64 path is held
+// to, at 2.5 % of its run time.  The table
 // (holo_sincos_table.inc, 32 KB, written by gen_sincos_table.py) is correctly
 // rounded, so S_{2048-j} = -S_j and C_{2048-j} = C_j exactly and sin stays
 // odd and cos even bit for bit.  Only multiplies, FMAs, rint and integer ops are
@@ -35,18 +41,22 @@ HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
   constexpr double kStepsOverPi = 0x1.45f306dc9c883p+8;  // 1024 / pi
   constexpr double kP1 = 0x1.921fb54442d18p-9;
   constexpr double kP2 = 0x1.1a62633145c07p-63;
-  // k + 1.5 * 2^48 has ulp 1/16, so its low word is 16 k mod 2^32: the byte
-  // offset of entry k mod 2048 is one AND away (|k| < 2^47)
-  constexpr double kMagic = 0x1.8p48;
-  const double k = __builtin_rint(x * kStepsOverPi);
+  // kk = x 1024/pi + 1.5 * 2^52 in one rounding: its ulp is 1, so kk - 1.5 * 2^52
+  // is x 1024/pi rounded to the nearest integer k (ties to even, so k(-x) =
+  // -k(x)), and the low word of kk is k mod 2^32 (|k| < 2^51).  Two FP64
+  // operations and two integer ones, where rint, a multiply and an add would
+  // be three FP64 ones.
+  constexpr double kShift = 0x1.8p52;
+  const double kk = __builtin_fma(x, kStepsOverPi, kShift);
+  const double k = kk - kShift;
   double r = __builtin_fma(-k, kP1, x);
   r = __builtin_fma(-k, kP2, r);
-  const unsigned off = (unsigned)__builtin_bit_cast(unsigned long long, k + kMagic) & 0x7ff0u;
+  const unsigned off = ((unsigned)__builtin_bit_cast(unsigned long long, kk) << 4) & 0x7ff0u;
   const double* e = reinterpret_cast<const double*>(reinterpret_cast<const char*>(tab) + off);
   const double S = e[0], C = e[1];
   const double u = r * r;
   const double sr = __builtin_fma(r * u, -0x1.5555555555555p-3, r);
-  const double cr = __builtin_fma(u, __builtin_fma(u, 0x1.5555555555555p-5, -0.5), 1.0);
+  const double cr = __builtin_fma(u, -0x1.fffffa8c96463p-2, 1.0);  // 1 + C2 r^2
   s = __builtin_fma(S, cr, C * sr);
   c = __builtin_fma(C, cr, -(S * sr));
 }

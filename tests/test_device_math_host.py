@@ -1,13 +1,19 @@
 """The FP64 kernels' own sin/cos (Cody-Waite reduction modulo pi/1024, a
-correctly rounded table of (sin, cos)(j pi/1024) and short Taylor polynomials,
-holo_math.hpp) through its host build: accuracy against mpmath and glibc over
-the superposition's argument range and the reduction's edges, and exact
-odd/even symmetry.  tests/test_gpu_parity.py::test_fp64_device_sincos_bits
-checks that the device produces the same bits."""
+correctly rounded table of (sin, cos)(j pi/1024), sin r = r - r^3/6 and the
+minimax cos r = 1 + C2 r^2 on |r| <= pi/2048, holo_math.hpp) through its host
+build: accuracy against mpmath and glibc over the superposition's argument
+range and the reduction's edges, and exact odd/even symmetry.  The bound is
+the cos term's 3.96e-14 plus rounding; the Taylor coefficient -1/2 in its place
+(2.3e-13) or a wrong table step fails it, and the size of the minimax error is
+pinned from both sides next to a table step.
+tests/test_gpu_parity.py::test_fp64_device_sincos_bits checks that the device
+produces the same bits."""
 import math
 
 import mpmath as mp
 import numpy as np
+
+TOL = 4.2e-14  # 3.96e-14 (the cos term) + rounding
 
 
 def test_sincos_accuracy_random(pkg):
@@ -18,7 +24,7 @@ def test_sincos_accuracy_random(pkg):
     for x in xs:
         s, c = pkg.host_sincos(x)
         worst = max(worst, abs(s - math.sin(x)), abs(c - math.cos(x)))
-    assert worst <= 2.5e-16  # measured: 2.2e-16 (one ulp of values near 1)
+    assert worst <= TOL
 
 
 def test_sincos_reduction_edges_against_mpmath(pkg):
@@ -31,8 +37,24 @@ def test_sincos_reduction_edges_against_mpmath(pkg):
             for ulps in (-3, -1, 0, 1, 3):
                 x = base + ulps * math.ulp(base)
                 s, c = pkg.host_sincos(x)
-                assert abs(s - float(mp.sin(mp.mpf(x)))) <= 2.5e-16
-                assert abs(c - float(mp.cos(mp.mpf(x)))) <= 2.5e-16
+                assert abs(s - float(mp.sin(mp.mpf(x)))) <= TOL
+                assert abs(c - float(mp.cos(mp.mpf(x)))) <= TOL
+
+
+def test_sincos_minimax_error_size(pkg):
+    """At r = pi/2048 past a table step the cos term's error is close to its
+    bound (3.96e-14; the Taylor -1/2 gives 2.3e-13 there, the r^4 term ~1e-16),
+    and it vanishes at the step itself."""
+    mp.mp.dps = 40
+    for j in (1, 77, 512, 1500):
+        x = float((j + mp.mpf(0.5)) * mp.pi / 1024)  # r = +-pi/2048
+        s, c = pkg.host_sincos(x)
+        e = math.hypot(s - float(mp.sin(mp.mpf(x))), c - float(mp.cos(mp.mpf(x))))
+        assert 3.0e-14 <= e <= TOL
+        x = float(j * mp.pi / 1024)  # r ~ 0
+        s, c = pkg.host_sincos(x)
+        assert abs(s - float(mp.sin(mp.mpf(x)))) <= 3e-16
+        assert abs(c - float(mp.cos(mp.mpf(x)))) <= 3e-16
 
 
 def test_sincos_odd_even_bitwise(pkg):

@@ -190,10 +190,11 @@ def test_fp64_golden_fixture(pkg, ctx):
 
 
 @pytest.mark.parametrize("R", [1, 2, 4])
-@pytest.mark.parametrize("splits", [0, 1, 3, 7])
+@pytest.mark.parametrize("splits", [0, 1, 3, 7, 64])
 def test_fp64_launch_shape_independent(pkg, ctx, oracle, R, splits):
     """Targets per lane, source splits and ragged sizes do not change results
-    beyond summation order (the reference's Geometry hints, cuda/init.h:42-60)."""
+    beyond summation order (the reference's Geometry hints, cuda/init.h:42-60).
+    64 splits over 777 targets take the one-workgroup-per-target reduction."""
     pkg.set_option("r_f64", R)
     pkg.set_option("splits", splits)
     amp, phase, u, v = rand_geom(3, 2053, 777)
@@ -261,7 +262,11 @@ def test_fp64_inverse_distance_accuracy(pkg, ctx):
     v[:, 0] = x
     y = ctx.transform_full(np.ones(1), np.zeros(1), np.zeros((1, 3)), v, +1, 632.8e-9, pkg.FP64)
     d = np.sqrt(x * x)  # the kernel's d: correctly rounded sqrt of the rounded square
-    amp = np.hypot(y.real, y.imag)  # |(cos, sin)| = 1 to ~2e-16
+    # y = inv (c, s) with (s, c) the kernel's sin/cos of RN(d K) (same bits on
+    # the host); |(c, s)| differs from 1 by up to 4e-14, so it is divided out
+    K = (2 * math.pi) / 632.8e-9
+    cs = np.array([pkg.host_sincos(float(dd * K)) for dd in d])
+    amp = (y.real * cs[:, 1] + y.imag * cs[:, 0]) / (cs[:, 0] ** 2 + cs[:, 1] ** 2)
     err = np.abs(amp * d - 1.0)
     # measured on MI355X: 3.7e-15 at worst, a few 1e-16 typically; the FP64
     # phase itself carries ~5e-10 rad of rounding at these arguments
