@@ -35,7 +35,7 @@ PEAK_FP64 = 78.6e12           # MI355X FP64 vector (and matrix) peak, FLOP/s
 # per pair, profiles/k_superpose_f64_counters.md) on the chain's lattice sources
 # (one plane, rows of one x): the hand-written sqrt and sin/cos cost more than
 # the reference's nominal 29.
-COUNTED_FLOP_PER_PAIR = 45.25
+COUNTED_FLOP_PER_PAIR = 44.25
 
 
 def log(*a):

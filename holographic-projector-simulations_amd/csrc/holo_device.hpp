@@ -147,16 +147,19 @@ __device__ __forceinline__ void sincos_rev_f32(float t, float& s, float& c) {
 // ---------------------------------------------------------------------------
 // FP32-split helpers shared by the tiled and lattice kernels.
 
-// One sample of the FP32-split estimator: d and the fractional cycle in FP64,
-// sin/cos in FP32, FP32 accumulation (flushed to FP64 by the caller).  OBL:
-// amplitude |dz| / d^2 instead of 1 / d.
+// One sample of the FP32-split estimator: d (to a few 1e-15, not always
+// correctly rounded) and the fractional cycle in FP64, sin/cos in FP32, FP32
+// accumulation (flushed to FP64 by the caller).  OBL: amplitude |dz| / d^2
+// instead of 1 / d.
 template <int SIGN, bool OBL = false>
 __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
                                              double inv_lambda, float& fr, float& fi) {
   const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
   double d, id;
-  hm_sqrt_inv(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
-  const float f = (float)frac_cycle(d * inv_lambda);
+  hm_sqrt_inv_fast(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  // the cycle's fraction in [0, 1), exact (v_fract_f64), then rounded to FP32
+  // (within 3e-8 cycles)
+  const float f = (float)__builtin_amdgcn_fract(d * inv_lambda);
   float sn, cs;
   sincos_rev_f32(f, sn, cs);
   if (SIGN < 0) sn = -sn;

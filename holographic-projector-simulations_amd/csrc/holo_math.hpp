@@ -14,10 +14,11 @@
 // and cos r = 1 + C2 r^2 within 3.96e-14: C2 = -1/2 + (sqrt 2 - 1) R^2 / 12,
 // R = pi/2048, is the minimax choice for that one term (the error
 // r^4/24 - (C2 + 1/2) r^2 takes +-3.96e-14 at r = R and r^2 = 6 (C2 + 1/2);
-// the Taylor term -1/2 alone would leave up to 2.3e-13, and r^4/24 one more
-// FMA per pair).  The error only scales the phasor (S_j, C_j) cos r by 1 +
-// O(4e-14): 10^-13-level against the 1e-12 relative L2 the FP64 path is held
-// to, at 2.5 % of its run time.  The table
+// the Taylor term -1/2 alone would leave up to 2.3e-13, and the r^4/24 term
+// costs one more FMA per pair, 3 % of the FP64 kernel's time).  The error e
+// moves (sin x, cos x) by e (S_j, C_j), along the part (S_j, C_j) cos r of the
+// result: an amplitude error below 4e-14 and no phase error, against the
+// 1e-12 relative L2 to which the FP64 path is held.  The table
 // (holo_sincos_table.inc, 32 KB, written by gen_sincos_table.py) is correctly
 // rounded, so S_{2048-j} = -S_j and C_{2048-j} = C_j exactly and sin stays
 // odd and cos even bit for bit.  Only multiplies, FMAs, rint and integer ops are
@@ -43,9 +44,7 @@ HM_HD void hm_sincos(double x, const double* tab, double& s, double& c) {
   constexpr double kP2 = 0x1.1a62633145c07p-63;
   // kk = x 1024/pi + 1.5 * 2^52 in one rounding: its ulp is 1, so kk - 1.5 * 2^52
   // is x 1024/pi rounded to the nearest integer k (ties to even, so k(-x) =
-  // -k(x)), and the low word of kk is k mod 2^32 (|k| < 2^51).  Two FP64
-  // operations and two integer ones, where rint, a multiply and an add would
-  // be three FP64 ones.
+  // -k(x)), and the low word of kk is k mod 2^32 (|k| < 2^51).
   constexpr double kShift = 0x1.8p52;
   const double kk = __builtin_fma(x, kStepsOverPi, kShift);
   const double k = kk - kShift;
@@ -94,6 +93,24 @@ HM_HD void hm_sqrt_inv(double s, double& d, double& inv) {
   double h;
   hm_sqrt_hinv(s, d, h);
   inv = h + h;
+}
+
+// d ~ sqrt(s) and inv ~ 1/d from the Goldschmidt step alone (no Markstein
+// correction): both within a few 1e-15 relative, as inv above, but d is not
+// always correctly rounded.  For FP32-split pairs, which need d / lambda to
+// ~1e-8 cycles (at 1 m and 632.8 nm, 4e-15 of d is 6e-9 cycles), at 5 FP64
+// operations and one v_rsq_f64 instead of 8.
+HM_HD void hm_sqrt_inv_fast(double s, double& d, double& inv) {
+#pragma clang fp contract(off)
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double y = __builtin_amdgcn_rsq(s);
+#else
+  const double y = 1.0 / __builtin_sqrt(s);
+#endif
+  const double g = s * y;
+  const double e = __builtin_fma(-g, 0.5 * y, 0.5);
+  d = __builtin_fma(g, e, g);
+  inv = __builtin_fma(y, e, y);
 }
 
 #undef HM_HD

@@ -477,7 +477,9 @@ def test_plan_info_grid(pkg, ctx, workloads):
 
 
 def test_mc_indices_match_host_replica(pkg, ctx, oracle):
-    for seed, N in ((0, 1 << 20), (123456789012345, 1000), (2 ** 63 + 5, 7)):
+    # N up to 2^31 - 1 exercises the carry of the two-product Lemire map
+    for seed, N in ((0, 1 << 20), (123456789012345, 1000), (2 ** 63 + 5, 7),
+                    (77, 2 ** 31 - 1), (5, 3 * 2 ** 29 + 12345)):
         idx = ctx.mc_indices(seed, N, 1000, 64, 33)
         sidx = ctx.mc_indices(seed, N, 3, 5, 33, shared=True)
         for j in range(0, 64, 7):
@@ -513,6 +515,22 @@ def test_mc_fp32_split_matches_fp64(pkg, ctx, shared):
     amp, phase, u, v = rand_geom(14, 5000, 3000)
     y64 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP64, shared=shared)
     y32 = ctx.transform_mc(amp, phase, u, v, 1024, 7, +1, LAM, pkg.FP32_SPLIT, shared=shared)
+    assert rel_l2(y32, y64) <= 1e-6
+
+
+@pytest.mark.parametrize("z", [0.02, 1.0, 3.0])
+@pytest.mark.parametrize("shared", [False, True])
+def test_mc_fp32_split_distance_range(pkg, ctx, workloads, z, shared):
+    """The FP32-split pair takes d from one Goldschmidt step (no correct
+    rounding; within a few 1e-15): its phase d / lambda stays within ~1e-8
+    cycles of FP64's up to 3 m (4.7e6 cycles), so the same draws agree with
+    the FP64 estimator to 1e-6."""
+    u = workloads.grid(64, 64, 10e-6, 0.0)
+    v = workloads.grid(40, 30, 50e-6, z)
+    g = np.random.default_rng(41)
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    y64 = ctx.transform_mc(amp, phase, u, v, 512, 3, -1, LAM, pkg.FP64, shared=shared)
+    y32 = ctx.transform_mc(amp, phase, u, v, 512, 3, -1, LAM, pkg.FP32_SPLIT, shared=shared)
     assert rel_l2(y32, y64) <= 1e-6
 
 

@@ -254,6 +254,30 @@ def test_mc_index_matches_python_restatement(oracle):
         assert oracle.mc_index(seed, m, k, N) == py_mc_index(seed, m, k, N)
 
 
+def test_lemire_two_product_form_is_exact():
+    """The kernels map r = hi 2^32 + lo into [0, N) as
+    (hi N + (lo N >> 32)) >> 32 with 32 x 32 -> 64-bit products
+    (holo_device.hpp, lemire_index); for N < 2^32 that equals the 128-bit
+    (r N) >> 64 the oracle forms.  Extreme words (all ones, carries out of the
+    low product) and random ones; a form that drops lo N, or adds it without
+    the shift, fails."""
+    g = np.random.default_rng(21)
+    words = [0, 1, 2 ** 32 - 1, 2 ** 31, 2 ** 64 - 1, 2 ** 64 - 2 ** 32, 2 ** 32,
+             0xFFFFFFFF00000001] + [int(x) for x in g.integers(0, 2 ** 63, 3000)] + \
+            [int(x) | (1 << 63) for x in g.integers(0, 2 ** 63, 3000)]
+    Ns = [1, 2, 7, 1000, 2 ** 20, 2 ** 31 - 1, 2 ** 32 - 1, 3 * 2 ** 29 + 12345] + \
+         [int(x) for x in g.integers(1, 2 ** 32, 200)]
+    dropped = 0
+    for i, r in enumerate(words):
+        hi, lo = r >> 32, r & 0xFFFFFFFF
+        for N in Ns[i % 7::7] + Ns[:8]:
+            t = hi * N + ((lo * N) >> 32)
+            assert t < 2 ** 64
+            assert t >> 32 == (r * N) >> 64
+            dropped += (hi * N) >> 32 != (r * N) >> 64
+    assert dropped > 0  # the low product matters on some of these inputs
+
+
 def test_mc_shared_index_matches_python_restatement(oracle):
     """Batched MC: group g's draws use counter (k/2, g, 0x80000000, 0x4d43)."""
     g = np.random.default_rng(12)
