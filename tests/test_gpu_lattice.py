@@ -388,6 +388,40 @@ def test_lattice_near_field_series(pkg, ctx, oracle, workloads, z, sign, obliqui
     pkg.set_option("grid_series", 1)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_lattice_random_near_field(pkg, ctx, seed, tiles):
+    """Seeded near-field geometries (4 - 20 cm, targets up to 1 cm off axis,
+    pitches 4 - 16 um, 400 - 700 nm, either sign, with or without obliquity),
+    where the GEMM bounds fail for most (target, tile) pairs and the series
+    path and the exact path share the work: within 1e-6 of FP64, with the
+    series path on and off."""
+    g = np.random.default_rng(2000 + seed)
+    nx, ny = 16 * int(g.integers(2, 17)), 32 * int(g.integers(1, 7))
+    px, py = g.uniform(4e-6, 16e-6, 2)
+    x = (np.arange(nx) - (nx - 1) / 2.0) * px
+    y = (np.arange(ny) - (ny - 1) / 2.0) * py
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    u = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    z = float(np.exp(g.uniform(np.log(0.04), np.log(0.2))))
+    v = np.zeros((int(g.integers(300, 900)), 3))
+    v[:, :2] = (g.random((len(v), 2)) - 0.5) * 2 * g.uniform(2e-3, 1e-2)
+    v[:, 2] = z
+    lam = g.uniform(400e-9, 700e-9)
+    sign = 1 if g.random() < 0.5 else -1
+    obl = bool(g.random() < 0.4)
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    plan = ctx.plan(u)
+    assert plan.lattice() == (nx, ny)
+    y64 = plan.transform(amp, phase, v, sign, lam, pkg.FP64, obliquity=obl)
+    for series in (1, 0):
+        pkg.set_option("grid_series", series)
+        y32 = plan.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
+        err = rel_l2(y32, y64)
+        print(f"seed {seed}: {nx}x{ny}, z {z:.3f} m, series {series}: rel L2 {err:.3e}")
+        assert err <= 1e-6
+    pkg.set_option("grid_series", 1)
+
+
 def test_lattice_near_field_full_plane(pkg, ctx, oracle, workloads, tiles):
     """The benchmark's 1024^2 lattice with the 1024^2 target plane moved to
     z = 0.1 m: FP32-split within 1e-6 of the FP64 kernel over every target and
