@@ -169,6 +169,9 @@ def main():
     ap.add_argument("--no-series", action="store_true",
                     help="library option grid_series = 0: near-field pairs of the lattice "
                          "kernels take the exact path instead of the q-series")
+    ap.add_argument("--no-near", action="store_true",
+                    help="library option grid_near = 0: waves off the lattice kernels' "
+                         "far-field bounds take the q-series and exact paths per pair")
     ap.add_argument("--target-waves", type=int, default=0,
                     help="library option target_waves (waves a launch aims for when it splits "
                          "sources; 0 = the library default)")
@@ -195,6 +198,8 @@ def main():
         holo.set_option("mc_passes", a.mc_passes)
     if a.no_series:
         holo.set_option("grid_series", 0)
+    if a.no_near:
+        holo.set_option("grid_near", 0)
     import importlib
     wl = importlib.import_module("holo_amd.workloads")
     w = wl.chain_xyz(0)

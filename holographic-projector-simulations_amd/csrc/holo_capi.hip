@@ -120,7 +120,7 @@ struct holo_plan {
   double g16_px = 0, g16_py = 0;
   double g16_cx0 = 0, g16_cx1 = 0, g16_cy0 = 0, g16_cy1 = 0;
   float g16_rho_x = 0, g16_rho_y = 0;
-  DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_ts;
+  DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_wqn, g16_ts;
   // the 16 x 32 tiles formed by pairs of 16 x 16 tiles (16x16 tiles 2T, 2T+1)
   // when the padded count of tile columns is even (g32_tiles = 0 otherwise)
   size_t g32_tiles = 0;
@@ -724,6 +724,7 @@ holo_status upload_grid16(holo_ctx* ctx, holo_plan* p, const HostGrid& grid) {
   HOLO_TRY(ensure(p->g16_perm, (256 * n16 + nt) * sizeof(int32_t), ctx->device));
   HOLO_TRY(ensure(p->g16_wc, (256 * n16 + nt) * sizeof(float2), ctx->device));
   HOLO_TRY(ensure(p->g16_wq, 6 * 64 * n16 * sizeof(uint4), ctx->device));
+  HOLO_TRY(ensure(p->g16_wqn, 4 * 64 * n16 * sizeof(uint4), ctx->device));
   HOLO_TRY(ensure(p->g16_ts, n16 * sizeof(float), ctx->device));
   HOLO_HIP(hipMemcpyAsync(p->g16_centre.ptr, grid.centre16.data(), n16 * sizeof(double4),
                           hipMemcpyHostToDevice, ctx->stream));
@@ -872,6 +873,7 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
                                        static_cast<float2*>(p->g16_wc.ptr), ctx->stream));
     HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
                                          pairs, static_cast<uint4*>(p->g16_wq.ptr),
+                                         static_cast<uint4*>(p->g16_wqn.ptr),
                                          static_cast<float*>(p->g16_ts.ptr), ctx->stream));
     holo::GridTiles g16;  // the 16 x 16 tiles
     g16.perm = static_cast<const int32_t*>(p->g16_perm.ptr);
@@ -906,6 +908,7 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, pairs ? 2 : 1, gt, g16,
                                            static_cast<const float2*>(p->g16_wc.ptr),
                                            static_cast<const uint4*>(p->g16_wq.ptr),
+                                           static_cast<const uint4*>(p->g16_wqn.ptr),
                                            static_cast<const float*>(p->g16_ts.ptr), v, M,
                                            lambda, reinterpret_cast<double2*>(y),
                                            static_cast<double2*>(ctx->partial.ptr), pe,
@@ -1115,6 +1118,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid32 = value != 0;
   } else if (n == "grid_series") {
     t.grid_series = value != 0;
+  } else if (n == "grid_near") {
+    t.grid_near = value != 0;
   } else if (n == "mc_lattice") {
     t.mc_lattice = value != 0;
   } else if (n == "mc_passes") {
@@ -1141,6 +1146,7 @@ long holo_get_option(const char* name) {
   if (n == "grid16") return t.grid16 ? 1 : 0;
   if (n == "grid32") return t.grid32 ? 1 : 0;
   if (n == "grid_series") return t.grid_series ? 1 : 0;
+  if (n == "grid_near") return t.grid_near ? 1 : 0;
   if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;
   if (n == "mc_passes") return t.mc_passes;
   return -1;

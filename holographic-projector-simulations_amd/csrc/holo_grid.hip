@@ -45,6 +45,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "holo_device.hpp"
 #include "holo_internal.hpp"
 
@@ -63,6 +65,7 @@ struct GridBounds {
   double cx0, cx1, cy0, cy1, z;
   float rho_x, rho_y;
   int series = 0;  // 16-row kernels: near pairs take the series path where it holds
+  int near_gemm = 0;  // 16-row kernels: waves off the far-field bounds take the near-field GEMM
 };
 
 // The accuracy bounds of one (target, tile), or of a target against every
@@ -387,11 +390,16 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
 // i = 2a' + c (a = 8 rb + a') of the A operand and k-slots e < 8 of column
 // 16 kc + 4g + e/2, component e%2 -- the same slots as the Y operand of target
 // j = l%16.  Per row block and chunk four MFMAs accumulate W_hi Y_hi,
-// W_hi Y_lo, W_lo Y_hi and the correction (i k_a 2^-13 W / tscale) (Y Qt kb),
-// kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13; the lane receives T of rows
+// W_hi Y_lo, W_lo Y_hi and the correction (i k_a kCrossW W / tscale) (Y Qt kb),
+// kb = s 4 pi c2 (-2 r_x) p_x p_y^2 / kCrossW; the lane receives T of rows
 // 8 rb + 2g, 8 rb + 2g + 1 for target j.
 
 constexpr int kGrid16Flush = 2;  // tiles summed in FP32 between FP64 flushes
+// f16 scales of the correction operands: the weight side carries k_a kCrossW W
+// (and k_a^2 kCross2W W), the column side the inverse, so that both sides stay
+// in f16's normal range from a few cm to about a metre.
+constexpr float kCrossW = 0x1p-4f;
+constexpr float kCross2W = 0x1p-8f;
 constexpr double kRowMid = 28.25;  // midpoint of k_a^2 over a tile's 16 rows
 constexpr float kRowDev = 28.f;    // largest |k_a^2 - kRowMid|
 
@@ -411,6 +419,46 @@ __device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, f
   const float ampsep = kappa * kappa * Pm * Qm;
   return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
          beta * rx2c * Qm <= 1e-6f && ampsep <= 1e-6f;
+}
+
+// Near-field GEMM of the 16 x 16 tiles.  The cross factor is carried to second
+// order, with the q^3 terms and the quadratic amplitude: with u = p_x k_a,
+// P = u^2 - 2 r_x u and
+//   C = 2 c2 P Q + 3 c3 (P^2 Q + P Q^2)   (cycles),   phi = s 2 pi C,
+//   e^{i phi} ~ 1 + i phi - phi^2 / 2 = 1 + i phi1 u + (i phi2 - phi1^2 / 2) u^2 + O(u^3),
+//   phi1 = s 2 pi (-4 c2 r_x Q - 6 c3 r_x Q^2),  phi2 = s 2 pi (2 c2 + 12 c3 r_x^2) Q,
+// the rows X_a = (1 + k1 P + k2 P^2) e^{i s 2 pi (alpha P + c2 P^2 + c3 P^3)} and
+// the columns Y_b likewise in Q.  T_a = sum_b W Y + u sum_b W (i phi1 Y) +
+// u^2 sum_b W (i phi2 - phi1^2 / 2) Y: the u term as the existing correction
+// with both of its operands split into f16 hi and lo, the u^2 term as one more
+// f16 product.  Dropped, and bounded here per (target, tile): the u^3 terms
+// (12 c3 |r_x| Q u^3 in C, phi1 phi2 u^3 in phi^2 / 2), phi^3 / 6, the q^4
+// phase term, the amplitude's cross term (2 k2 - k1^2) P Q and third order,
+// and the f16 rounding of the u^2 operand.  NaN fails.
+__device__ __forceinline__ bool grid16_near_ok(float Pm, float Qm, float arx, float rhox,
+                                               float rx2, float ry2, float ia, float idf,
+                                               bool obl) {
+#pragma clang fp contract(off)
+  const float qmax = Pm + Qm;
+  const float id2 = idf * idf, id4 = id2 * id2;
+  const float c2a = 0.125f * ia * id2, c3 = 0.0625f * ia * id4;  // |c2|, c3 (cycles)
+  const float round_err = 6e-8f * (3.f + 0.5f * ia * (rx2 + ry2));
+  const float q4 = 0.0390625f * qmax * qmax * qmax * qmax * ia * id4 * id2;
+  const float r3 = rhox * rhox * rhox;
+  const float u3 = 12.f * c3 * arx * Qm * r3;
+  const float twopi = 0.5f * kFourPi;
+  const float phi1u = twopi * (4.f * c2a + 6.f * c3 * Qm) * arx * rhox * Qm;
+  const float phi2u2 = twopi * (2.f * c2a + 12.f * c3 * arx * arx) * Qm * rhox * rhox;
+  const float ps = phi1u + phi2u2;
+  const float ph3 = ps * ps * ps * (1.f / 6.f);
+  const float mix = phi1u * phi2u2;
+  const float ampx = (obl ? 1.f : 0.5f) * id4 * Pm * Qm;
+  const float sq = qmax * id2;
+  const float amp3 = (obl ? 1.f : 0.3125f) * sq * sq * sq;
+  const float f16c2 = 1e-3f * (phi2u2 + 0.5f * phi1u * phi1u);
+  return round_err <= kGuardCycles && q4 <= 0.05f * kGuardCycles &&
+         u3 <= 0.05f * kGuardCycles && ph3 <= 1e-7f && mix <= 1e-7f && ampx <= 1e-7f &&
+         amp3 <= 1e-7f && f16c2 <= 1e-7f;
 }
 
 // x rounded to BITS significant bits (the low 24 - BITS bits of the FP32
@@ -516,12 +564,16 @@ __device__ __forceinline__ void series_tile_pairs(const float2* __restrict__ w, 
 // block rb, wq[(6t + 3rb + q) * 64 + lane] holds, for lane l = 16 g + i,
 // row i = 2a' + c (a = 8 rb + a') and columns 4g .. 4g + 3 (re, im) of the real
 // form of W / tscale split into f16 hi (q = 0) and lo (q = 1), and of
-// i k_a 2^-13 W / tscale (q = 2).  wc: 256 slots per tile, slot 16 a + b.
+// i k_a kCrossW W / tscale (q = 2, its f16 hi part).  wqn (nullable), the
+// near-field GEMM's extra operands: wqn[(4t + 2rb + q) * 64 + lane] holds the
+// lo part of i k_a kCrossW W / tscale (q = 0) and k_a^2 kCross2W W / tscale in
+// the real form of W (q = 1).  wc: 256 slots per tile, slot 16 a + b.
 // pair: tiles 2T and 2T + 1 (the two halves of a 16 x 32 tile) share one
 // tscale, the power of two above the larger of their largest components.
 __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restrict__ wc,
                                                            uint32_t n_tiles, uint32_t pair,
                                                            uint4* __restrict__ wq,
+                                                           uint4* __restrict__ wqn,
                                                            float* __restrict__ tscale) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t t = blockIdx.x * kGridWaves + (threadIdx.x >> 6);
@@ -552,8 +604,9 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
   const float is = ldexpf(1.f, -e);
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb) {
-    const float ka = ((float)(8 * rb + ap) - 7.5f) * 0x1p-13f;
-    h8v hi, lo, cr;
+    const float kr = (float)(8 * rb + ap) - 7.5f;
+    const float ka = kr * kCrossW, ka2 = kr * kr * kCross2W;
+    h8v hi, lo, cr, crl, c2;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const float vr = w[rb][q].x * is, vi = w[rb][q].y * is;
@@ -568,22 +621,33 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
       split_f16(A1, h, l);
       hi[2 * q + 1] = h;
       lo[2 * q + 1] = l;
-      cr[2 * q] = (_Float16)C0;
-      cr[2 * q + 1] = (_Float16)C1;
+      split_f16(C0, h, l);
+      cr[2 * q] = h;
+      crl[2 * q] = l;
+      split_f16(C1, h, l);
+      cr[2 * q + 1] = h;
+      crl[2 * q + 1] = l;
+      // k^2 W (real scaling): the same real form as W
+      c2[2 * q] = (_Float16)(ka2 * A0);
+      c2[2 * q + 1] = (_Float16)(ka2 * A1);
     }
     wq[((size_t)t * 6 + 3 * rb + 0) * 64 + lane] = __builtin_bit_cast(uint4, hi);
     wq[((size_t)t * 6 + 3 * rb + 1) * 64 + lane] = __builtin_bit_cast(uint4, lo);
     wq[((size_t)t * 6 + 3 * rb + 2) * 64 + lane] = __builtin_bit_cast(uint4, cr);
+    if (wqn) {
+      wqn[((size_t)t * 4 + 2 * rb + 0) * 64 + lane] = __builtin_bit_cast(uint4, crl);
+      wqn[((size_t)t * 4 + 2 * rb + 1) * 64 + lane] = __builtin_bit_cast(uint4, c2);
+    }
   }
   if (lane == 0) tscale[t] = ldexpf(1.f, e);
 }
 
 hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, uint4* wq,
-                                 float* tscale, hipStream_t stream) {
+                                 uint4* wqn, float* tscale, hipStream_t stream) {
   if (!n_tiles) return hipSuccess;
   if (pair && (n_tiles & 1u)) return hipErrorInvalidValue;
   const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
-  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, pair ? 1u : 0u, wq, tscale);
+  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, pair ? 1u : 0u, wq, wqn, tscale);
   return hipGetLastError();
 }
 
@@ -593,13 +657,16 @@ hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, 
 //   k0 = (sx_hi, sx_lo, sy_hi, sy_lo)   linear phase, cycles per pitch
 //   k1 = (ax, ay, cx, cy)               quadratic phase coefficients
 //   k2 = (qx, qy, kappa p_x^2, kappa p_y^2)
-//   k3 = (f, A, kb, 0)
+//   k3 = (f, A, kb, kb3)
+// and for the near-field GEMM
+//   k4 = (c3 p_x^6, c3 p_y^6, k2 p_x^4, k2 p_y^4)   cubic phase, quadratic amplitude
+//   k5 = (e2, 0, 0, 0)                            the k_a^2 correction's phase term
 // and rr, ri: the FP64 sums of target 16 B + j, rows 8 rb + 2g + c, in slot
 // [B][lane] (lane-contiguous, so free of bank conflicts; in LDS rather than
 // registers: they are touched once per kGrid16Flush tiles, and the registers
 // keep the kernel at 3 or 4 waves).
 struct Grid16Lds {
-  float4 k[4][kGridWaves][64];
+  float4 k[6][kGridWaves][64];
   double rr[kGridWaves][4][64];
   double ri[kGridWaves][4][64];
 };
@@ -623,20 +690,24 @@ __device__ __forceinline__ bool grid16_far(double vx, double vy, double vz, doub
 // Tiles t0 .. t1 - 1 of 16 x 16 NKC sources for the wave's 64 targets: centre[t],
 // slots 256 NKC t .. (perm, wc), MFMA operands of chunk kc in the layout of
 // 16 x 16 tile NKC t + kc (wq), scale tscale[NKC t].  far_all: every target of
-// the wave passed grid16_far for these tiles (no per-tile check).  Adds into
-// lds.rr / lds.ri (FP32 partial sums, flushed every kGrid16Flush tiles) and, for
-// (target, tile) pairs that fail the bounds, into ex_r / ex_i.
-template <int SIGN, bool OBL, int NKC>
+// the wave passed grid16_far for these tiles (no per-tile check).  NEAR (16 x 16
+// tiles only): the near-field GEMM (grid16_near_ok), with the extra operands
+// wqn.  Adds into lds.rr / lds.ri (FP32 partial sums, flushed every
+// kGrid16Flush tiles) and, for (target, tile) pairs that fail the bounds, into
+// ex_r / ex_i.
+template <int SIGN, bool OBL, int NKC, bool NEAR = false>
 __device__ __forceinline__ void grid16_tiles(
     Grid16Lds& lds, const double4* __restrict__ centre, const int32_t* __restrict__ perm,
     const double* __restrict__ u, const float2* __restrict__ wc,
-    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t t0, uint32_t t1,
+    const uint4* __restrict__ wq, const uint4* __restrict__ wqn,
+    const float* __restrict__ tscale, uint32_t t0, uint32_t t1,
     double vx, double vy, double vz, double inv_lambda, double px, double py,
     const GridBounds& gb, bool far_all, double& ex_r, double& ex_i) {
   // every fused operation is written out, so that the same tiles give the same
   // bits whichever kernel this is inlined into
 #pragma clang fp contract(off)
   static_assert(NKC == 1 || NKC == 2, "16 or 32 columns");
+  static_assert(!NEAR || NKC == 1, "the near-field GEMM takes 16 x 16 tiles");
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t g = lane >> 4, j = lane & 15;
   // this lane's rows 8 rb + 2g + c and columns 16 kc + 4g + e, as k = index - centre
@@ -649,10 +720,14 @@ __device__ __forceinline__ void grid16_tiles(
   }
   const float px2f = (float)(px * px), py2f = (float)(py * py);
   const float px4f = px2f * px2f, py4f = py2f * py2f;
+  const float px6f = (float)(px * px * px * px * px * px), py6f = (float)(py * py * py * py * py * py);
   const float ipxf = (float)(1.0 / px), ipyf = (float)(1.0 / py);
-  // kb = s 4 pi c2 (-2 r_x) p_x p_y^2 2^13 = c2 r_x kbf
-  const float kbf = (float)((double)SIGN * -8.0 * M_PI * px * (py * py) * 8192.0);
+  // kb = s 4 pi c2 (-2 r_x) p_x p_y^2 / kCrossW = c2 r_x kbf
+  const float kbf = (float)((double)SIGN * -8.0 * M_PI * px * (py * py) / kCrossW);
+  // e2 = s 2 pi (2 c2 + 12 c3 r_x^2) p_x^2 p_y^2 / kCross2W = (2 c2 + 12 c3 r_x^2) e2f
+  const float e2f = (float)((double)SIGN * 2.0 * M_PI * (px * px) * (py * py) / kCross2W);
   // alpha_y = alpha (1 + mid / d^2): the row-midpoint share of the cross term
+  // (the near-field GEMM carries the whole k_a^2 share in its own operand)
   const double mid = -0.5 * kRowMid * px * px;
   const float rx2 = gb.rho_x * gb.rho_x, ry2 = gb.rho_y * gb.rho_y;
   const float rx2c = kRowDev * px2f;
@@ -660,44 +735,179 @@ __device__ __forceinline__ void grid16_tiles(
 #pragma unroll
   for (int B = 0; B < 4; ++B) fr[B] = fi[B] = 0.f;
   int since = 0;
+  // one tile's GEMM and row phasors for the wave's 64 targets (constants in
+  // lds.k), NB: with the near-field operands
+  const auto tile_gemm = [&](uint32_t t, auto nb) __attribute__((always_inline)) {
+    constexpr bool NB = decltype(nb)::value;
+    h8v W[6 * NKC];
+#pragma unroll
+    for (int q = 0; q < 6 * NKC; ++q)
+      W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 * NKC + q) * 64 + lane]);
+    h8v Wn[NB ? 4 : 1];
+    if constexpr (NB) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        Wn[q] = __builtin_bit_cast(h8v, wqn[((size_t)t * 4 + q) * 64 + lane]);
+    }
+    const float ts = tscale[(size_t)t * NKC];
+#pragma unroll
+    for (int B = 0; B < 4; ++B) {
+      const float4 k0 = lds.k[0][wv][16 * B + j];
+      const float4 k1 = lds.k[1][wv][16 * B + j];
+      const float4 k2 = lds.k[2][wv][16 * B + j];
+      const float4 k3 = lds.k[3][wv][16 * B + j];
+      float4 k4 = make_float4(0.f, 0.f, 0.f, 0.f), k5 = k4;
+      if constexpr (NB) {
+        k4 = lds.k[4][wv][16 * B + j];
+        k5 = lds.k[5][wv][16 * B + j];
+      }
+      f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int c = 0; c < NKC; ++c) {
+        h8v yh, yl, yq, yql, y2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float k = kc[c][q], kk = k * k;
+          const float Qt = __builtin_fmaf(k2.y, k, kk);
+          const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
+          float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
+          tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
+          if constexpr (NB) tq = __builtin_fmaf(k4.y * Qt, Qt * Qt, tq);
+          float sn, cs;
+          sincos_rev_f32(r + tq, sn, cs);
+          if (SIGN < 0) sn = -sn;
+          float am;
+          if constexpr (NB)
+            am = __builtin_fmaf(Qt, __builtin_fmaf(k4.w, Qt, k2.w), 1.f);  // 1 + k1 Q + k2 Q^2
+          else
+            am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
+          const float qk = NB ? Qt * __builtin_fmaf(k3.w, Qt, k3.z) : Qt * k3.z;
+          h2v hi, lo, cq;
+          y_operand(cs, sn, am, qk, hi, lo, cq);
+          yh[2 * q] = hi[0];
+          yh[2 * q + 1] = hi[1];
+          yl[2 * q] = lo[0];
+          yl[2 * q + 1] = lo[1];
+          if constexpr (NB) {
+            // the u correction split into f16 hi and lo; the u^2 correction
+            // (i phi2 - phi1^2 / 2) Y / kCross2W, phi1 p_x = qk kCrossW
+            const float aq = am * qk;
+            _Float16 h, l;
+            split_f16(cs * aq, h, l);
+            yq[2 * q] = h;
+            yql[2 * q] = l;
+            split_f16(sn * aq, h, l);
+            yq[2 * q + 1] = h;
+            yql[2 * q + 1] = l;
+            const float re2 = -0.5f * (kCrossW * kCrossW / kCross2W) * qk * qk, im2 = k5.x * Qt;
+            y2[2 * q] = (_Float16)(am * (cs * re2 - sn * im2));
+            y2[2 * q + 1] = (_Float16)(am * (cs * im2 + sn * re2));
+          } else {
+            yq[2 * q] = cq[0];
+            yq[2 * q + 1] = cq[1];
+          }
+        }
+        const h8v* Wc = W + 6 * c;
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yh, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yh, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yl, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yl, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[1], yh, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[4], yh, D1, 0, 0, 0);
+        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[2], yq, D0, 0, 0, 0);
+        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[5], yq, D1, 0, 0, 0);  // T / (tscale A)
+        if constexpr (NB) {
+          D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[2], yql, D0, 0, 0, 0);
+          D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[5], yql, D1, 0, 0, 0);
+          D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wn[0], yq, D0, 0, 0, 0);
+          D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wn[2], yq, D1, 0, 0, 0);
+          D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wn[1], y2, D0, 0, 0, 0);
+          D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wn[3], y2, D1, 0, 0, 0);
+        }
+      }
+      const float at = k3.y * ts, akt = at * k2.z, ak2t = at * k4.z;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float k = kr[q], kk = k * k;
+        const float Pt = __builtin_fmaf(k2.x, k, kk);
+        const float r = __builtin_amdgcn_fractf(k0.x * k);  // exact
+        float tp = __builtin_fmaf(k0.y, k, __builtin_fmaf(k1.x, kk, k3.x));
+        tp = __builtin_fmaf(k1.z * Pt, Pt, tp);
+        if constexpr (NB) tp = __builtin_fmaf(k4.x * Pt, Pt * Pt, tp);
+        float sn, cs;
+        sincos_rev_f32(r + tp, sn, cs);
+        float am;
+        if constexpr (NB)
+          am = __builtin_fmaf(Pt, __builtin_fmaf(ak2t, Pt, akt), at);  // tscale A (1 + k1 P + k2 P^2)
+        else
+          am = __builtin_fmaf(akt, Pt, at);  // tscale A (1 + kappa P)
+        const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
+        const f4v& D = (q >> 1) ? D1 : D0;
+        const float tr = D[2 * (q & 1)], ti = D[2 * (q & 1) + 1];
+        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
+        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
+      }
+    }
+  };
   for (uint32_t t = t0; t < t1; ++t) {
+    // near_mode (wave-uniform, NEAR only): some target of the wave fails the
+    // far-field bounds of this tile, so the wave takes the near-field GEMM for it
+    bool near_mode = false;
     {
       const double4 c = centre[t];
       const double rx = vx - c.x, ry = vy - c.y, rz = vz - c.z;
       double dref, id;
       hm_sqrt_inv(__builtin_fma(rz, rz, __builtin_fma(ry, ry, rx * rx)), dref, id);
-      // the linear phase needs FP64 (it spans several cycles and is split
-      // exactly); the small coefficients are FP32 products
       const double al = 0.5 * id * inv_lambda;
-      const double aly = al * __builtin_fma(mid, id * id, 1.0);
-      const double sx = -2.0 * al * rx * px, sy = -2.0 * aly * ry * py;
-      const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
-      float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
       const float idf = (float)id, alf = (float)al;
       const float rxf = (float)rx, ryf = (float)ry;
       const float id2f = idf * idf;
       const float c2f = -0.25f * alf * id2f;
       const float kf = OBL ? -id2f : -0.5f * id2f;
       const float amp = OBL ? __builtin_fabsf((float)rz) * id2f : idf;
-      bool near = false;
-      if (!far_all)
-        near = !grid16_pairs_ok(2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
-                                2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2, rx2, ry2, rx2c,
-                                2.f * alf, idf, kf);
+      const float Pm = 2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
+                  Qm = 2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2;
+      const bool far_ok = far_all || grid16_pairs_ok(Pm, Qm, rx2, ry2, rx2c, 2.f * alf, idf, kf);
+      bool near = !far_ok;
+      if constexpr (NEAR) {
+        near_mode = __builtin_amdgcn_ballot_w64(!far_ok) != 0;
+        if (near_mode)
+          near = !grid16_near_ok(Pm, Qm, __builtin_fabsf(rxf), gb.rho_x, rx2, ry2, 2.f * alf,
+                                 idf, OBL);
+      }
+      // the linear phase needs FP64 (it spans several cycles and is split
+      // exactly); the small coefficients are FP32 products
+      const double aly = al * __builtin_fma(near_mode ? 0.0 : mid, id * id, 1.0);
+      const double sx = -2.0 * al * rx * px, sy = -2.0 * aly * ry * py;
+      const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
+      float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
       float4 k1 = make_float4(alf * px2f, (float)aly * py2f, c2f * px4f, c2f * py4f);
       float4 k2 = make_float4(-2.f * rxf * ipxf, -2.f * ryf * ipyf, kf * px2f, kf * py2f);
-      float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp, c2f * rxf * kbf, 0.f);
+      const float kb = c2f * rxf * kbf;
+      float4 k3 = make_float4((float)frac_cycle(dref * inv_lambda), amp, kb, 0.f);
+      float4 k4 = make_float4(0.f, 0.f, 0.f, 0.f), k5 = k4;
+      if (NEAR && near_mode) {
+        // c3 = 1 / (16 D^5 lambda), k2 = 3 / (8 D^4) (|dz| / d^2: 1 / D^4)
+        const float c3f = 0.0625f * (2.f * alf) * id2f * id2f;
+        const float k2a = (OBL ? 1.f : 0.375f) * id2f * id2f;
+        k4 = make_float4(c3f * px6f, c3f * py6f, k2a * px4f, k2a * py4f);
+        // the q^2 part of the u correction: 6 c3 / (4 c2) p_y^2 = -0.75 p_y^2 / D^2
+        k3.w = -0.75f * kb * id2f * py2f;
+        k5.x = (2.f * c2f + 12.f * c3f * rxf * rxf) * e2f;
+      }
       if (near) {
         k0 = make_float4(0.f, 0.f, 0.f, 0.f);
-        k1 = k2 = k3 = k0;
+        k1 = k2 = k3 = k4 = k5 = k0;
       }
       lds.k[0][wv][lane] = k0;
       lds.k[1][wv][lane] = k1;
       lds.k[2][wv][lane] = k2;
       lds.k[3][wv][lane] = k3;
+      if (NEAR && near_mode) {
+        lds.k[4][wv][lane] = k4;
+        lds.k[5][wv][lane] = k5;
+      }
       if (near) {
-        const float Pm = 2.f * __builtin_fabsf(rxf) * gb.rho_x + rx2,
-                    Qm = 2.f * __builtin_fabsf(ryf) * gb.rho_y + ry2;
         if (gb.series && series_pairs_ok<OBL>(Pm + Qm, rx2, ry2, 2.f * alf, idf))
           series_tile_pairs<SIGN, OBL, NKC>(wc + (size_t)t * 256 * NKC, rx, ry, rz, dref, id,
                                             inv_lambda, px, py, ex_r, ex_i);
@@ -712,67 +922,13 @@ __device__ __forceinline__ void grid16_tiles(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    h8v W[6 * NKC];
-#pragma unroll
-    for (int q = 0; q < 6 * NKC; ++q)
-      W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 * NKC + q) * 64 + lane]);
-    const float ts = tscale[(size_t)t * NKC];
-#pragma unroll
-    for (int B = 0; B < 4; ++B) {
-      const float4 k0 = lds.k[0][wv][16 * B + j];
-      const float4 k1 = lds.k[1][wv][16 * B + j];
-      const float4 k2 = lds.k[2][wv][16 * B + j];
-      const float4 k3 = lds.k[3][wv][16 * B + j];
-      f4v D0 = {0.f, 0.f, 0.f, 0.f}, D1 = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int c = 0; c < NKC; ++c) {
-        h8v yh, yl, yq;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float k = kc[c][q], kk = k * k;
-          const float Qt = __builtin_fmaf(k2.y, k, kk);
-          const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
-          float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
-          tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
-          float sn, cs;
-          sincos_rev_f32(r + tq, sn, cs);
-          const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
-          h2v hi, lo, cq;
-          y_operand(cs, SIGN > 0 ? sn : -sn, am, Qt * k3.z, hi, lo, cq);
-          yh[2 * q] = hi[0];
-          yh[2 * q + 1] = hi[1];
-          yl[2 * q] = lo[0];
-          yl[2 * q + 1] = lo[1];
-          yq[2 * q] = cq[0];
-          yq[2 * q + 1] = cq[1];
-        }
-        const h8v* Wc = W + 6 * c;
-        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yh, D0, 0, 0, 0);
-        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yh, D1, 0, 0, 0);
-        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[0], yl, D0, 0, 0, 0);
-        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[3], yl, D1, 0, 0, 0);
-        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[1], yh, D0, 0, 0, 0);
-        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[4], yh, D1, 0, 0, 0);
-        D0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[2], yq, D0, 0, 0, 0);
-        D1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(Wc[5], yq, D1, 0, 0, 0);  // T / (tscale A)
-      }
-      const float at = k3.y * ts, akt = at * k2.z;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float k = kr[q], kk = k * k;
-        const float Pt = __builtin_fmaf(k2.x, k, kk);
-        const float r = __builtin_amdgcn_fractf(k0.x * k);  // exact
-        float tp = __builtin_fmaf(k0.y, k, __builtin_fmaf(k1.x, kk, k3.x));
-        tp = __builtin_fmaf(k1.z * Pt, Pt, tp);
-        float sn, cs;
-        sincos_rev_f32(r + tp, sn, cs);
-        const float am = __builtin_fmaf(akt, Pt, at);  // tscale A (1 + kappa P)
-        const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
-        const f4v& D = (q >> 1) ? D1 : D0;
-        const float tr = D[2 * (q & 1)], ti = D[2 * (q & 1) + 1];
-        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
-        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
-      }
+    if constexpr (NEAR) {
+      if (near_mode)
+        tile_gemm(t, std::true_type{});
+      else
+        tile_gemm(t, std::false_type{});
+    } else {
+      tile_gemm(t, std::false_type{});
     }
     if (++since == kGrid16Flush) {
       since = 0;
@@ -799,10 +955,11 @@ __device__ __forceinline__ void grid16_tiles(
 // n_tail sources after a padded lattice (tail_perm, tail_wc) are summed
 // exactly by every target in the first source split.
 template <int SIGN, bool OBL, int NKC>
-__global__ __launch_bounds__(kBlock) void k_superpose_grid16(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_superpose_grid16(
     const double4* __restrict__ centre, const int32_t* __restrict__ perm,
     const double* __restrict__ u, const float2* __restrict__ wc,
-    const uint4* __restrict__ wq, const float* __restrict__ tscale, uint32_t n_tiles,
+    const uint4* __restrict__ wq, const uint4* __restrict__ wqn,
+    const float* __restrict__ tscale, uint32_t n_tiles,
     uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
     double py, GridBounds gb, const double4* __restrict__ centre16, GridBounds gb16,
     const int32_t* __restrict__ tail_perm, const float2* __restrict__ tail_wc, uint32_t n_tail,
@@ -823,13 +980,22 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
   const float rx2c = kRowDev * (float)(px * px);
   double ex_r = 0.0, ex_i = 0.0;
   const bool far = __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb, rx2c));
-  if (NKC == 1 || far) {
-    grid16_tiles<SIGN, OBL, NKC>(lds, centre, perm, u, wc, wq, tscale, t0, t1, vx, vy, vz,
-                                 inv_lambda, px, py, gb, far, ex_r, ex_i);
+  if (far) {
+    grid16_tiles<SIGN, OBL, NKC>(lds, centre, perm, u, wc, wq, wqn, tscale, t0, t1, vx, vy, vz,
+                                 inv_lambda, px, py, gb, true, ex_r, ex_i);
   } else {
-    const bool far16 = __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb16, rx2c));
-    grid16_tiles<SIGN, OBL, 1>(lds, centre16, perm, u, wc, wq, tscale, 2 * t0, 2 * t1, vx, vy,
-                               vz, inv_lambda, px, py, gb16, far16, ex_r, ex_i);
+    // 16 x 16 tiles: those of the launch (NKC = 1), or the halves of its 16 x 32
+    // tiles, whose looser bounds keep more pairs on the far-field GEMM
+    const double4* c16 = NKC == 1 ? centre : centre16;
+    const GridBounds& b16 = NKC == 1 ? gb : gb16;
+    const uint32_t s0 = NKC * t0, s1 = NKC * t1;
+    const bool far16 = NKC == 1 ? false : __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb16, rx2c));
+    if (far16 || !b16.near_gemm)
+      grid16_tiles<SIGN, OBL, 1>(lds, c16, perm, u, wc, wq, wqn, tscale, s0, s1, vx, vy, vz,
+                                 inv_lambda, px, py, b16, far16, ex_r, ex_i);
+    else  // the near-field GEMM, with the series and exact paths beyond its bounds
+      grid16_tiles<SIGN, OBL, 1, true>(lds, c16, perm, u, wc, wq, wqn, tscale, s0, s1, vx, vy,
+                                       vz, inv_lambda, px, py, b16, false, ex_r, ex_i);
   }
   // in blocks of 256 sources, so that each FP32 partial sum stays short
   if (blockIdx.y == 0)
@@ -850,9 +1016,9 @@ __global__ __launch_bounds__(kBlock) void k_superpose_grid16(
 
 hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
                                    const GridTiles& halves, const float2* wc, const uint4* wq,
-                                   const float* tscale, const double* v, size_t M, double lambda,
-                                   double2* y, double2* partial, size_t partial_cap,
-                                   hipStream_t stream) {
+                                   const uint4* wqn, const float* tscale, const double* v,
+                                   size_t M, double lambda, double2* y, double2* partial,
+                                   size_t partial_cap, hipStream_t stream) {
   if (M == 0) return hipSuccess;
   if (nkc != 1 && nkc != 2) return hipErrorInvalidValue;
   if (nkc == 2 && (halves.n_tiles != 2 * gt.n_tiles || !halves.centre))
@@ -865,19 +1031,21 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
   double2* out = S > 1 ? partial : y;
   const double il = 1.0 / lambda;
   const int series = tuning().grid_series ? 1 : 0;
-  const GridBounds gb{gt.cx0, gt.cx1, gt.cy0, gt.cy1, gt.z, gt.rho_x, gt.rho_y, series};
-  const GridBounds gb16{halves.cx0, halves.cx1, halves.cy0, halves.cy1, halves.z, halves.rho_x,
-                        halves.rho_y, series};
+  const int near_gemm = tuning().grid_near && wqn ? 1 : 0;
+  const GridBounds gb{gt.cx0,  gt.cx1,   gt.cy0,   gt.cy1, gt.z,
+                      gt.rho_x, gt.rho_y, series, near_gemm};
+  const GridBounds gb16{halves.cx0,   halves.cx1,   halves.cy0, halves.cy1, halves.z,
+                        halves.rho_x, halves.rho_y, series,     near_gemm};
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
       if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
-            gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
+            gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
+            il, gt.px, gt.py, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
-            gt.centre, gt.perm, gt.u, wc, wq, tscale, gt.n_tiles, chunk, v, (uint32_t)M, il,
-            gt.px, gt.py, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
+            gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
+            il, gt.px, gt.py, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
     });
   });
   hipError_t e = hipGetLastError();

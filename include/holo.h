@@ -110,7 +110,10 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * (0/1: with "grid16", where the padded count of 16-column tile columns is
  * even, tiles of 16 x 32; default 1), "grid_series" (0/1: pairs of the 16-row
  * kernels too near for the GEMM factorisation are summed by the q-series to
- * third order where its bounds hold, else exactly; default 1), "mc_passes"
+ * third order where its bounds hold, else exactly; default 1), "grid_near"
+ * (0/1: waves of targets off the 16-row kernels' far-field bounds take the
+ * near-field GEMM, its cross factor to second order, before the q-series and
+ * exact paths; default 1), "mc_passes"
  * (per-target Monte Carlo: source ranges, one launch each, 0 = automatic,
  * 1 = one launch), "mc_lattice" (0/1: per-target
  * FP32-split Monte Carlo on product-grid sources reads positions from row

@@ -364,37 +364,46 @@ def test_lattice_padded_planes(pkg, ctx, oracle, workloads, n, aspect, sign, til
         assert rel_l2(y2, y32) > 1e-4
 
 
-@pytest.mark.parametrize("z", [0.1, 0.15, 0.25])
+# near-field paths of the 16-row kernels: (grid_near, grid_series)
+NEAR_PATHS = ((1, 1), (0, 1), (0, 0))
+
+
+@pytest.mark.parametrize("z", [0.06, 0.075, 0.1, 0.15, 0.25])
 @pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
 def test_lattice_near_field_series(pkg, ctx, oracle, workloads, z, sign, obliquity, tiles):
-    """Targets 0.1 - 0.25 m from a 10 um lattice, up to 8 mm off axis: the
-    GEMM factorisation's bounds fail for most (target, tile) pairs there; the
-    16-row kernels sum them by the q-series to third order per pair (option
-    grid_series) and stay within 1e-6 of the oracle, as does the exact path
-    (grid_series = 0)."""
+    """Targets 0.06 - 0.25 m from a 10 um lattice, up to 8 mm off axis: the
+    far-field GEMM's bounds fail for most (target, tile) pairs there.  The
+    16-row kernels take the near-field GEMM (cross factor to second order,
+    option grid_near), then the q-series to third order per pair (option
+    grid_series), then exact pairs; each combination stays within 1e-6 of the
+    oracle, and the near-field GEMM is a different path (other rounding)."""
     u = workloads.grid(256, 192, 10e-6, 0.0)
-    g = np.random.default_rng(int(z * 100))
+    g = np.random.default_rng(int(z * 1000))
     amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
     v = targets(int(z * 1000), 900, z, half=8e-3)
     sub = np.arange(0, 900, 3)
     ref = oracle.transform_full(sign, LAM, amp, phase, u, v[sub], obliquity=obliquity)
     plan = ctx.plan(u)
-    for series in (1, 0):
+    ys = {}
+    for near, series in NEAR_PATHS:
+        pkg.set_option("grid_near", near)
         pkg.set_option("grid_series", series)
         y = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
         err = rel_l2(y[sub], ref)
-        print(f"z={z} series={series}: rel L2 {err:.3e}")
+        print(f"z={z} near={near} series={series}: rel L2 {err:.3e}")
         assert err <= 1e-6
-    pkg.set_option("grid_series", 1)
+        ys[near, series] = y
+    if tiles != 8 and z < 0.2:
+        assert not np.array_equal(ys[1, 1], ys[0, 1])
 
 
 @pytest.mark.parametrize("seed", range(8))
 def test_lattice_random_near_field(pkg, ctx, seed, tiles):
     """Seeded near-field geometries (4 - 20 cm, targets up to 1 cm off axis,
     pitches 4 - 16 um, 400 - 700 nm, either sign, with or without obliquity),
-    where the GEMM bounds fail for most (target, tile) pairs and the series
-    path and the exact path share the work: within 1e-6 of FP64, with the
-    series path on and off."""
+    where the far-field GEMM bounds fail for most (target, tile) pairs and the
+    near-field GEMM, the series path and the exact path share the work: within
+    1e-6 of FP64 with each of them switched on and off."""
     g = np.random.default_rng(2000 + seed)
     nx, ny = 16 * int(g.integers(2, 17)), 32 * int(g.integers(1, 7))
     px, py = g.uniform(4e-6, 16e-6, 2)
@@ -413,13 +422,14 @@ def test_lattice_random_near_field(pkg, ctx, seed, tiles):
     plan = ctx.plan(u)
     assert plan.lattice() == (nx, ny)
     y64 = plan.transform(amp, phase, v, sign, lam, pkg.FP64, obliquity=obl)
-    for series in (1, 0):
+    for near, series in NEAR_PATHS:
+        pkg.set_option("grid_near", near)
         pkg.set_option("grid_series", series)
         y32 = plan.transform(amp, phase, v, sign, lam, pkg.FP32_SPLIT, obliquity=obl)
         err = rel_l2(y32, y64)
-        print(f"seed {seed}: {nx}x{ny}, z {z:.3f} m, series {series}: rel L2 {err:.3e}")
+        print(f"seed {seed}: {nx}x{ny}, z {z:.3f} m, near {near} series {series}: "
+              f"rel L2 {err:.3e}")
         assert err <= 1e-6
-    pkg.set_option("grid_series", 1)
 
 
 def test_lattice_near_field_full_plane(pkg, ctx, oracle, workloads, tiles):
