@@ -432,6 +432,27 @@ def test_lattice_random_near_field(pkg, ctx, seed, tiles):
         assert err <= 1e-6
 
 
+@pytest.mark.parametrize("z", [0.07, 0.12])
+def test_lattice_near_field_complex_weights_and_adjoint(pkg, ctx, oracle, workloads, z, tiles):
+    """Complex weights (the adjoint's sources) on a lattice plan with targets in
+    the near field, and the adjoint of such a plane: the near-field GEMM takes
+    the weights' real form like the far-field one."""
+    u = workloads.grid(160, 128, 8e-6, 0.0)
+    g = np.random.default_rng(int(z * 1000))
+    gz = g.standard_normal(len(u)) + 1j * g.standard_normal(len(u))
+    v = targets(int(z * 100), 700, z, half=6e-3)
+    plan = ctx.plan(u)
+    for near in (1, 0):
+        pkg.set_option("grid_near", near)
+        yc = plan.transform_complex(gz, v, -1, LAM, pkg.FP32_SPLIT)
+        assert rel_l2(yc, oracle.transform_cartesian(-1, LAM, gz, u, v)) <= 1e-6
+        amp, theta = weights(int(z * 1000) + 1, len(v))
+        gt = g.standard_normal(len(u)) + 1j * g.standard_normal(len(u))
+        ga, gth = plan.adjoint(gt, amp, theta, v, +1, LAM, pkg.FP32_SPLIT)
+        ra, rt = oracle.adjoint(+1, LAM, amp, theta, v, gt, u)
+        assert rel_l2(ga, ra) <= 1e-6 and rel_l2(gth, rt) <= 1e-6
+
+
 def test_lattice_near_field_full_plane(pkg, ctx, oracle, workloads, tiles):
     """The benchmark's 1024^2 lattice with the 1024^2 target plane moved to
     z = 0.1 m: FP32-split within 1e-6 of the FP64 kernel over every target and
