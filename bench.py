@@ -146,8 +146,9 @@ def main():
                     help="timed repeats (after one warm-up) of each side-section measurement")
     ap.add_argument("--sections", default="chain,fp32,mc,adjoint",
                     help="comma list of chain, fp32, mc, adjoint (profiling runs pick one), "
-                         "and lattices (not run by default: the FP32-split lattice kernel's "
-                         "tile shapes on narrow and square lattices)")
+                         "and lattices and distance (not run by default: the FP32-split "
+                         "lattice kernel's tile shapes on narrow and square lattices, and its "
+                         "speed against FP64 from 0.05 to 0.35 m)")
     ap.add_argument("--hologram-steps", type=int, default=20,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
     ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32), default=32,
@@ -316,6 +317,9 @@ def main():
     if "lattices" in sections:
         out["lattices"] = run_lattices(holo, ctx, wl, T, max(5, a.repeats))
 
+    if "distance" in sections:
+        out["distance"] = run_distance(holo, ctx, wl, T)
+
     if "adjoint" in sections:
         hol = importlib.import_module("holo_amd.hologram")
         n = max(20, a.hologram_steps)
@@ -332,6 +336,40 @@ def main():
 
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def run_distance(holo, ctx, wl, T, R=3):
+    """FP32-split against FP64 on the 1024^2 plane (10 um) -> 1024^2 targets
+    (7.56 um) as the target plane moves from 0.05 to 0.35 m, with the lattice
+    kernel's near-field GEMM on (default) and off."""
+    import torch
+    p3 = wl.plane_1024(0).stages[0]
+    amp, ph, u = T(p3.amp), T(p3.phase), T(p3.u)
+    plan = holo.Plan(ctx, u)
+    res = {}
+    for z in (0.05, 0.06, 0.075, 0.1, 0.15, 0.2, 0.25, 0.35):
+        vz = p3.v.copy()
+        vz[:, 2] = z
+        v = T(vz)
+        y64 = torch.empty((v.shape[0], 2), dtype=torch.float64, device=v.device)
+        y32 = torch.empty_like(y64)
+        t64 = time_events(lambda: plan.transform(amp, ph, v, p3.sign, p3.lam, holo.FP64,
+                                                 out=y64), R)
+        f64 = torch.view_as_complex(y64).cpu().numpy()
+        row = {"fp64_ms": t64["median_ms"]}
+        for near in (1, 0):
+            holo.set_option("grid_near", near)
+            t32 = time_events(lambda: plan.transform(amp, ph, v, p3.sign, p3.lam,
+                                                     holo.FP32_SPLIT, out=y32), R)
+            key = "near_gemm" if near else "series_exact"
+            row[key] = {"ms": t32["median_ms"], "speedup": t64["median_ms"] / t32["median_ms"],
+                        "rel_l2_vs_fp64": rel_l2(torch.view_as_complex(y32).cpu().numpy(), f64)}
+        holo.set_option("grid_near", 1)
+        res[f"{z:g}"] = row
+        log(f"[bench] distance {z:g} m: FP64 {row['fp64_ms']:.1f} ms, FP32-split "
+            f"{row['near_gemm']['speedup']:.2f}x (near-field GEMM off: "
+            f"{row['series_exact']['speedup']:.2f}x), rel L2 {row['near_gemm']['rel_l2_vs_fp64']:.3g}")
+    return res
 
 
 def run_lattices(holo, ctx, wl, T, R):
