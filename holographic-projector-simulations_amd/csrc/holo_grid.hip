@@ -116,6 +116,19 @@ __device__ __forceinline__ void y_operand(float c, float s, float am, float qk, 
   q[1] = (_Float16)(s * aq);
 }
 
+// The far-field Y operand of the 16-row kernels: as y_operand, with the
+// correction operand (c, s) qk without the factor am.  That drops
+// (am - 1) times the cross term, below |kappa| Qm times the cross bound
+// (grid16_pairs_ok bounds the product), and one multiply per column.
+__device__ __forceinline__ void y_operand_far(float c, float s, float am, float qk, h2v& hi,
+                                              h2v& lo, h2v& q) {
+  hi = __builtin_bit_cast(h2v, __builtin_amdgcn_cvt_pkrtz(c, s));
+  lo[0] = (_Float16)__builtin_fmaf(am, c, -(float)hi[0]);
+  lo[1] = (_Float16)__builtin_fmaf(am, s, -(float)hi[1]);
+  q[0] = (_Float16)(c * qk);
+  q[1] = (_Float16)(s * qk);
+}
+
 // MFMA operands of each tile's weights.  One wave per tile; lane l = 16 g + i
 // holds row i = 2a + c, columns b = 2g, 2g+1 (re, im) of the real form of
 // W / tscale, split into f16 hi and lo, and of i gx'_a W / tscale (gx' =
@@ -417,8 +430,9 @@ __device__ __forceinline__ bool grid16_pairs_ok(float Pm, float Qm, float rx2, f
   const float beta = kFourPi * 0.125f * ia * idf * idf;
   const float cross = beta * Pm * Qm;
   const float ampsep = kappa * kappa * Pm * Qm;
+  const float ampq = __builtin_fabsf(kappa) * Qm * cross;  // y_operand_far's dropped share
   return round_err <= kGuardCycles && trunc_err <= 0.05f * kGuardCycles && cross <= 1.5e-3f &&
-         beta * rx2c * Qm <= 1e-6f && ampsep <= 1e-6f;
+         beta * rx2c * Qm <= 1e-6f && ampsep <= 1e-6f && ampq <= 1e-7f;
 }
 
 // Near-field GEMM of the 16 x 16 tiles.  The cross factor is carried to second
@@ -782,8 +796,8 @@ __device__ __forceinline__ void grid16_tiles(
           else
             am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
           const float qk = NB ? Qt * __builtin_fmaf(k3.w, Qt, k3.z) : Qt * k3.z;
-          h2v hi, lo, cq;
-          y_operand(cs, sn, am, qk, hi, lo, cq);
+          h2v hi, lo, cq;  // cq: the far-field correction (the near-field GEMM forms its own)
+          y_operand_far(cs, sn, am, qk, hi, lo, cq);
           yh[2 * q] = hi[0];
           yh[2 * q + 1] = hi[1];
           yl[2 * q] = lo[0];
