@@ -61,7 +61,9 @@ def read_profile(kernel: str, key: str):
     return None
 
 
-CPU_SHARE = 16  # CPUs a one-GPU job gets on the GPU box, whatever os.cpu_count() reports
+# The CPU share of a one-GPU job on a shared multi-GPU host (16 of its cores), whatever
+# os.cpu_count() reports there; on smaller hosts every available core is used.
+CPU_SHARE = 16
 
 
 def cpu_baseline(stage, n_targets: int, repeats: int):
