@@ -247,11 +247,13 @@ holo_status finish_out(holo_ctx* ctx, T* host, const T* devp, size_t count,
 
 // ---------------------------------------------------------------------------
 // Error report of a result r at M targets (holo.h): the FP64 formulation at
-// count = min(M, kErrTargets) stride targets, then sqrt(sum |r - r64|^2 /
-// sum |r64|^2) on the device.  The scratch is one context buffer.
+// count = min(M, kErrTargets) targets drawn by the energy of r, then the
+// weighted mean square of their relative errors on the device.  The scratch
+// is one context buffer.
 struct ErrScratch {
   double* err = nullptr;      // the report, when the caller's pointer is host memory
-  uint32_t* tmap = nullptr;   // stride subset m_i
+  double* energy = nullptr;   // kErrChunks partial sums of |r|^2
+  uint32_t* tmap = nullptr;   // sampled targets m_i
   double* vsub = nullptr;     // target positions at m_i
   double2* ref = nullptr;     // FP64 values at m_i
   double2* partial = nullptr; // split partials of the FP64 launch
@@ -262,13 +264,15 @@ constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 holo_status err_scratch(holo_ctx* ctx, ErrScratch* e) {
   const size_t n = holo::kErrTargets, np = holo::kFewTargetSplits * n;
-  const size_t o_tmap = 256, o_vsub = o_tmap + align256(n * sizeof(uint32_t)),
+  const size_t o_energy = 256, o_tmap = o_energy + align256(holo::kErrChunks * sizeof(double)),
+               o_vsub = o_tmap + align256(n * sizeof(uint32_t)),
                o_ref = o_vsub + align256(3 * n * sizeof(double)),
                o_part = o_ref + align256(n * sizeof(double2)),
                total = o_part + np * sizeof(double2);
   HOLO_TRY(ensure(ctx->errb, total, ctx->device));
   char* b = static_cast<char*>(ctx->errb.ptr);
   e->err = reinterpret_cast<double*>(b);
+  e->energy = reinterpret_cast<double*>(b + o_energy);
   e->tmap = reinterpret_cast<uint32_t*>(b + o_tmap);
   e->vsub = reinterpret_cast<double*>(b + o_vsub);
   e->ref = reinterpret_cast<double2*>(b + o_ref);
@@ -296,10 +300,11 @@ holo_status report_error(holo_ctx* ctx, bool fp64, const double* v, size_t M, co
   ErrScratch e;
   HOLO_TRY(err_scratch(ctx, &e));
   const size_t count = std::min(M, holo::kErrTargets);
-  HOLO_HIP(holo::launch_subset(v, M, count, e.tmap, e.vsub, ctx->stream));
+  const double2* r = reinterpret_cast<const double2*>(y);
+  HOLO_HIP(holo::launch_error_targets(r, M, v, count, e.energy, e.tmap, e.vsub, ctx->stream));
   HOLO_TRY(run(e, count));
-  HOLO_HIP(holo::launch_rel_l2(reinterpret_cast<const double2*>(y), e.tmap, e.ref, count,
-                               dev ? err_out : e.err, ctx->stream));
+  HOLO_HIP(holo::launch_rel_l2(r, M, e.tmap, e.ref, count, e.energy, dev ? err_out : e.err,
+                               ctx->stream));
   if (!dev)
     HOLO_HIP(hipMemcpyAsync(err_out, e.err, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   return HOLO_OK;

@@ -257,15 +257,19 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
 
 hipError_t launch_zero(double* x, size_t n, hipStream_t stream);
 
-// Error report of an FP32-split result y (M targets): tmap[i] = floor(i M /
-// count) and vsub[i] = v[tmap[i]] for i < count; then *err (device) =
-// sqrt(sum_i |y[tmap[i]] - ref[i]|^2 / sum_i |ref[i]|^2), summed in a fixed
-// order, with ref the FP64 sum at vsub.
+// Error report of an FP32-split result y (M values): launch_error_targets
+// picks count <= kErrTargets targets by the energy of y (tmap[i], positions
+// vsub[i]; E: kErrChunks doubles of partial energies, kept for
+// launch_rel_l2), the caller evaluates the FP64 sum ref at vsub, and
+// launch_rel_l2 writes the report to *err (device): the sampled relative
+// errors' mean square, weighted as the sampling implies, or with no finite
+// positive energy the stride-subset relative L2.  Fixed summation orders.
 constexpr size_t kErrTargets = 256;
-hipError_t launch_subset(const double* v, size_t M, size_t count, uint32_t* tmap, double* vsub,
-                         hipStream_t stream);
-hipError_t launch_rel_l2(const double2* y, const uint32_t* tmap, const double2* ref,
-                         size_t count, double* err, hipStream_t stream);
+constexpr size_t kErrChunks = 256;
+hipError_t launch_error_targets(const double2* y, size_t M, const double* v, size_t count,
+                                double* E, uint32_t* tmap, double* vsub, hipStream_t stream);
+hipError_t launch_rel_l2(const double2* y, size_t M, const uint32_t* tmap, const double2* ref,
+                         size_t count, const double* E, double* err, hipStream_t stream);
 
 // Adjoint epilogue: grad_amp = Re(e^{-i theta} b), grad_theta = A Im(...).
 hipError_t launch_adjoint_epilogue(const double2* b, const double* amp,

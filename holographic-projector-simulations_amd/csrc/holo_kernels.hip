@@ -1153,29 +1153,17 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
 }
 
 // ---------------------------------------------------------------------------
-// Error report of an FP32-split result: the FP64 sum at a stride subset of
-// its targets (tmap[i] = floor(i M / count)), then the relative L2 distance
-// in a fixed order.
-
-__global__ __launch_bounds__(kBlock) void k_subset(const double* __restrict__ v, uint32_t M,
-                                                   uint32_t count, uint32_t* __restrict__ tmap,
-                                                   double* __restrict__ vsub) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= count) return;
-  const uint32_t m = (uint32_t)(((uint64_t)i * M) / count);
-  tmap[i] = m;
-  vsub[3 * (size_t)i] = v[3 * (size_t)m];
-  vsub[3 * (size_t)i + 1] = v[3 * (size_t)m + 1];
-  vsub[3 * (size_t)i + 2] = v[3 * (size_t)m + 2];
-}
-
-hipError_t launch_subset(const double* v, size_t M, size_t count, uint32_t* tmap, double* vsub,
-                         hipStream_t stream) {
-  if (!count) return hipSuccess;
-  const unsigned blocks = (unsigned)((count + kBlock - 1) / kBlock);
-  k_subset<<<blocks, kBlock, 0, stream>>>(v, (uint32_t)M, (uint32_t)count, tmap, vsub);
-  return hipGetLastError();
-}
+// Error report of an FP32-split result y (M values).  The sampled targets are
+// chosen by the result's energy, so that their relative errors, averaged,
+// estimate the relative L2 error of the whole result: m_i (i < count) is the
+// target at which the running sum of |y|^2 in target order first exceeds
+// (i + 1/2) / count of the total (systematic sampling with probability
+// |y_m|^2 / sum |y|^2), and
+//   err = sqrt((1 / count) sum_i |y[m_i] - ref_i|^2 / |y[m_i]|^2),
+// whose expectation over a random offset of the strata is
+// sum_m |y_m - y64_m|^2 / sum_m |y_m|^2.  A result without a finite positive
+// total takes stride targets m_i = floor(i M / count) and
+// sqrt(sum |y - ref|^2 / sum |ref|^2) there.  Every sum has a fixed order.
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -1183,18 +1171,147 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
-// One workgroup: sqrt(sum_i |y[tmap[i]] - ref[i]|^2 / sum_i |ref[i]|^2).
+// inclusive prefix sum over the 64 lanes (a fixed order)
+__device__ __forceinline__ double wave_scan(double x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double t = __shfl_up(x, off);
+    if (lane >= off) x += t;
+  }
+  return x;
+}
+
+__device__ __forceinline__ double energy(const double2 a) { return a.x * a.x + a.y * a.y; }
+
+// sum of E[0 .. n) by one wave, the same bits wherever it is called
+__device__ __forceinline__ double energy_total(const double* __restrict__ E, uint32_t n) {
+  double s = 0.0;
+  for (uint32_t b = threadIdx.x & 63; b < n; b += 64) s += E[b];
+  return wave_sum(s);
+}
+
+__device__ __forceinline__ bool energy_mode(double total) {
+  return total > 0.0 && __builtin_isfinite(total);  // NaN fails
+}
+
+// Block b: E[b] = sum of |y|^2 over its chunk [b L, min(M, (b + 1) L)).
+__global__ __launch_bounds__(kBlock) void k_energy_chunks(const double2* __restrict__ y,
+                                                          uint32_t M, uint32_t L,
+                                                          double* __restrict__ E) {
+  __shared__ double red[kBlock / 64];
+  const uint64_t b0 = (uint64_t)blockIdx.x * L, b1 = min<uint64_t>(M, b0 + L);
+  double s = 0.0;
+  for (uint64_t m = b0 + threadIdx.x; m < b1; m += kBlock) s += energy(y[m]);
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) t += red[w];
+    E[blockIdx.x] = t;
+  }
+}
+
+// One wave per sample i: m_i as above, into tmap[i] and its position into vsub.
+__global__ __launch_bounds__(kBlock) void k_error_targets(
+    const double2* __restrict__ y, uint32_t M, uint32_t L, const double* __restrict__ E,
+    uint32_t nchunks, const double* __restrict__ v, uint32_t count, uint32_t* __restrict__ tmap,
+    double* __restrict__ vsub) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  if (i >= count) return;  // the whole wave
+  const double total = energy_total(E, nchunks);
+  uint32_t m = (uint32_t)(((uint64_t)i * M) / count);
+  if (energy_mode(total)) {
+    const double c = ((double)i + 0.5) / (double)count * total;
+    // the chunk where the running sum passes c (else the last one with energy)
+    uint32_t cb = nchunks, lastb = nchunks;
+    double before = 0.0, lastbefore = 0.0, run = 0.0;
+    for (uint32_t base = 0; base < nchunks && cb == nchunks; base += 64) {
+      const double e = base + lane < nchunks ? E[base + lane] : 0.0;
+      const double incl = wave_scan(e) + run;
+      const uint64_t hit = __ballot(incl > c && e > 0.0);
+      const uint64_t pos = __ballot(e > 0.0);
+      if (hit) {
+        const uint32_t f = (uint32_t)__builtin_ctzll(hit);
+        cb = base + f;
+        before = __shfl(incl - e, f);
+      } else if (pos) {
+        const uint32_t f = 63u - (uint32_t)__builtin_clzll(pos);
+        lastb = base + f;
+        lastbefore = __shfl(incl - e, f);
+      }
+      run = __shfl(incl, 63);
+    }
+    if (cb == nchunks) {
+      cb = lastb;
+      before = lastbefore;
+    }
+    // within the chunk, likewise (else its last target with energy)
+    const uint64_t b0 = (uint64_t)cb * L, b1 = min<uint64_t>(M, b0 + L);
+    uint64_t found = b1, last = b0;
+    run = before;
+    for (uint64_t base = b0; base < b1 && found == b1; base += 64) {
+      const double e = base + lane < b1 ? energy(y[base + lane]) : 0.0;
+      const double incl = wave_scan(e) + run;
+      const uint64_t hit = __ballot(incl > c && e > 0.0);
+      const uint64_t pos = __ballot(e > 0.0);
+      if (hit) found = base + (uint64_t)__builtin_ctzll(hit);
+      else if (pos) last = base + 63u - (uint64_t)__builtin_clzll(pos);
+      run = __shfl(incl, 63);
+    }
+    m = (uint32_t)(found < b1 ? found : last);
+  }
+  if (lane == 0) {
+    tmap[i] = m;
+    vsub[3 * (size_t)i] = v[3 * (size_t)m];
+    vsub[3 * (size_t)i + 1] = v[3 * (size_t)m + 1];
+    vsub[3 * (size_t)i + 2] = v[3 * (size_t)m + 2];
+  }
+}
+
+uint32_t error_chunk_len(size_t M) {
+  return (uint32_t)std::max<size_t>((M + kErrChunks - 1) / kErrChunks, 1);
+}
+
+hipError_t launch_error_targets(const double2* y, size_t M, const double* v, size_t count,
+                                double* E, uint32_t* tmap, double* vsub, hipStream_t stream) {
+  if (!count) return hipSuccess;
+  const uint32_t L = error_chunk_len(M);
+  const uint32_t nchunks = (uint32_t)((M + L - 1) / L);
+  k_energy_chunks<<<nchunks, kBlock, 0, stream>>>(y, (uint32_t)M, L, E);
+  const unsigned blocks = (unsigned)((count + kBlock / 64 - 1) / (kBlock / 64));
+  k_error_targets<<<blocks, kBlock, 0, stream>>>(y, (uint32_t)M, L, E, nchunks, v,
+                                                 (uint32_t)count, tmap, vsub);
+  return hipGetLastError();
+}
+
+// One workgroup: the report from the sampled targets (energy or stride mode,
+// as k_error_targets chose from the same E).
 __global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y,
                                                    const uint32_t* __restrict__ tmap,
                                                    const double2* __restrict__ ref,
-                                                   uint32_t count, double* __restrict__ err) {
+                                                   uint32_t count, const double* __restrict__ E,
+                                                   uint32_t nchunks, double* __restrict__ err) {
   __shared__ double red[2][kBlock / 64];
+  __shared__ double s_total;
+  if (threadIdx.x < 64) {
+    const double t = energy_total(E, nchunks);
+    if (threadIdx.x == 0) s_total = t;
+  }
+  __syncthreads();
+  const bool weighted = energy_mode(s_total);
   double num = 0.0, den = 0.0;
   for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
     const double2 a = y[tmap[i]], b = ref[i];
     const double dr = a.x - b.x, di = a.y - b.y;
-    num += dr * dr + di * di;
-    den += b.x * b.x + b.y * b.y;
+    if (weighted) {
+      num += (dr * dr + di * di) / energy(a);
+    } else {
+      num += dr * dr + di * di;
+      den += energy(b);
+    }
   }
   num = wave_sum(num);
   den = wave_sum(den);
@@ -1209,13 +1326,16 @@ __global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y
       n += red[0][w];
       d += red[1][w];
     }
-    err[0] = d > 0.0 ? __builtin_sqrt(n / d) : (n > 0.0 ? INFINITY : 0.0);
+    if (weighted) err[0] = __builtin_sqrt(n / (double)count);
+    else err[0] = d > 0.0 ? __builtin_sqrt(n / d) : (n > 0.0 ? INFINITY : 0.0);
   }
 }
 
-hipError_t launch_rel_l2(const double2* y, const uint32_t* tmap, const double2* ref,
-                         size_t count, double* err, hipStream_t stream) {
-  k_rel_l2<<<1, kBlock, 0, stream>>>(y, tmap, ref, (uint32_t)count, err);
+hipError_t launch_rel_l2(const double2* y, size_t M, const uint32_t* tmap, const double2* ref,
+                         size_t count, const double* E, double* err, hipStream_t stream) {
+  const uint32_t L = error_chunk_len(M);
+  k_rel_l2<<<1, kBlock, 0, stream>>>(y, tmap, ref, (uint32_t)count, E,
+                                     (uint32_t)((M + L - 1) / L), err);
   return hipGetLastError();
 }
 

@@ -29,11 +29,15 @@
  *  - Error report (err_rel_l2, nullable, of every call that returns a field
  *    or gradients): for HOLO_FP32_SPLIT the call also evaluates the FP64
  *    formulation at count = min(M, 256) stride targets m_i = floor(i M /
- *    count) and writes sqrt(sum_i |r[m_i] - r64_i|^2 / sum_i |r64_i|^2) of
- *    its result r; for HOLO_FP64 it writes 0 (the result is the FP64
- *    formulation).  The extra work is 256 N FP64 pairs (256 K samples for
- *    Monte Carlo).  err_rel_l2 is a host pointer, or a device pointer with
- *    HOLO_DEVICE_POINTERS (the call then stays asynchronous).
+ *    count) and writes sqrt((M / count) sum_i |r[m_i] - r64_i|^2 /
+ *    sum_m |r[m]|^2) for its result r of M values: the relative L2 error of
+ *    the whole result, its squared error estimated from the sampled targets
+ *    and its norm taken over every target (exact when M <= 256, up to the
+ *    FP32-split error of that norm); for HOLO_FP64 it writes 0 (the result is
+ *    the FP64 formulation).  The extra work is 256 N FP64 pairs (256 K
+ *    samples for Monte Carlo) and one pass over r.  err_rel_l2 is a host
+ *    pointer, or a device pointer with HOLO_DEVICE_POINTERS (the call then
+ *    stays asynchronous).
  */
 #ifndef HOLO_H
 #define HOLO_H
