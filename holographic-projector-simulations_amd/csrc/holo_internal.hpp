@@ -258,12 +258,13 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
 hipError_t launch_zero(double* x, size_t n, hipStream_t stream);
 
 // Error report of an FP32-split result y (M values): launch_error_targets
-// picks count <= kErrTargets targets by the energy of y (tmap[i], positions
-// vsub[i]; E: kErrChunks doubles of partial energies, kept for
-// launch_rel_l2), the caller evaluates the FP64 sum ref at vsub, and
-// launch_rel_l2 writes the report to *err (device): the sampled relative
-// errors' mean square, weighted as the sampling implies, or with no finite
-// positive energy the stride-subset relative L2.  Fixed summation orders.
+// picks count <= kErrTargets targets, half drawn by the energy of y and half
+// spread evenly (every target when M <= count) (tmap[i], positions vsub[i];
+// E: kErrChunks doubles of partial energies, kept for launch_rel_l2), the
+// caller evaluates the FP64 sum ref at vsub, and launch_rel_l2 writes the
+// report to *err (device): the importance-weighted estimate of the whole
+// result's relative L2 error, or with no finite positive energy the even
+// targets' relative L2.  Fixed summation orders.
 constexpr size_t kErrTargets = 256;
 constexpr size_t kErrChunks = 256;
 hipError_t launch_error_targets(const double2* y, size_t M, const double* v, size_t count,

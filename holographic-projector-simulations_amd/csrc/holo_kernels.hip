@@ -1153,17 +1153,20 @@ hipError_t launch_normalize_polar(const double2* y, size_t M, double* amp,
 }
 
 // ---------------------------------------------------------------------------
-// Error report of an FP32-split result y (M values).  The sampled targets are
-// chosen by the result's energy, so that their relative errors, averaged,
-// estimate the relative L2 error of the whole result: m_i (i < count) is the
-// target at which the running sum of |y|^2 in target order first exceeds
-// (i + 1/2) / count of the total (systematic sampling with probability
-// |y_m|^2 / sum |y|^2), and
-//   err = sqrt((1 / count) sum_i |y[m_i] - ref_i|^2 / |y[m_i]|^2),
-// whose expectation over a random offset of the strata is
-// sum_m |y_m - y64_m|^2 / sum_m |y_m|^2.  A result without a finite positive
-// total takes stride targets m_i = floor(i M / count) and
-// sqrt(sum |y - ref|^2 / sum |ref|^2) there.  Every sum has a fixed order.
+// Error report of an FP32-split result y (M values), from count FP64 values
+// at sampled targets: n_e of them drawn by the result's energy and n_u
+// spread evenly (multiple importance sampling with the balance heuristic).
+// Energy draw i < n_e is the target at which the running sum of |y|^2 in
+// target order first exceeds (i + 1/2) / n_e of the total T; even draw j is
+// target floor(j M / n_u).  Then
+//   err^2 = sum_i |y[m_i] - ref_i|^2 / (n_e |y[m_i]|^2 + n_u T / M),
+// an unbiased estimate (under random offsets of the strata) of
+// sum_m |y_m - y64_m|^2 / T, the whole result's squared relative L2 error:
+// the energy draws find error that grows with the field (a focus), the even
+// ones error spread over dark targets.  M <= count takes every target once
+// (n_e = 0: the exact value).  A result without a finite positive T takes
+// the even targets and sqrt(sum |y - ref|^2 / sum |ref|^2) there.  Every sum
+// has a fixed order.
 
 __device__ __forceinline__ double wave_sum(double x) {
 #pragma unroll
@@ -1216,15 +1219,17 @@ __global__ __launch_bounds__(kBlock) void k_energy_chunks(const double2* __restr
 // One wave per sample i: m_i as above, into tmap[i] and its position into vsub.
 __global__ __launch_bounds__(kBlock) void k_error_targets(
     const double2* __restrict__ y, uint32_t M, uint32_t L, const double* __restrict__ E,
-    uint32_t nchunks, const double* __restrict__ v, uint32_t count, uint32_t* __restrict__ tmap,
-    double* __restrict__ vsub) {
+    uint32_t nchunks, const double* __restrict__ v, uint32_t count, uint32_t ne,
+    uint32_t* __restrict__ tmap, double* __restrict__ vsub) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t i = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
   if (i >= count) return;  // the whole wave
   const double total = energy_total(E, nchunks);
-  uint32_t m = (uint32_t)(((uint64_t)i * M) / count);
-  if (energy_mode(total)) {
-    const double c = ((double)i + 0.5) / (double)count * total;
+  const bool draw = energy_mode(total) && i < ne;
+  const uint32_t j = energy_mode(total) ? i - ne : i, nu = energy_mode(total) ? count - ne : count;
+  uint32_t m = draw ? 0u : (uint32_t)(((uint64_t)j * M) / nu);
+  if (draw) {
+    const double c = ((double)i + 0.5) / (double)ne * total;
     // the chunk where the running sum passes c (else the last one with energy)
     uint32_t cb = nchunks, lastb = nchunks;
     double before = 0.0, lastbefore = 0.0, run = 0.0;
@@ -1275,6 +1280,10 @@ uint32_t error_chunk_len(size_t M) {
   return (uint32_t)std::max<size_t>((M + kErrChunks - 1) / kErrChunks, 1);
 }
 
+uint32_t error_energy_draws(size_t M, size_t count) {
+  return M <= count ? 0u : (uint32_t)(count / 2);
+}
+
 hipError_t launch_error_targets(const double2* y, size_t M, const double* v, size_t count,
                                 double* E, uint32_t* tmap, double* vsub, hipStream_t stream) {
   if (!count) return hipSuccess;
@@ -1283,16 +1292,18 @@ hipError_t launch_error_targets(const double2* y, size_t M, const double* v, siz
   k_energy_chunks<<<nchunks, kBlock, 0, stream>>>(y, (uint32_t)M, L, E);
   const unsigned blocks = (unsigned)((count + kBlock / 64 - 1) / (kBlock / 64));
   k_error_targets<<<blocks, kBlock, 0, stream>>>(y, (uint32_t)M, L, E, nchunks, v,
-                                                 (uint32_t)count, tmap, vsub);
+                                                 (uint32_t)count, error_energy_draws(M, count),
+                                                 tmap, vsub);
   return hipGetLastError();
 }
 
-// One workgroup: the report from the sampled targets (energy or stride mode,
-// as k_error_targets chose from the same E).
-__global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y,
+// One workgroup: the report from the sampled targets (weighted, or the even
+// targets alone, as k_error_targets chose from the same E).
+__global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y, uint32_t M,
                                                    const uint32_t* __restrict__ tmap,
                                                    const double2* __restrict__ ref,
-                                                   uint32_t count, const double* __restrict__ E,
+                                                   uint32_t count, uint32_t ne,
+                                                   const double* __restrict__ E,
                                                    uint32_t nchunks, double* __restrict__ err) {
   __shared__ double red[2][kBlock / 64];
   __shared__ double s_total;
@@ -1302,12 +1313,14 @@ __global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y
   }
   __syncthreads();
   const bool weighted = energy_mode(s_total);
+  // n_u T / M, the even draws' share of the weight
+  const double even = (double)(count - ne) * (s_total / (double)M);
   double num = 0.0, den = 0.0;
   for (uint32_t i = threadIdx.x; i < count; i += kBlock) {
     const double2 a = y[tmap[i]], b = ref[i];
     const double dr = a.x - b.x, di = a.y - b.y;
     if (weighted) {
-      num += (dr * dr + di * di) / energy(a);
+      num += (dr * dr + di * di) / ((double)ne * energy(a) + even);
     } else {
       num += dr * dr + di * di;
       den += energy(b);
@@ -1326,7 +1339,7 @@ __global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y
       n += red[0][w];
       d += red[1][w];
     }
-    if (weighted) err[0] = __builtin_sqrt(n / (double)count);
+    if (weighted) err[0] = __builtin_sqrt(n);
     else err[0] = d > 0.0 ? __builtin_sqrt(n / d) : (n > 0.0 ? INFINITY : 0.0);
   }
 }
@@ -1334,7 +1347,8 @@ __global__ __launch_bounds__(kBlock) void k_rel_l2(const double2* __restrict__ y
 hipError_t launch_rel_l2(const double2* y, size_t M, const uint32_t* tmap, const double2* ref,
                          size_t count, const double* E, double* err, hipStream_t stream) {
   const uint32_t L = error_chunk_len(M);
-  k_rel_l2<<<1, kBlock, 0, stream>>>(y, tmap, ref, (uint32_t)count, E,
+  k_rel_l2<<<1, kBlock, 0, stream>>>(y, (uint32_t)M, tmap, ref, (uint32_t)count,
+                                     error_energy_draws(M, count), E,
                                      (uint32_t)((M + L - 1) / L), err);
   return hipGetLastError();
 }

@@ -28,14 +28,18 @@
  *  - One context per host thread; calls on one context are not reentrant.
  *  - Error report (err_rel_l2, nullable, of every call that returns a field
  *    or gradients): for HOLO_FP32_SPLIT the call also evaluates the FP64
- *    formulation at count = min(M, 256) stride targets m_i = floor(i M /
- *    count) and writes sqrt((M / count) sum_i |r[m_i] - r64_i|^2 /
- *    sum_m |r[m]|^2) for its result r of M values: the relative L2 error of
- *    the whole result, its squared error estimated from the sampled targets
- *    and its norm taken over every target (exact when M <= 256, up to the
- *    FP32-split error of that norm); for HOLO_FP64 it writes 0 (the result is
- *    the FP64 formulation).  The extra work is 256 N FP64 pairs (256 K
- *    samples for Monte Carlo) and one pass over r.  err_rel_l2 is a host
+ *    formulation r64 at count = min(M, 256) targets of its result r (M
+ *    values) and writes an estimate of the relative L2 error of the whole
+ *    result, sqrt(sum_m |r - r64|^2 / T) with T = sum_m |r|^2.  n_e =
+ *    count / 2 targets are drawn by energy (draw i is the first target at
+ *    which the running sum of |r|^2 exceeds (i + 1/2) / n_e of T), n_u =
+ *    count - n_e are m_j = floor(j M / n_u), and the estimate is
+ *    sqrt(sum_i |r[m_i] - r64_i|^2 / (n_e |r[m_i]|^2 + n_u T / M)); with
+ *    M <= 256 every target is taken once (n_e = 0) and the value is exact.
+ *    A result without a finite positive T reports the relative L2 error at
+ *    the targets floor(i M / count).  For HOLO_FP64 it writes 0 (the result
+ *    is the FP64 formulation).  The extra work is 256 N FP64 pairs (256 K
+ *    samples for Monte Carlo) and two passes over r.  err_rel_l2 is a host
  *    pointer, or a device pointer with HOLO_DEVICE_POINTERS (the call then
  *    stays asynchronous).
  */

@@ -1,23 +1,22 @@
 """The error report of every result (err_rel_l2 of holo.h).
 
 An FP32-split call also evaluates the FP64 formulation at count = min(M, 256)
-targets drawn by the energy of its result r: m_i is the first target at which
-the running sum of |r|^2 exceeds (i + 1/2) / count of the total, and the
-report is sqrt(mean_i |r[m_i] - r64_i|^2 / |r[m_i]|^2), an estimate of the
-relative L2 error of the whole result (its expectation under a random offset
-of the strata).  A result without positive finite energy falls back to the
-stride targets floor(i M / count) and their relative L2 error.  Each test
-replays the choice on the host, evaluates the oracle at the chosen targets
-and requires the two reports to agree to 1e-3 of the value (the library's
-FP64 values are within 1e-12 of the oracle's); FP64 calls report 0.  Host
-and device pointers both carry it.  A focused field, and a field whose energy
-lies away from any stride subset, check that the report follows the whole
-result's error.
+targets, half drawn by the energy of its result r and half spread evenly, and
+reports the importance-weighted estimate of the whole result's relative L2
+error (the host replica and its formula are in _report.py; every target once,
+so the exact value, when M <= 256).  Each test replays the choice on the
+host, evaluates the oracle at the chosen targets and requires the two reports
+to agree to 1e-3 of the value (the library's FP64 values are within 1e-12 of
+the oracle's); FP64 calls report 0.  Host and device pointers both carry it.
+A focused field whose energy lies away from any stride subset checks that the
+report follows the whole result's error.
 """
 import sys
 
 import numpy as np
 import pytest
+
+from _report import pick, report_of, tmap
 
 pytestmark = pytest.mark.gpu
 LAM = 632.8e-9
@@ -32,36 +31,6 @@ def _announce(request):
 
 def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
-
-
-def tmap(M, count=256):
-    count = min(M, count)
-    return (np.arange(count, dtype=np.int64) * M) // count
-
-
-def pick(r, count=256):
-    """The report's targets for result r (all M values), as the library picks them."""
-    r = np.asarray(r)
-    M = len(r)
-    count = min(M, count)
-    e = np.abs(r) ** 2
-    C = np.cumsum(e)
-    total = C[-1]
-    if not (total > 0 and np.isfinite(total)):
-        return tmap(M, count), False
-    c = (np.arange(count) + 0.5) / count * total
-    m = np.searchsorted(C, c, side="right")
-    last = int(np.flatnonzero(e > 0)[-1])
-    return np.where(m < M, m, last).astype(np.int64), True
-
-
-def report_of(r, sub, ref, weighted=True):
-    """The report's formula on the host: r (all M values), ref = r64 at sub."""
-    r = np.asarray(r)
-    a, b = r[sub], np.asarray(ref)
-    if weighted:
-        return float(np.sqrt(np.mean(np.abs(a - b) ** 2 / np.abs(a) ** 2)))
-    return rel_l2(a, b)
 
 
 def agree(reported, host):
@@ -91,9 +60,9 @@ def test_transform_full_and_plan(pkg, ctx, oracle, workloads, geometry, M):
     v[:, 2] = 0.3
     # transform_full, host pointers
     y = ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP32_SPLIT, report_error=True)
-    sub, _ = pick(y)
-    ref = oracle.transform_full(-1, LAM, amp, phase, u, v, targets=sub)
-    agree(ctx.last_rel_l2, report_of(y, sub, ref))
+    pk = pick(y)
+    ref = oracle.transform_full(-1, LAM, amp, phase, u, v, targets=pk[0])
+    agree(ctx.last_rel_l2, report_of(y, pk, ref))
     # plan, device pointers (the report stays on the device)
     da, dp, du, dv = to_dev(amp, phase, u, v)
     plan = ctx.plan(du)
@@ -101,9 +70,9 @@ def test_transform_full_and_plan(pkg, ctx, oracle, workloads, geometry, M):
     e = ctx.last_rel_l2
     assert hasattr(e, "device") and e.device.type == "cuda"
     yd = yd.cpu().numpy()
-    subd, _ = pick(yd)
-    refd = oracle.transform_full(-1, LAM, amp, phase, u, v, targets=subd)
-    agree(float(e.item()), report_of(yd, subd, refd))
+    pkd = pick(yd)
+    refd = oracle.transform_full(-1, LAM, amp, phase, u, v, targets=pkd[0])
+    agree(float(e.item()), report_of(yd, pkd, refd))
     # FP64 reports 0
     ctx.transform_full(amp, phase, u, v, -1, LAM, pkg.FP64, report_error=True)
     assert ctx.last_rel_l2 == 0.0
@@ -118,9 +87,9 @@ def test_plan_complex_weights_and_obliquity(pkg, ctx, oracle, workloads):
     v = workloads.grid(40, 30, 7.56e-6, 0.25)
     plan = ctx.plan(u)
     y = plan.transform_complex(w, v, +1, LAM, pkg.FP32_SPLIT, obliquity=True, report_error=True)
-    sub, _ = pick(y)
-    ref = oracle.transform_cartesian(+1, LAM, w, u, v[sub], obliquity=True)
-    agree(ctx.last_rel_l2, report_of(y, sub, ref))
+    pk = pick(y)
+    ref = oracle.transform_cartesian(+1, LAM, w, u, v[pk[0]], obliquity=True)
+    agree(ctx.last_rel_l2, report_of(y, pk, ref))
 
 
 @pytest.mark.parametrize("shared", [False, True])
@@ -138,9 +107,9 @@ def test_monte_carlo(pkg, ctx, oracle, workloads, shared, lattice):
     K = 333
     y = ctx.transform_mc(amp, phase, u, v, K, 9, +1, LAM, pkg.FP32_SPLIT, shared=shared,
                          report_error=True)
-    sub, _ = pick(y)
-    ref = oracle.transform_mc(+1, LAM, amp, phase, u, v, K, 9, targets=sub, shared=shared)
-    agree(ctx.last_rel_l2, report_of(y, sub, ref))
+    pk = pick(y)
+    ref = oracle.transform_mc(+1, LAM, amp, phase, u, v, K, 9, targets=pk[0], shared=shared)
+    agree(ctx.last_rel_l2, report_of(y, pk, ref))
     ctx.transform_mc(amp, phase, u, v, K, 9, +1, LAM, pkg.FP64, shared=shared,
                      report_error=True)
     assert ctx.last_rel_l2 == 0.0
@@ -157,17 +126,17 @@ def test_adjoint(pkg, ctx, oracle, workloads):
     plan_v = ctx.plan(v)
     ga, gt = plan_v.adjoint(gz, amp, theta, u, +1, LAM, pkg.FP32_SPLIT, report_error=True)
     b = np.exp(1j * theta) * (ga + 1j * gt / amp)
-    sub, _ = pick(b)
-    rb = oracle.transform_cartesian(-1, LAM, gz, v, u[sub])
-    agree(ctx.last_rel_l2, report_of(b, sub, rb))
+    pk = pick(b)
+    rb = oracle.transform_cartesian(-1, LAM, gz, v, u[pk[0]])
+    agree(ctx.last_rel_l2, report_of(b, pk, rb))
 
 
 def test_focused_field_report_follows_whole_result(pkg, ctx, oracle, workloads):
     """Sources phased to focus near target 1, every 8th target (a stride
     subset of 2048) 2 mm to the side in the dark: the report stays within a
     factor 2 of the whole result's relative L2 error against the oracle and
-    samples the bright targets, while the stride subset's own relative error
-    would be far higher."""
+    draws its energy samples from the bright targets, while the stride subset's
+    own relative error would be far higher."""
     u = workloads.grid(96, 64, 10e-6, 0.0)
     M = 2048
     v = np.zeros((M, 3))
@@ -181,9 +150,9 @@ def test_focused_field_report_follows_whole_result(pkg, ctx, oracle, workloads):
     y = ctx.transform_full(amp, phase, u, v, +1, LAM, pkg.FP32_SPLIT, report_error=True)
     rep = ctx.last_rel_l2
     full = oracle.transform_full(+1, LAM, amp, phase, u, v)
-    sub, weighted = pick(y)
-    assert weighted and np.mean(sub % 8 == 0) < 0.05
-    agree(rep, report_of(y, sub, full[sub]))
+    pk = pick(y)
+    assert pk[1] and np.mean(pk[0][:pk[2]] % 8 == 0) < 0.05
+    agree(rep, report_of(y, pk, full[pk[0]]))
     whole = rel_l2(y, full)
     assert 0.5 * whole <= rep <= 2.0 * whole, (rep, whole)
     stride = tmap(M)
