@@ -263,7 +263,7 @@ def main():
             log(f"[bench] plane {src_desc}: FP32-split, 1 warm-up + {R} runs")
             t32 = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam,
                                                       holo.FP32_SPLIT, out=y32), R)
-            # the same call with its error report (FP64 at 256 stride targets)
+            # the same call with its error report (FP64 at 256 sampled targets)
             t32r = time_events(lambda: plan3.transform(amp3, ph3, v3, p3.sign, p3.lam,
                                                        holo.FP32_SPLIT, out=y32,
                                                        report_error=True), R)

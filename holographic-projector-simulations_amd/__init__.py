@@ -212,8 +212,9 @@ class Context:
         self.device = device
         # error report of the last call made with report_error=True: the
         # relative L2 distance of an FP32-split result from the FP64
-        # formulation at 256 stride targets (0 for FP64); a float, or a
-        # one-element device tensor for calls on device tensors (no sync)
+        # formulation, estimated from 256 sampled targets (include/holo.h;
+        # 0 for FP64); a float, or a one-element device tensor for calls on
+        # device tensors (no sync)
         self.last_rel_l2 = None
 
     def close(self):
@@ -299,7 +300,7 @@ class Context:
                      obliquity: bool = False, report_error: bool = False):
         """Monte Carlo estimate with K draws per target (shared: per group of 1024).
         report_error: self.last_rel_l2 receives the distance from the FP64
-        estimator with the same indices at 256 stride targets."""
+        estimator with the same indices, estimated from 256 sampled targets."""
         extra = (MC_SHARED if shared else 0) | (OBLIQUITY if obliquity else 0)
         device = _is_torch(amp) and amp.is_cuda
         amp, phase, u, v = _prep([amp, phase, u, v], device)

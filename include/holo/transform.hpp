@@ -129,8 +129,8 @@ struct Settings {
   // light-field model (matlab/holofunctions.m:226-227); the CUDA path has none.
   bool obliquity = false;
   // Error report of the last transform (holo.h): relative L2 distance of an
-  // HOLO_FP32_SPLIT field from the FP64 formulation at 256 stride targets;
-  // 0 after an HOLO_FP64 transform.
+  // HOLO_FP32_SPLIT field from the FP64 formulation, estimated from 256
+  // sampled targets; 0 after an HOLO_FP64 transform.
   double last_rel_l2 = 0.0;
 };
 inline Settings& settings() {

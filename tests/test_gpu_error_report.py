@@ -95,7 +95,7 @@ def test_plan_complex_weights_and_obliquity(pkg, ctx, oracle, workloads):
 @pytest.mark.parametrize("shared", [False, True])
 @pytest.mark.parametrize("lattice", [False, True])
 def test_monte_carlo(pkg, ctx, oracle, workloads, shared, lattice):
-    """The FP64 estimator with the same indices at the stride targets."""
+    """The FP64 estimator with the same indices at the sampled targets."""
     g = np.random.default_rng(2)
     if lattice:
         u = workloads.grid(100, 80, 10e-6, 0.0)
