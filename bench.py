@@ -342,14 +342,14 @@ def main():
 
 def run_distance(holo, ctx, wl, T, R=3):
     """FP32-split against FP64 on the 1024^2 plane (10 um) -> 1024^2 targets
-    (7.56 um) as the target plane moves from 0.05 to 0.35 m, with the lattice
+    (7.56 um) as the target plane moves from 0.02 to 0.35 m, with the lattice
     kernel's near-field GEMM on (default) and off."""
     import torch
     p3 = wl.plane_1024(0).stages[0]
     amp, ph, u = T(p3.amp), T(p3.phase), T(p3.u)
     plan = holo.Plan(ctx, u)
     res = {}
-    for z in (0.05, 0.06, 0.075, 0.1, 0.15, 0.2, 0.25, 0.35):
+    for z in (0.02, 0.03, 0.04, 0.05, 0.06, 0.075, 0.1, 0.15, 0.2, 0.25, 0.35):
         vz = p3.v.copy()
         vz[:, 2] = z
         v = T(vz)

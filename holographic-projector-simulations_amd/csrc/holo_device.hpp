@@ -182,14 +182,43 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
 // sin/cos and weights.
 constexpr float kGuardCycles = 2e-7f;
 
-// POLAR_W: w holds (|w|, arg w in cycles) rather than (re, im).
+// POLAR_W: w holds (|w|, arg w in cycles) rather than (re, im).  Sources
+// come 8 at a time, their (wave-uniform) loads issued before any of them is
+// evaluated, so that the dependent loads through perm overlap; padding slots
+// (perm < 0) are skipped as before, in the same order.
 template <int SIGN, bool OBL, bool POLAR_W = false>
 __device__ __forceinline__ void exact_tile_pairs(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
     const float2* __restrict__ w, int T, double vx, double vy, double vz,
     double inv_lambda, double& ar, double& ai) {
   float fr = 0.f, fi = 0.f;
-  for (int j = 0; j < T; ++j) {
+  int j0 = 0;
+  for (; j0 + 8 <= T; j0 += 8) {
+    int32_t n[8];
+    float2 wj[8];
+    double4 p[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) n[h] = perm[j0 + h];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      wj[h] = w[j0 + h];
+      const size_t m = n[h] < 0 ? 0 : (size_t)n[h];  // a padding slot reads source 0, unused
+      p[h] = make_double4(u[3 * m], u[3 * m + 1], u[3 * m + 2], 0.0);
+    }
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      if (n[h] < 0) continue;
+      float2 ww = wj[h];
+      if constexpr (POLAR_W) {
+        float sn, cs;
+        sincos_rev_f32(ww.y, sn, cs);
+        ww = make_float2(ww.x * cs, ww.x * sn);
+      }
+      p[h].w = __builtin_bit_cast(double, ww);
+      mc_pair_f32s<SIGN, OBL>(p[h], vx, vy, vz, inv_lambda, fr, fi);
+    }
+  }
+  for (int j = j0; j < T; ++j) {
     const int32_t n = perm[j];
     if (n < 0) continue;
     float2 wj = w[j];
