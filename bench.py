@@ -153,11 +153,12 @@ def main():
                          "speed against FP64 from 0.05 to 0.35 m)")
     ap.add_argument("--hologram-steps", type=int, default=20,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
-    ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32), default=32,
+    ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32, 64), default=64,
                     help="tile columns of the FP32-split lattice kernel on uniform lattices "
-                         "(32: 16 x 32 tiles, options grid16 and grid32; 16: 16 x 16 tiles, "
-                         "grid32 = 0; 8: the 8 x 8 kernel, grid16 = 0; 0: the tiled kernel "
-                         "for arbitrary point sets, grid = 0)")
+                         "(64: 16 x 64 tiles where the tile columns form quads, else 16 x 32, "
+                         "options grid16, grid32 and grid64; 32: 16 x 32 tiles, grid64 = 0; "
+                         "16: 16 x 16 tiles, grid32 = 0; 8: the 8 x 8 kernel, grid16 = 0; "
+                         "0: the tiled kernel for arbitrary point sets, grid = 0)")
     ap.add_argument("--source-points", type=int, default=0,
                     help="fp32 and mc sections: this many sources from the reference's plane "
                          "generator (10.24 mm wide; 1105920 = 1024 x 1080, a 1051 x 1052 lattice "
@@ -192,8 +193,9 @@ def main():
     import torch
     import __graft_entry__
     holo = __graft_entry__.load_package()
-    holo.set_option("grid16", 1 if a.lattice_tiles in (16, 32) else 0)
-    holo.set_option("grid32", 1 if a.lattice_tiles == 32 else 0)
+    holo.set_option("grid16", 1 if a.lattice_tiles in (16, 32, 64) else 0)
+    holo.set_option("grid32", 1 if a.lattice_tiles in (32, 64) else 0)
+    holo.set_option("grid64", 1 if a.lattice_tiles == 64 else 0)
     holo.set_option("grid", 0 if a.lattice_tiles == 0 else 1)
     if a.target_waves:
         holo.set_option("target_waves", a.target_waves)
@@ -376,9 +378,9 @@ def run_distance(holo, ctx, wl, T, R=3):
 
 def run_lattices(holo, ctx, wl, T, R):
     """FP32-split on uniform lattices of 8, 24, 40 and 1024 rows (10 um pitch,
-    about 65k or 1M sources) -> 1024^2 targets at 0.35 m, with 16 x 32, 16 x 16
-    and 8 x 8 tiles: narrow lattices pad their rows to 16 (8 rows: half the
-    slots are padding)."""
+    about 65k or 1M sources) -> 1024^2 targets at 0.35 m, with 16 x 64, 16 x 32,
+    16 x 16 and 8 x 8 tiles: narrow lattices pad their rows to 16 (8 rows: half
+    the slots are padding)."""
     import torch
     res = {}
     v = T(wl.grid(1024, 1024, 7.56e-6, 0.35))
@@ -389,14 +391,17 @@ def run_lattices(holo, ctx, wl, T, R):
         plan = holo.Plan(ctx, uu)
         y = torch.empty((v.shape[0], 2), dtype=torch.float64, device=v.device)
         row = {}
-        for name, g16, g32 in (("16x32", 1, 1), ("16x16", 1, 0), ("8x8", 0, 1)):
+        for name, g16, g32, g64 in (("16x64", 1, 1, 1), ("16x32", 1, 1, 0), ("16x16", 1, 0, 0),
+                                    ("8x8", 0, 1, 1)):
             holo.set_option("grid16", g16)
             holo.set_option("grid32", g32)
+            holo.set_option("grid64", g64)
             t = time_events(lambda: plan.transform(amp, ph, v, +1, 632.8e-9, holo.FP32_SPLIT,
                                                    out=y), R)
             row[name] = {"pairs_per_s": len(u) * v.shape[0] / (t["median_ms"] / 1e3), "ms": t}
         holo.set_option("grid16", 1)
         holo.set_option("grid32", 1)
+        holo.set_option("grid64", 1)
         res[f"{nx}x{ny}"] = row
         log(f"[bench] lattice {nx}x{ny}: " + ", ".join(
             f"{k} {r['pairs_per_s']:.3g}" for k, r in row.items()))

@@ -127,6 +127,12 @@ struct holo_plan {
   double g32_cx0 = 0, g32_cx1 = 0, g32_cy0 = 0, g32_cy1 = 0;
   float g32_rho_y = 0;
   DevBuf g32_centre;
+  // the 16 x 64 tiles formed by quads of 16 x 16 tiles (4T .. 4T + 3) when the
+  // padded count of tile columns is a multiple of 4 (g64_tiles = 0 otherwise)
+  size_t g64_tiles = 0;
+  double g64_cx0 = 0, g64_cx1 = 0, g64_cy0 = 0, g64_cy1 = 0;
+  float g64_rho_y = 0;
+  DevBuf g64_centre;
   // Every call rewrites the weight buffers above.  `ran` is recorded after
   // each call on its stream; a call on another stream waits for it first.
   mutable hipEvent_t ran = nullptr;
@@ -514,6 +520,9 @@ struct HostGrid {
   double cx0_32 = INFINITY, cx1_32 = -INFINITY, cy0_32 = INFINITY, cy1_32 = -INFINITY;
   float rho32_y = 0;
   std::vector<double4> centre32;
+  double cx0_64 = INFINITY, cx1_64 = -INFINITY, cy0_64 = INFINITY, cy1_64 = -INFINITY;
+  float rho64_y = 0;
+  std::vector<double4> centre64;
   std::vector<int32_t> tail;  // sources after the lattice (padded lattices)
 };
 
@@ -565,6 +574,22 @@ void build_grid16(const double* u, HostGrid& g) {
       g.cx1_32 = std::max(g.cx1_32, cx);
       g.cy0_32 = std::min(g.cy0_32, cy);
       g.cy1_32 = std::max(g.cy1_32, cy);
+    }
+  // quads of tiles 4T .. 4T + 3: 16 x 64 tiles centred at
+  // x_0 + (16I + 7.5) p_x, y_0 + (64J' + 31.5) p_y (no extra padding for them)
+  if (TJ % 4) return;
+  g.rho64_y = std::nextafter((float)(31.5 * std::fabs(py)), INFINITY);
+  const float2 rho64{g.rho16_x, g.rho64_y};
+  g.centre64.reserve(TI * TJ / 4);
+  for (size_t I = 0; I < TI; ++I)
+    for (size_t J = 0; J < TJ / 4; ++J) {
+      const double cx = x0 + ((double)(16 * I) + 7.5) * px;
+      const double cy = y0 + ((double)(64 * J) + 31.5) * py;
+      g.centre64.push_back(double4{cx, cy, g.z, __builtin_bit_cast(double, rho64)});
+      g.cx0_64 = std::min(g.cx0_64, cx);
+      g.cx1_64 = std::max(g.cx1_64, cx);
+      g.cy0_64 = std::min(g.cy0_64, cy);
+      g.cy1_64 = std::max(g.cy1_64, cy);
     }
 }
 
@@ -751,6 +776,18 @@ holo_status upload_grid16(holo_ctx* ctx, holo_plan* p, const HostGrid& grid) {
     HOLO_HIP(hipMemcpyAsync(p->g32_centre.ptr, grid.centre32.data(), n32 * sizeof(double4),
                             hipMemcpyHostToDevice, ctx->stream));
   }
+  const size_t n64 = grid.centre64.size();
+  if (n64) {
+    p->g64_tiles = n64;
+    p->g64_cx0 = grid.cx0_64;
+    p->g64_cx1 = grid.cx1_64;
+    p->g64_cy0 = grid.cy0_64;
+    p->g64_cy1 = grid.cy1_64;
+    p->g64_rho_y = grid.rho64_y;
+    HOLO_TRY(ensure(p->g64_centre, n64 * sizeof(double4), ctx->device));
+    HOLO_HIP(hipMemcpyAsync(p->g64_centre.ptr, grid.centre64.data(), n64 * sizeof(double4),
+                            hipMemcpyHostToDevice, ctx->stream));
+  }
   return HOLO_OK;
 }
 
@@ -873,11 +910,13 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
   } else if (p->g16_tiles && holo::tuning().grid && holo::tuning().grid16) {
     const size_t nt = p->g16_tiles;
     const bool pairs = p->g32_tiles && holo::tuning().grid32;
+    const bool quads = pairs && p->g64_tiles && holo::tuning().grid64;
+    const int nkc = quads ? 4 : (pairs ? 2 : 1);
     HOLO_HIP(holo::launch_tile_weights(wf, w0, w1, static_cast<const int32_t*>(p->g16_perm.ptr),
                                        256 * nt + p->g16_tail,
                                        static_cast<float2*>(p->g16_wc.ptr), ctx->stream));
     HOLO_HIP(holo::launch_grid16_weights(static_cast<const float2*>(p->g16_wc.ptr), (uint32_t)nt,
-                                         pairs, static_cast<uint4*>(p->g16_wq.ptr),
+                                         nkc, static_cast<uint4*>(p->g16_wq.ptr),
                                          static_cast<uint4*>(p->g16_wqn.ptr),
                                          static_cast<float*>(p->g16_ts.ptr), ctx->stream));
     holo::GridTiles g16;  // the 16 x 16 tiles
@@ -897,20 +936,30 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     g16.tail_perm = g16.perm + 256 * nt;
     g16.tail_wc = static_cast<const float2*>(p->g16_wc.ptr) + 256 * nt;
     g16.n_tail = (uint32_t)p->g16_tail;
-    holo::GridTiles gt = g16;  // the tiles of the launch: pairs of them, or themselves
+    holo::GridTiles g32 = g16;  // the pairs of them
     if (pairs) {
-      gt.centre = static_cast<const double4*>(p->g32_centre.ptr);
-      gt.n_tiles = (uint32_t)p->g32_tiles;
-      gt.cx0 = p->g32_cx0;
-      gt.cx1 = p->g32_cx1;
-      gt.cy0 = p->g32_cy0;
-      gt.cy1 = p->g32_cy1;
-      gt.rho_y = p->g32_rho_y;
+      g32.centre = static_cast<const double4*>(p->g32_centre.ptr);
+      g32.n_tiles = (uint32_t)p->g32_tiles;
+      g32.cx0 = p->g32_cx0;
+      g32.cx1 = p->g32_cx1;
+      g32.cy0 = p->g32_cy0;
+      g32.cy1 = p->g32_cy1;
+      g32.rho_y = p->g32_rho_y;
+    }
+    holo::GridTiles gt = pairs ? g32 : g16;  // the tiles of the launch
+    if (quads) {
+      gt.centre = static_cast<const double4*>(p->g64_centre.ptr);
+      gt.n_tiles = (uint32_t)p->g64_tiles;
+      gt.cx0 = p->g64_cx0;
+      gt.cx1 = p->g64_cx1;
+      gt.cy0 = p->g64_cy0;
+      gt.cy1 = p->g64_cy1;
+      gt.rho_y = p->g64_rho_y;
     }
     const int S = holo::grid_splits(M, gt.n_tiles);
     const size_t pe = S > 1 ? (size_t)S * M : 0;
     if (pe) HOLO_TRY(ensure(ctx->partial, pe * sizeof(double2), ctx->device));
-    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, pairs ? 2 : 1, gt, g16,
+    HOLO_HIP(holo::launch_superpose_grid16(sign, obliquity, nkc, gt, g32, g16,
                                            static_cast<const float2*>(p->g16_wc.ptr),
                                            static_cast<const uint4*>(p->g16_wq.ptr),
                                            static_cast<const uint4*>(p->g16_wqn.ptr),
@@ -1121,6 +1170,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid16 = value != 0;
   } else if (n == "grid32") {
     t.grid32 = value != 0;
+  } else if (n == "grid64") {
+    t.grid64 = value != 0;
   } else if (n == "grid_series") {
     t.grid_series = value != 0;
   } else if (n == "grid_near") {
@@ -1150,6 +1201,7 @@ long holo_get_option(const char* name) {
   if (n == "grid") return t.grid ? 1 : 0;
   if (n == "grid16") return t.grid16 ? 1 : 0;
   if (n == "grid32") return t.grid32 ? 1 : 0;
+  if (n == "grid64") return t.grid64 ? 1 : 0;
   if (n == "grid_series") return t.grid_series ? 1 : 0;
   if (n == "grid_near") return t.grid_near ? 1 : 0;
   if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;

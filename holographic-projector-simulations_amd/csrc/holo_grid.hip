@@ -370,26 +370,28 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
   return launch_reduce_partials(S, M, partial, y, stream);
 }
 
-// ---- 16 x 16 and 16 x 32 tiles on uniform lattices ------------------------
+// ---- 16 x 16, 16 x 32 and 16 x 64 tiles on uniform lattices --------------
 //
 // On a lattice with constant pitches (p_x, p_y) the sources of a tile of 16
-// rows (along x) and 16 NKC columns (along y, NKC = 1 or 2) sit at
+// rows (along x) and 16 NKC columns (along y, NKC = 1, 2 or 4) sit at
 // gx_a = p_x k_a, gy_b = p_y k_b from its centre, k_a = a - 7.5 (a < 16),
 // k_b = b - (8 NKC - 1/2) (b < 16 NKC).  A 16 x 32 tile is the pair of 16 x 16
-// tiles 2T, 2T + 1 of the same rows, so both kernels share the slots (16 a + b
-// of each half), the weights' layout and the exact path's source order.
+// tiles 2T, 2T + 1 of the same rows, and a 16 x 64 tile the quad 4T .. 4T + 3,
+// so all share the slots (16 a + b of each 16 x 16 tile), the weights' layout
+// and the exact path's source order.
 // In units of the pitch, P = p_x^2 Pt with Pt = k_a^2 + qx k_a,
 // qx = -2 r_x / p_x (likewise Q, Qt, qy), and the row phase is
 //   alpha P + c2 P^2 = sx k_a + ax k_a^2 + cx Pt^2,
 //   sx = -2 alpha r_x p_x (cycles per pitch), ax = alpha p_x^2, cx = c2 p_x^4.
 // The linear part spans several cycles over a tile, so it is evaluated
 // without rounding: sx = sh + sl with sh keeping 20 significant bits (19 for
-// 32 columns), which makes sh k exact in FP32 (2k is an odd integer below 16,
-// or 32); its fraction is exact too (v_fract_f32), and only then are sl k, the
+// 32 columns, 18 for 64), which makes sh k exact in FP32 (2k is an odd integer
+// below 16, 32 or 64); its fraction is exact too (v_fract_f32), and only then are sl k, the
 // quadratic terms and f added, so the FP32 argument of sin/cos stays below 1.5
 // cycles however large the tile's phase span.  Per tile and target that leaves
 // the same rounding as the 8 x 8 kernel with a quarter of the per-target
-// constants and half the phasors per pair (16 x 16), or 3/8 (16 x 32).
+// constants and half the phasors per pair (16 x 16), 3/8 (16 x 32) or 5/16
+// (16 x 64).
 //
 // The cross term 2 c2 P Q: its part linear in k_a goes through the GEMM (the
 // correction below); of its k_a^2 part, 2 c2 p_x^2 k_a^2 Q, the share of the
@@ -531,7 +533,7 @@ __device__ __forceinline__ void series_tile_pairs(const float2* __restrict__ w, 
 #pragma clang fp contract(off)
   const double al = 0.5 * id * inv_lambda;
   const double sx = -2.0 * al * rx * px, sy = -2.0 * al * ry * py;
-  const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
+  const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : (NKC == 2 ? 19 : 18)>(sy);
   const float sxl = (float)(sx - (double)sxh), syl = (float)(sy - (double)syh);
   const float ax = (float)(al * px * px), ay = (float)(al * py * py);
   const float mx = (float)(-2.0 * rx * px), my = (float)(-2.0 * ry * py);
@@ -582,10 +584,11 @@ __device__ __forceinline__ void series_tile_pairs(const float2* __restrict__ w, 
 // near-field GEMM's extra operands: wqn[(4t + 2rb + q) * 64 + lane] holds the
 // lo part of i k_a kCrossW W / tscale (q = 0) and k_a^2 kCross2W W / tscale in
 // the real form of W (q = 1).  wc: 256 slots per tile, slot 16 a + b.
-// pair: tiles 2T and 2T + 1 (the two halves of a 16 x 32 tile) share one
-// tscale, the power of two above the larger of their largest components.
+// group (1, 2 or 4): tiles group T .. group T + group - 1 (the halves of a
+// 16 x 32 tile, the quarters of a 16 x 64 one) share one tscale, the power of
+// two above the largest of their largest components.
 __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restrict__ wc,
-                                                           uint32_t n_tiles, uint32_t pair,
+                                                           uint32_t n_tiles, uint32_t group,
                                                            uint4* __restrict__ wq,
                                                            uint4* __restrict__ wqn,
                                                            float* __restrict__ tscale) {
@@ -602,12 +605,13 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
       w[rb][q] = wc[(size_t)t * 256 + 16 * (8 * rb + ap) + 4 * g + q];
       mx = fmaxf(mx, fmaxf(fabsf(w[rb][q].x), fabsf(w[rb][q].y)));
     }
-  if (pair) {  // n_tiles is even: the partner t ^ 1 exists
+  for (uint32_t o = 1; o < group; ++o) {  // n_tiles is a multiple of group
+    const uint32_t tt = (t & ~(group - 1u)) | ((t + o) & (group - 1u));
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float2 x = wc[(size_t)(t ^ 1u) * 256 + 16 * (8 * rb + ap) + 4 * g + q];
+        const float2 x = wc[(size_t)tt * 256 + 16 * (8 * rb + ap) + 4 * g + q];
         mx = fmaxf(mx, fmaxf(fabsf(x.x), fabsf(x.y)));
       }
   }
@@ -656,12 +660,14 @@ __global__ __launch_bounds__(kBlock) void k_grid16_weights(const float2* __restr
   if (lane == 0) tscale[t] = ldexpf(1.f, e);
 }
 
-hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, uint4* wq,
+hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, int group, uint4* wq,
                                  uint4* wqn, float* tscale, hipStream_t stream) {
   if (!n_tiles) return hipSuccess;
-  if (pair && (n_tiles & 1u)) return hipErrorInvalidValue;
+  if ((group != 1 && group != 2 && group != 4) || n_tiles % (uint32_t)group)
+    return hipErrorInvalidValue;
   const unsigned blocks = (n_tiles + kGridWaves - 1) / kGridWaves;
-  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, pair ? 1u : 0u, wq, wqn, tscale);
+  k_grid16_weights<<<blocks, kBlock, 0, stream>>>(wc, n_tiles, (uint32_t)group, wq, wqn,
+                                                  tscale);
   return hipGetLastError();
 }
 
@@ -720,7 +726,7 @@ __device__ __forceinline__ void grid16_tiles(
   // every fused operation is written out, so that the same tiles give the same
   // bits whichever kernel this is inlined into
 #pragma clang fp contract(off)
-  static_assert(NKC == 1 || NKC == 2, "16 or 32 columns");
+  static_assert(NKC == 1 || NKC == 2 || NKC == 4, "16, 32 or 64 columns");
   static_assert(!NEAR || NKC == 1, "the near-field GEMM takes 16 x 16 tiles");
   const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const uint32_t g = lane >> 4, j = lane & 15;
@@ -863,6 +869,90 @@ __device__ __forceinline__ void grid16_tiles(
       }
     }
   };
+  // the far-field GEMM of 16 x 64 tiles, with the column chunks outermost: one
+  // chunk's weight operands live at a time (24 VGPRs; all four chunks' would
+  // take 96), the four blocks' accumulators across chunks, and each block's
+  // constants read from LDS where they are used (the empty asm with a memory
+  // clobber keeps the compiler from holding all four blocks' constants in
+  // registers).  The same operations in the same order per block as tile_gemm.
+  const auto tile_gemm_far = [&](uint32_t t) __attribute__((always_inline)) {
+    f4v D[4][2];
+#pragma unroll
+    for (int B = 0; B < 4; ++B)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) D[B][h] = f4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < NKC; ++c) {
+      h8v W[6];
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        W[q] = __builtin_bit_cast(h8v, wq[((size_t)t * 6 * NKC + 6 * c + q) * 64 + lane]);
+#pragma unroll
+      for (int B = 0; B < 4; ++B) {
+        __asm__ volatile("" ::: "memory");
+        const float4 k0 = lds.k[0][wv][16 * B + j];
+        const float4 k1 = lds.k[1][wv][16 * B + j];
+        const float4 k2 = lds.k[2][wv][16 * B + j];
+        const float4 k3 = lds.k[3][wv][16 * B + j];
+        h8v yh, yl, yq;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float k = kc[c][q], kk = k * k;
+          const float Qt = __builtin_fmaf(k2.y, k, kk);
+          const float r = __builtin_amdgcn_fractf(k0.z * k);  // exact
+          float tq = __builtin_fmaf(k0.w, k, k1.y * kk);
+          tq = __builtin_fmaf(k1.w * Qt, Qt, tq);
+          float sn, cs;
+          sincos_rev_f32(r + tq, sn, cs);
+          if (SIGN < 0) sn = -sn;
+          const float am = __builtin_fmaf(k2.w, Qt, 1.f);  // 1 + kappa Q
+          const float qk = Qt * k3.z;
+          h2v hi, lo, cq;
+          y_operand_far(cs, sn, am, qk, hi, lo, cq);
+          yh[2 * q] = hi[0];
+          yh[2 * q + 1] = hi[1];
+          yl[2 * q] = lo[0];
+          yl[2 * q + 1] = lo[1];
+          yq[2 * q] = cq[0];
+          yq[2 * q + 1] = cq[1];
+        }
+        D[B][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yh, D[B][0], 0, 0, 0);
+        D[B][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yh, D[B][1], 0, 0, 0);
+        D[B][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[0], yl, D[B][0], 0, 0, 0);
+        D[B][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[3], yl, D[B][1], 0, 0, 0);
+        D[B][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[1], yh, D[B][0], 0, 0, 0);
+        D[B][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[4], yh, D[B][1], 0, 0, 0);
+        D[B][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[2], yq, D[B][0], 0, 0, 0);
+        D[B][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(W[5], yq, D[B][1], 0, 0, 0);
+      }
+    }
+    const float ts = tscale[(size_t)t * NKC];
+#pragma unroll
+    for (int B = 0; B < 4; ++B) {
+      __asm__ volatile("" ::: "memory");
+      const float4 k0 = lds.k[0][wv][16 * B + j];
+      const float4 k1 = lds.k[1][wv][16 * B + j];
+      const float4 k2 = lds.k[2][wv][16 * B + j];
+      const float4 k3 = lds.k[3][wv][16 * B + j];
+      const float at = k3.y * ts, akt = at * k2.z;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float k = kr[q], kk = k * k;
+        const float Pt = __builtin_fmaf(k2.x, k, kk);
+        const float r = __builtin_amdgcn_fractf(k0.x * k);  // exact
+        float tp = __builtin_fmaf(k0.y, k, __builtin_fmaf(k1.x, kk, k3.x));
+        tp = __builtin_fmaf(k1.z * Pt, Pt, tp);
+        float sn, cs;
+        sincos_rev_f32(r + tp, sn, cs);
+        const float am = __builtin_fmaf(akt, Pt, at);  // tscale A (1 + kappa P)
+        const float xr = am * cs, xi = SIGN > 0 ? am * sn : -(am * sn);
+        const f4v& Dq = (q >> 1) ? D[B][1] : D[B][0];
+        const float tr = Dq[2 * (q & 1)], ti = Dq[2 * (q & 1) + 1];
+        fr[B] = __builtin_fmaf(xr, tr, __builtin_fmaf(-xi, ti, fr[B]));
+        fi[B] = __builtin_fmaf(xr, ti, __builtin_fmaf(xi, tr, fi[B]));
+      }
+    }
+  };
   for (uint32_t t = t0; t < t1; ++t) {
     // near_mode (wave-uniform, NEAR only): some target of the wave fails the
     // far-field bounds of this tile, so the wave takes the near-field GEMM for it
@@ -893,7 +983,7 @@ __device__ __forceinline__ void grid16_tiles(
       // exactly); the small coefficients are FP32 products
       const double aly = al * __builtin_fma(near_mode ? 0.0 : mid, id * id, 1.0);
       const double sx = -2.0 * al * rx * px, sy = -2.0 * aly * ry * py;
-      const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : 19>(sy);
+      const float sxh = hi_bits<20>(sx), syh = hi_bits<NKC == 1 ? 20 : (NKC == 2 ? 19 : 18)>(sy);
       float4 k0 = make_float4(sxh, (float)(sx - (double)sxh), syh, (float)(sy - (double)syh));
       float4 k1 = make_float4(alf * px2f, (float)aly * py2f, c2f * px4f, c2f * py4f);
       float4 k2 = make_float4(-2.f * rxf * ipxf, -2.f * ryf * ipyf, kf * px2f, kf * py2f);
@@ -941,6 +1031,8 @@ __device__ __forceinline__ void grid16_tiles(
         tile_gemm(t, std::true_type{});
       else
         tile_gemm(t, std::false_type{});
+    } else if constexpr (NKC == 4) {
+      tile_gemm_far(t);
     } else {
       tile_gemm(t, std::false_type{});
     }
@@ -965,9 +1057,12 @@ __device__ __forceinline__ void grid16_tiles(
 // and gb.  NKC = 2: the 16 x 32 tiles of centre and gb when every target of
 // the wave passes their far-field bounds; otherwise the wave takes the same
 // sources as the 16 x 16 tiles 2 t0 .. 2 t1 - 1 (centre16, gb16), whose
-// looser bounds send fewer (target, tile) pairs to the exact path.  The
-// n_tail sources after a padded lattice (tail_perm, tail_wc) are summed
-// exactly by every target in the first source split.
+// looser bounds send fewer (target, tile) pairs to the exact path.  NKC = 4:
+// the 16 x 64 tiles of centre and gb likewise, else the 16 x 32 tiles
+// 2 t0 .. 2 t1 - 1 (centre32, gb32) when the wave passes their bounds, else
+// the 16 x 16 tiles 4 t0 .. 4 t1 - 1.  The n_tail sources after a padded
+// lattice (tail_perm, tail_wc) are summed exactly by every target in the
+// first source split.
 template <int SIGN, bool OBL, int NKC>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_superpose_grid16(
     const double4* __restrict__ centre, const int32_t* __restrict__ perm,
@@ -975,7 +1070,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const uint4* __restrict__ wq, const uint4* __restrict__ wqn,
     const float* __restrict__ tscale, uint32_t n_tiles,
     uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
-    double py, GridBounds gb, const double4* __restrict__ centre16, GridBounds gb16,
+    double py, GridBounds gb, const double4* __restrict__ centre32, GridBounds gb32,
+    const double4* __restrict__ centre16, GridBounds gb16,
     const int32_t* __restrict__ tail_perm, const float2* __restrict__ tail_wc, uint32_t n_tail,
     double2* __restrict__ out) {
   __shared__ Grid16Lds lds;
@@ -994,9 +1090,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
   const float rx2c = kRowDev * (float)(px * px);
   double ex_r = 0.0, ex_i = 0.0;
   const bool far = __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb, rx2c));
+  bool far32 = false;
+  if constexpr (NKC == 4)
+    far32 = !far && __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb32, rx2c));
   if (far) {
     grid16_tiles<SIGN, OBL, NKC>(lds, centre, perm, u, wc, wq, wqn, tscale, t0, t1, vx, vy, vz,
                                  inv_lambda, px, py, gb, true, ex_r, ex_i);
+  } else if (far32) {
+    grid16_tiles<SIGN, OBL, 2>(lds, centre32, perm, u, wc, wq, wqn, tscale, 2 * t0, 2 * t1, vx,
+                               vy, vz, inv_lambda, px, py, gb32, true, ex_r, ex_i);
   } else {
     // 16 x 16 tiles: those of the launch (NKC = 1), or the halves of its 16 x 32
     // tiles, whose looser bounds keep more pairs on the far-field GEMM
@@ -1029,13 +1131,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 
 hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
-                                   const GridTiles& halves, const float2* wc, const uint4* wq,
-                                   const uint4* wqn, const float* tscale, const double* v,
-                                   size_t M, double lambda, double2* y, double2* partial,
+                                   const GridTiles& pairs, const GridTiles& halves,
+                                   const float2* wc, const uint4* wq, const uint4* wqn,
+                                   const float* tscale, const double* v, size_t M,
+                                   double lambda, double2* y, double2* partial,
                                    size_t partial_cap, hipStream_t stream) {
   if (M == 0) return hipSuccess;
-  if (nkc != 1 && nkc != 2) return hipErrorInvalidValue;
-  if (nkc == 2 && (halves.n_tiles != 2 * gt.n_tiles || !halves.centre))
+  if (nkc != 1 && nkc != 2 && nkc != 4) return hipErrorInvalidValue;
+  if (nkc > 1 && (halves.n_tiles != (uint32_t)nkc * gt.n_tiles || !halves.centre))
+    return hipErrorInvalidValue;
+  if (nkc == 4 && (pairs.n_tiles != 2 * gt.n_tiles || !pairs.centre))
     return hipErrorInvalidValue;
   if (gt.n_tail && (!gt.tail_perm || !gt.tail_wc)) return hipErrorInvalidValue;
   int S = grid_splits(M, gt.n_tiles);
@@ -1050,16 +1155,25 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
                       gt.rho_x, gt.rho_y, series, near_gemm};
   const GridBounds gb16{halves.cx0,   halves.cx1,   halves.cy0, halves.cy1, halves.z,
                         halves.rho_x, halves.rho_y, series,     near_gemm};
+  const GridBounds gb32{pairs.cx0,   pairs.cx1,   pairs.cy0, pairs.cy1, pairs.z,
+                        pairs.rho_x, pairs.rho_y, series,    near_gemm};
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
-      if (nkc == 2)
+      if (nkc == 4)
+        k_superpose_grid16<sg.value, (bool)obl.value, 4><<<grid, kBlock, 0, stream>>>(
+            gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
+            il, gt.px, gt.py, gb, pairs.centre, gb32, halves.centre, gb16, gt.tail_perm,
+            gt.tail_wc, gt.n_tail, out);
+      else if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
+            il, gt.px, gt.py, gb, gt.centre, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc,
+            gt.n_tail, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc, gt.n_tail, out);
+            il, gt.px, gt.py, gb, gt.centre, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc,
+            gt.n_tail, out);
     });
   });
   hipError_t e = hipGetLastError();

@@ -61,6 +61,7 @@ struct Tuning {
   bool grid = true;       // FP32-split: lattice kernel for product-grid sources
   bool grid16 = true;     // lattice kernel: 16 x 16 tiles on uniform lattices
   bool grid32 = true;     // ... as 16 x 32 tiles where the columns pair up
+  bool grid64 = true;     // ... as 16 x 64 tiles where they form quads
   bool grid_series = true; // 16-row kernels: near-field pairs by the q-series, not exactly
   bool grid_near = true;   // 16-row kernels: waves off the far-field bounds take the near-field GEMM
   bool mc_lattice = true; // per-target FP32-split MC: table gather on product grids
@@ -165,22 +166,27 @@ hipError_t launch_superpose_grid(int sign, bool obliquity, const GridTiles& gt,
 // 16 x 16 tiles of a uniform lattice (pitches gt.px, gt.py; gt.goff unused):
 // wc holds 256 slots per tile (slot 16a + b), launch_grid16_weights lays them
 // out as wq (6 x 8 f16 per tile and lane), wqn (the near-field GEMM's 4 x 8 f16
-// per tile and lane; nullable) and tscale; with pair, tiles 2T and 2T + 1 share
-// one tscale.  launch_superpose_grid16 with nkc = 1 takes the
-// 16 x 16 tiles gt (halves unused); with nkc = 2 it takes the 16 x 32 tiles
-// formed by the pairs (gt.n_tiles pairs, their centres in gt.centre, rho_y and
-// the bounds those of the pairs), reading the same wc, wq, perm and tscale
-// (from launch_grid16_weights with pair), and halves describes the same
-// sources as 16 x 16 tiles (2 gt.n_tiles of them): a wave whose targets fail
-// the 16 x 32 far-field bounds takes those.
-hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, bool pair, uint4* wq,
+// per tile and lane; nullable) and tscale; tiles group T .. group T + group - 1
+// share one tscale (group 1, 2 or 4).  launch_superpose_grid16 with nkc = 1
+// takes the 16 x 16 tiles gt (pairs and halves unused); with nkc = 2 it takes
+// the 16 x 32 tiles formed by pairs of 16 x 16 tiles (gt.n_tiles of them, their
+// centres in gt.centre, rho_y and the bounds those of the pairs), reading the
+// same wc, wq, perm and tscale (from launch_grid16_weights with group 2), and
+// halves describes the same sources as 16 x 16 tiles (2 gt.n_tiles of them): a
+// wave whose targets fail the 16 x 32 far-field bounds takes those.  With
+// nkc = 4, gt holds the 16 x 64 tiles formed by quads (group 4), pairs the
+// same sources as 16 x 32 tiles (2 gt.n_tiles) and halves as 16 x 16 tiles
+// (4 gt.n_tiles): a wave failing the 16 x 64 bounds takes the 16 x 32 tiles if
+// it passes theirs, else the 16 x 16 ones.
+hipError_t launch_grid16_weights(const float2* wc, uint32_t n_tiles, int group, uint4* wq,
                                  uint4* wqn, float* tscale, hipStream_t stream);
 // wqn null: no near-field GEMM (the waves off the far-field bounds take the
 // series and exact paths per (target, tile))
 hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const GridTiles& gt,
-                                   const GridTiles& halves, const float2* wc, const uint4* wq,
-                                   const uint4* wqn, const float* tscale, const double* v,
-                                   size_t M, double lambda, double2* y, double2* partial,
+                                   const GridTiles& pairs, const GridTiles& halves,
+                                   const float2* wc, const uint4* wq, const uint4* wqn,
+                                   const float* tscale, const double* v, size_t M,
+                                   double lambda, double2* y, double2* partial,
                                    size_t partial_cap, hipStream_t stream);
 // Source splits the lattice kernel uses for M targets over n_tiles tiles.
 int grid_splits(size_t M, size_t n_tiles);

@@ -116,7 +116,8 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * lattice kernel; default 1), "grid16" (0/1: on a uniform lattice that
  * kernel uses 16 x 16 tiles, padded to multiples of 16; default 1), "grid32"
  * (0/1: with "grid16", where the padded count of 16-column tile columns is
- * even, tiles of 16 x 32; default 1), "grid_series" (0/1: pairs of the 16-row
+ * even, tiles of 16 x 32; default 1), "grid64" (0/1: with "grid32", where
+ * that count is a multiple of 4, tiles of 16 x 64; default 1), "grid_series" (0/1: pairs of the 16-row
  * kernels too near for the GEMM factorisation are summed by the q-series to
  * third order where its bounds hold, else exactly; default 1), "grid_near"
  * (0/1: waves of targets off the 16-row kernels' far-field bounds take the
@@ -126,7 +127,7 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * 1 = one launch), "mc_lattice" (0/1: per-target
  * FP32-split Monte Carlo on product-grid sources reads positions from row
  * and column tables; same results; default 1).  Results do not depend on them beyond summation order and, for "grid",
- * "grid16" and "grid32", the FP32-split rounding (all paths stay within
+ * "grid16", "grid32" and "grid64", the FP32-split rounding (all paths stay within
  * ~1e-6).  Replaces the blockDim/gridDim/thread_size/n_streams hints of
  * cuda/input.h:116-122. */
 holo_status holo_set_option(const char *name, long value);

@@ -1,10 +1,11 @@
 """The matrix-core lattice kernels (HOLO_FP32_SPLIT on product-grid sources).
 
-Every test runs three times: with options "grid16" and "grid32" on (uniform
-lattices use 16-row tiles, padded to multiples of 16, of 32 columns where the
-tile columns pair up, else of 16), with "grid32" off (16 x 16 tiles), and with
-"grid16" off (8 x 8 tiles on product grids with sides multiples of 8, the
-tiled kernel on other planes).
+Every test runs four times: with options "grid16", "grid32" and "grid64" on
+(uniform lattices use 16-row tiles, padded to multiples of 16, of 64 columns
+where the tile columns form quads, else of 32 where they pair up, else of 16),
+with "grid64" off (16 x 32 tiles where the columns pair up), with "grid32" off
+(16 x 16 tiles), and with "grid16" off (8 x 8 tiles on product grids with
+sides multiples of 8, the tiled kernel on other planes).
 Bars: relative L2 <= 1e-6 against the FP64 oracle, as for every FP32-split
 path; the lattice and the tiled kernel agree to the same bar; results do not
 depend on the launch split or on the weights' scale.
@@ -26,13 +27,16 @@ def _announce(request):
     yield
 
 
-@pytest.fixture(autouse=True, params=[32, 16, 8], ids=["tiles32", "tiles16", "tiles8"])
+@pytest.fixture(autouse=True, params=[64, 32, 16, 8],
+                ids=["tiles64", "tiles32", "tiles16", "tiles8"])
 def tiles(request, pkg):
     pkg.set_option("grid16", 0 if request.param == 8 else 1)
-    pkg.set_option("grid32", 1 if request.param == 32 else 0)
+    pkg.set_option("grid32", 1 if request.param >= 32 else 0)
+    pkg.set_option("grid64", 1 if request.param == 64 else 0)
     yield request.param
     pkg.set_option("grid16", 1)
     pkg.set_option("grid32", 1)
+    pkg.set_option("grid64", 1)
 
 
 def rel_l2(a, b):
@@ -177,23 +181,28 @@ def test_lattice_complex_weights_and_adjoint(pkg, ctx, oracle, workloads):
     assert rel_l2(yc, oracle.transform_cartesian(-1, LAM, gz, v, u)) <= 1e-6
 
 
-def test_lattice_32_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloads, tiles):
-    """A uniform 128 x 96 lattice runs the 16 x 32 kernel with "grid16" and
-    "grid32" on, the 16 x 16 kernel with "grid32" off and the 8 x 8 kernel with
-    "grid16" off: all three meet the bar and round differently."""
-    u = workloads.grid(128, 96, 10e-6, 0.0)
+def test_lattice_64_32_16_and_8_tiles_are_different_paths(pkg, ctx, oracle, workloads, tiles):
+    """A uniform 128 x 128 lattice runs the 16 x 64 kernel with "grid16",
+    "grid32" and "grid64" on, the 16 x 32 kernel with "grid64" off, the 16 x 16
+    kernel with "grid32" off and the 8 x 8 kernel with "grid16" off: all four
+    meet the bar and round differently."""
+    u = workloads.grid(128, 128, 10e-6, 0.0)
     amp, phase = weights(17, len(u))
     v = targets(18, 500, 0.35, 4e-3)
     plan = ctx.plan(u)
     ref = oracle.transform_full(+1, LAM, amp, phase, u, v)
     ys = {}
-    for name, g16, g32 in (("32", 1, 1), ("16", 1, 0), ("8", 0, 1)):
+    for name, g16, g32, g64 in (("64", 1, 1, 1), ("32", 1, 1, 0), ("16", 1, 0, 0),
+                                ("8", 0, 1, 1)):
         pkg.set_option("grid16", g16)
         pkg.set_option("grid32", g32)
+        pkg.set_option("grid64", g64)
         ys[name] = plan.transform(amp, phase, v, +1, LAM, pkg.FP32_SPLIT)
         assert rel_l2(ys[name], ref) <= 1e-6, name
-    for a, b in (("32", "16"), ("32", "8"), ("16", "8")):
-        assert 0 < rel_l2(ys[a], ys[b]) <= 1e-6, (a, b)
+    names = list(ys)
+    for i, a in enumerate(names):
+        for b in names[i + 1:]:
+            assert 0 < rel_l2(ys[a], ys[b]) <= 1e-6, (a, b)
 
 
 @pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
@@ -220,17 +229,60 @@ def test_lattice_16x32_waves_fall_back_to_16x16(pkg, ctx, oracle, workloads, sig
     assert rel_l2(ys[1], ref) <= 1e-6
 
 
-@pytest.mark.parametrize("nx, ny", [(16, 32), (16, 16), (32, 32), (8, 8), (48, 96)])
+@pytest.mark.parametrize("nx, ny", [(16, 32), (16, 16), (32, 32), (8, 8), (48, 96), (16, 64),
+                                    (48, 128)])
 @pytest.mark.parametrize("M", [1, 65])
 def test_lattice_smallest_tilings(pkg, ctx, oracle, workloads, nx, ny, M, tiles):
-    """One 16 x 32 pair, one 16 x 16 tile, two pairs, one 8 x 8 tile and an odd
-    number of tile rows, against the oracle for one and for 65 targets."""
+    """One 16 x 32 pair, one 16 x 16 tile, two pairs, one 8 x 8 tile, an odd
+    number of tile rows, one 16 x 64 quad and two quads per row of tiles,
+    against the oracle for one and for 65 targets."""
     u = workloads.grid(nx, ny, 10e-6, 0.0)
     amp, phase = weights(27, len(u))
     v = targets(28, M, 0.3, 2e-3)
     plan = ctx.plan(u)
     assert plan.lattice() == (nx, ny)
     y = plan.transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
+
+
+@pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
+def test_lattice_16x64_waves_fall_back_to_16x32(pkg, ctx, oracle, workloads, sign, obliquity,
+                                                tiles):
+    """Targets within 0.1 mm of the axis of a 64 x 256 lattice at 0.15 m pass
+    the 16 x 32 far-field bounds (at most 0.83 of each) but fail the 16 x 64
+    ones (the dropped share of the cross term, at least 1.45 of its bound):
+    every wave takes the 16 x 32 tiles of the same sources.  With amplitudes
+    0.75 quads, pairs and tiles alike have weight scale 1, and the result is
+    the 16 x 32 kernel's, bit for bit."""
+    u = workloads.grid(64, 256, 10e-6, 0.0)
+    amp = np.full(len(u), 0.75)
+    phase = np.random.default_rng(29).random(len(u)) * 2 * np.pi
+    v = targets(30, 512, 0.15, 1e-4)
+    plan = ctx.plan(u)
+    ys = {}
+    for g64 in (1, 0):
+        pkg.set_option("grid16", 1)
+        pkg.set_option("grid32", 1)
+        pkg.set_option("grid64", g64)
+        ys[g64] = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+    assert np.array_equal(ys[1], ys[0])
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=obliquity)
+    assert rel_l2(ys[1], ref) <= 1e-6
+
+
+@pytest.mark.parametrize("quarter, scale", [(0, 1e-30), (3, 1e-30), (2, 0.0)])
+def test_lattice_tile_quad_weight_scales(pkg, ctx, oracle, workloads, quarter, scale, tiles):
+    """The four 16 x 16 quarters of a 16 x 64 tile share one weight scale: one
+    quarter's weights far below (or zero beside) the others' still meet the
+    bar."""
+    u = workloads.grid(32, 128, 10e-6, 0.0)
+    amp, phase = weights(31, len(u))
+    a2 = amp.reshape(32, 128).copy()
+    for c0 in (0, 64):  # that quarter of both 16 x 64 tiles of each row of tiles
+        a2[:, c0 + 16 * quarter:c0 + 16 * quarter + 16] *= scale
+    amp = a2.reshape(-1)
+    v = targets(32, 300, 0.3, 2e-3)
+    y = ctx.plan(u).transform(amp, phase, v, -1, LAM, pkg.FP32_SPLIT)
     assert rel_l2(y, oracle.transform_full(-1, LAM, amp, phase, u, v)) <= 1e-6
 
 
