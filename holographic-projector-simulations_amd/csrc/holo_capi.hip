@@ -121,6 +121,9 @@ struct holo_plan {
   double g16_cx0 = 0, g16_cx1 = 0, g16_cy0 = 0, g16_cy1 = 0;
   float g16_rho_x = 0, g16_rho_y = 0;
   DevBuf g16_centre, g16_perm, g16_wc, g16_wq, g16_wqn, g16_ts;
+  // their row x's and column y's (16 TI + 16 TJ) for the exact path
+  size_t g16_tj = 0, g16_nx = 0, g16_ny = 0;
+  DevBuf g16_axes;
   // the 16 x 32 tiles formed by pairs of 16 x 16 tiles (16x16 tiles 2T, 2T+1)
   // when the padded count of tile columns is even (g32_tiles = 0 otherwise)
   size_t g32_tiles = 0;
@@ -517,6 +520,11 @@ struct HostGrid {
   float rho16_x = 0, rho16_y = 0;
   std::vector<double4> centre16;
   std::vector<int32_t> perm16;
+  // the 16 x 16 tiles' row x's (16 TI, row a: x of source a ny) followed by
+  // their column y's (16 TJ, column b: y of source b), extended over the
+  // padding by the lattice's pitches; tj16 = TJ
+  std::vector<double> axes16;
+  size_t tj16 = 0;
   double cx0_32 = INFINITY, cx1_32 = -INFINITY, cy0_32 = INFINITY, cy1_32 = -INFINITY;
   float rho32_y = 0;
   std::vector<double4> centre32;
@@ -548,6 +556,12 @@ void build_grid16(const double* u, HostGrid& g) {
   const size_t TI = (nx + 15) / 16, TJ = (ny + 15) / 16 + ((ny + 15) / 16 % 2 && ny > 64);
   g.centre16.reserve(TI * TJ);
   g.perm16.reserve(256 * TI * TJ);
+  g.tj16 = TJ;
+  g.axes16.resize(16 * (TI + TJ));
+  for (size_t a = 0; a < 16 * TI; ++a)
+    g.axes16[a] = a < nx ? u[3 * a * ny] : x0 + (double)a * px;
+  for (size_t b = 0; b < 16 * TJ; ++b)
+    g.axes16[16 * TI + b] = b < ny ? u[3 * b + 1] : y0 + (double)b * py;
   for (size_t I = 0; I < TI; ++I)
     for (size_t J = 0; J < TJ; ++J) {
       const double cx = x0 + ((double)(16 * I) + 7.5) * px;
@@ -758,6 +772,13 @@ holo_status upload_grid16(holo_ctx* ctx, holo_plan* p, const HostGrid& grid) {
   HOLO_TRY(ensure(p->g16_ts, n16 * sizeof(float), ctx->device));
   HOLO_HIP(hipMemcpyAsync(p->g16_centre.ptr, grid.centre16.data(), n16 * sizeof(double4),
                           hipMemcpyHostToDevice, ctx->stream));
+  p->g16_tj = grid.tj16;
+  p->g16_nx = grid.nx;
+  p->g16_ny = grid.ny;
+  HOLO_TRY(ensure(p->g16_axes, grid.axes16.size() * sizeof(double), ctx->device));
+  HOLO_HIP(hipMemcpyAsync(p->g16_axes.ptr, grid.axes16.data(),
+                          grid.axes16.size() * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
   HOLO_HIP(hipMemcpyAsync(p->g16_perm.ptr, grid.perm16.data(), 256 * n16 * sizeof(int32_t),
                           hipMemcpyHostToDevice, ctx->stream));
   if (nt)
@@ -936,6 +957,13 @@ holo_status plan_run_body(holo_ctx* ctx, const holo_plan* p, int precision, int 
     g16.tail_perm = g16.perm + 256 * nt;
     g16.tail_wc = static_cast<const float2*>(p->g16_wc.ptr) + 256 * nt;
     g16.n_tail = (uint32_t)p->g16_tail;
+    if (holo::tuning().grid_rows) {
+      g16.xs = static_cast<const double*>(p->g16_axes.ptr);
+      g16.ys = g16.xs + 16 * (nt / p->g16_tj);
+    }
+    g16.tj = (uint32_t)p->g16_tj;
+    g16.nx = (uint32_t)p->g16_nx;
+    g16.ny = (uint32_t)p->g16_ny;
     holo::GridTiles g32 = g16;  // the pairs of them
     if (pairs) {
       g32.centre = static_cast<const double4*>(p->g32_centre.ptr);
@@ -1176,6 +1204,8 @@ holo_status holo_set_option(const char* name, long value) {
     t.grid_series = value != 0;
   } else if (n == "grid_near") {
     t.grid_near = value != 0;
+  } else if (n == "grid_rows") {
+    t.grid_rows = value != 0;
   } else if (n == "mc_lattice") {
     t.mc_lattice = value != 0;
   } else if (n == "mc_passes") {
@@ -1204,6 +1234,7 @@ long holo_get_option(const char* name) {
   if (n == "grid64") return t.grid64 ? 1 : 0;
   if (n == "grid_series") return t.grid_series ? 1 : 0;
   if (n == "grid_near") return t.grid_near ? 1 : 0;
+  if (n == "grid_rows") return t.grid_rows ? 1 : 0;
   if (n == "mc_lattice") return t.mc_lattice ? 1 : 0;
   if (n == "mc_passes") return t.mc_passes;
   return -1;

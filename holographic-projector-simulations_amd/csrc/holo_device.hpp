@@ -147,16 +147,15 @@ __device__ __forceinline__ void sincos_rev_f32(float t, float& s, float& c) {
 // ---------------------------------------------------------------------------
 // FP32-split helpers shared by the tiled and lattice kernels.
 
-// One sample of the FP32-split estimator: d (to a few 1e-15, not always
-// correctly rounded) and the fractional cycle in FP64, sin/cos in FP32, FP32
-// accumulation (flushed to FP64 by the caller).  OBL: amplitude |dz| / d^2
-// instead of 1 / d.
+// One pair of the FP32-split formulation from its squared distance s and
+// axial offset dz: d (to a few 1e-15, not always correctly rounded) and the
+// fractional cycle in FP64, sin/cos in FP32, FP32 accumulation (flushed to
+// FP64 by the caller).  OBL: amplitude |dz| / d^2 instead of 1 / d.
 template <int SIGN, bool OBL = false>
-__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
-                                             double inv_lambda, float& fr, float& fi) {
-  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
+__device__ __forceinline__ void pair_f32s_s(double s, double dz, float2 w, double inv_lambda,
+                                            float& fr, float& fi) {
   double d, id;
-  hm_sqrt_inv_fast(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), d, id);
+  hm_sqrt_inv_fast(s, d, id);
   // the cycle's fraction in [0, 1), exact (v_fract_f64), then rounded to FP32
   // (within 3e-8 cycles)
   const float f = (float)__builtin_amdgcn_fract(d * inv_lambda);
@@ -164,13 +163,23 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
   sincos_rev_f32(f, sn, cs);
   if (SIGN < 0) sn = -sn;
   const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
+  const float pr = idf * cs, pi = idf * sn;
+  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
+  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+}
+
+// One sample of the FP32-split estimator at the source p (position in
+// .x .y .z, the FP32 weight packed in .w).
+template <int SIGN, bool OBL = false>
+__device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double vy, double vz,
+                                             double inv_lambda, float& fr, float& fi) {
+  const double dx = vx - p.x, dy = vy - p.y, dz = vz - p.z;
   // the FP32 weight packed in .w, unpacked by integer ops (no trip through memory)
   const uint64_t wb = __builtin_bit_cast(uint64_t, p.w);
   const float2 w = make_float2(__builtin_bit_cast(float, (uint32_t)wb),
                                __builtin_bit_cast(float, (uint32_t)(wb >> 32)));
-  const float pr = idf * cs, pi = idf * sn;
-  fr = __builtin_fmaf(w.x, pr, __builtin_fmaf(-w.y, pi, fr));
-  fi = __builtin_fmaf(w.x, pi, __builtin_fmaf(w.y, pr, fi));
+  pair_f32s_s<SIGN, OBL>(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), dz, w,
+                         inv_lambda, fr, fi);
 }
 
 // Accuracy guard: per (target, tile) the bound q_max of |q| over the tile
@@ -230,6 +239,49 @@ __device__ __forceinline__ void exact_tile_pairs(
     const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
                                    __builtin_bit_cast(double, wj));
     mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
+  }
+  ar += fr;
+  ai += fi;
+}
+
+// The exact sum of one target over one 16 x 16 tile of a uniform lattice,
+// whose rows have one x and columns one y each: xs holds the tile's 16 row x's,
+// ys its 16 column y's (entries beyond the lattice finite and unused), z the
+// lattice's z, w the tile's 256 weights (slot 16 i + j = row i, column j).
+// Rows below ni and columns below nj are sources, the other slots padding.
+// dx^2 is formed once per row and dy once per column; each pair then takes
+// the operations of mc_pair_f32s, in the same order, and the pairs come in
+// slot order, so the sum has the bits of exact_tile_pairs over the tile's
+// slots, without the per-pair gathers of three positions through perm.
+template <int SIGN, bool OBL>
+__device__ __forceinline__ void exact_lattice_tile(
+    const double* __restrict__ xs, const double* __restrict__ ys, double z, int ni, int nj,
+    const float2* __restrict__ w, double vx, double vy, double vz, double inv_lambda,
+    double& ar, double& ai) {
+  float fr = 0.f, fi = 0.f;
+  const double dz = vz - z;
+  double dy[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) dy[j] = vy - ys[j];
+  if (nj == 16) {  // every column a source: straight-line rows
+    for (int i = 0; i < ni; ++i) {
+      const double dx = vx - xs[i];
+      const double dx2 = dx * dx;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        pair_f32s_s<SIGN, OBL>(__builtin_fma(dz, dz, __builtin_fma(dy[j], dy[j], dx2)), dz,
+                               w[16 * i + j], inv_lambda, fr, fi);
+    }
+  } else {
+    for (int i = 0; i < ni; ++i) {
+      const double dx = vx - xs[i];
+      const double dx2 = dx * dx;
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
+        if (j < nj)
+          pair_f32s_s<SIGN, OBL>(__builtin_fma(dz, dz, __builtin_fma(dy[j], dy[j], dx2)), dz,
+                                 w[16 * i + j], inv_lambda, fr, fi);
+    }
   }
   ar += fr;
   ai += fi;

@@ -68,6 +68,15 @@ struct GridBounds {
   int near_gemm = 0;  // 16-row kernels: waves off the far-field bounds take the near-field GEMM
 };
 
+// A uniform lattice's 16 x 16 tiles for the exact path, beside the table
+// `axes` of their row x's (16 per row of tiles) followed, from index ys, by
+// their column y's (16 per column of tiles): the number of tile columns and
+// the lattice's sides.  (The table is a kernel argument of its own, so that
+// its loads, wave-uniform, go through the scalar cache.)
+struct LatticeAxes {
+  uint32_t ys, tj, nx, ny;
+};
+
 // The accuracy bounds of one (target, tile), or of a target against every
 // tile when given the largest lateral offsets and the smallest distance: FP32
 // rounding of the phase, truncation of the q-series, the dropped second order
@@ -722,7 +731,8 @@ __device__ __forceinline__ void grid16_tiles(
     const uint4* __restrict__ wq, const uint4* __restrict__ wqn,
     const float* __restrict__ tscale, uint32_t t0, uint32_t t1,
     double vx, double vy, double vz, double inv_lambda, double px, double py,
-    const GridBounds& gb, bool far_all, double& ex_r, double& ex_i) {
+    const GridBounds& gb, const double* __restrict__ axes, const LatticeAxes& ax, bool far_all,
+    double& ex_r, double& ex_i) {
   // every fused operation is written out, so that the same tiles give the same
   // bits whichever kernel this is inlined into
 #pragma clang fp contract(off)
@@ -1015,7 +1025,16 @@ __device__ __forceinline__ void grid16_tiles(
         if (gb.series && series_pairs_ok<OBL>(Pm + Qm, rx2, ry2, 2.f * alf, idf))
           series_tile_pairs<SIGN, OBL, NKC>(wc + (size_t)t * 256 * NKC, rx, ry, rz, dref, id,
                                             inv_lambda, px, py, ex_r, ex_i);
-        else
+        else if (NKC == 1 && axes) {
+          // tile t = I tj + J holds rows 16 I .. and columns 16 J .. of the lattice
+          // (wave-uniform, kept in scalar registers so that xs and ys take scalar loads)
+          const uint32_t I = __builtin_amdgcn_readfirstlane(t / ax.tj), J = t - I * ax.tj;
+          const int ni = min(16, (int)ax.nx - 16 * (int)I), nj = min(16, (int)ax.ny - 16 * (int)J);
+          if (nj > 0)
+            exact_lattice_tile<SIGN, OBL>(axes + 16 * (size_t)I, axes + ax.ys + 16 * (size_t)J, gb.z,
+                                          ni, nj, wc + (size_t)t * 256, vx, vy, vz, inv_lambda,
+                                          ex_r, ex_i);
+        } else
           exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC,
                                       wc + (size_t)t * 256 * NKC, 256 * NKC, vx, vy, vz,
                                       inv_lambda, ex_r, ex_i);
@@ -1071,7 +1090,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const float* __restrict__ tscale, uint32_t n_tiles,
     uint32_t chunk, const double* __restrict__ v, uint32_t M, double inv_lambda, double px,
     double py, GridBounds gb, const double4* __restrict__ centre32, GridBounds gb32,
-    const double4* __restrict__ centre16, GridBounds gb16,
+    const double4* __restrict__ centre16, GridBounds gb16, const double* __restrict__ axes,
+    LatticeAxes ax,
     const int32_t* __restrict__ tail_perm, const float2* __restrict__ tail_wc, uint32_t n_tail,
     double2* __restrict__ out) {
   __shared__ Grid16Lds lds;
@@ -1095,10 +1115,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     far32 = !far && __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb32, rx2c));
   if (far) {
     grid16_tiles<SIGN, OBL, NKC>(lds, centre, perm, u, wc, wq, wqn, tscale, t0, t1, vx, vy, vz,
-                                 inv_lambda, px, py, gb, true, ex_r, ex_i);
+                                 inv_lambda, px, py, gb, axes, ax, true, ex_r, ex_i);
   } else if (far32) {
     grid16_tiles<SIGN, OBL, 2>(lds, centre32, perm, u, wc, wq, wqn, tscale, 2 * t0, 2 * t1, vx,
-                               vy, vz, inv_lambda, px, py, gb32, true, ex_r, ex_i);
+                               vy, vz, inv_lambda, px, py, gb32, axes, ax, true, ex_r, ex_i);
   } else {
     // 16 x 16 tiles: those of the launch (NKC = 1), or the halves of its 16 x 32
     // tiles, whose looser bounds keep more pairs on the far-field GEMM
@@ -1108,10 +1128,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     const bool far16 = NKC == 1 ? false : __all(grid16_far<OBL>(vx, vy, vz, inv_lambda, gb16, rx2c));
     if (far16 || !b16.near_gemm)
       grid16_tiles<SIGN, OBL, 1>(lds, c16, perm, u, wc, wq, wqn, tscale, s0, s1, vx, vy, vz,
-                                 inv_lambda, px, py, b16, far16, ex_r, ex_i);
+                                 inv_lambda, px, py, b16, axes, ax, far16, ex_r, ex_i);
     else  // the near-field GEMM, with the series and exact paths beyond its bounds
       grid16_tiles<SIGN, OBL, 1, true>(lds, c16, perm, u, wc, wq, wqn, tscale, s0, s1, vx, vy,
-                                       vz, inv_lambda, px, py, b16, false, ex_r, ex_i);
+                                       vz, inv_lambda, px, py, b16, axes, ax, false, ex_r, ex_i);
   }
   // in blocks of 256 sources, so that each FP32 partial sum stays short
   if (blockIdx.y == 0)
@@ -1157,22 +1177,30 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
                         halves.rho_x, halves.rho_y, series,     near_gemm};
   const GridBounds gb32{pairs.cx0,   pairs.cx1,   pairs.cy0, pairs.cy1, pairs.z,
                         pairs.rho_x, pairs.rho_y, series,    near_gemm};
+  // the exact path's row and column tables (16 x 16 tiles)
+  const double* axes = halves.xs;
+  const LatticeAxes ax{(uint32_t)(halves.ys - halves.xs), halves.tj, halves.nx, halves.ny};
+  if (axes && (!halves.ys || !ax.tj ||
+               (uint64_t)ax.tj * ((ax.nx + 15) / 16) !=
+                   (uint64_t)(nkc == 1 ? gt.n_tiles : halves.n_tiles) ||
+               ax.ys != 16 * ((ax.nx + 15) / 16)))
+    return hipErrorInvalidValue;
   with_sign(sign, [&](auto sg) {
     with_bool(obliquity, [&](auto obl) {
       if (nkc == 4)
         k_superpose_grid16<sg.value, (bool)obl.value, 4><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, pairs.centre, gb32, halves.centre, gb16, gt.tail_perm,
+            il, gt.px, gt.py, gb, pairs.centre, gb32, halves.centre, gb16, axes, ax, gt.tail_perm,
             gt.tail_wc, gt.n_tail, out);
       else if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, gt.centre, gb, halves.centre, gb16, gt.tail_perm, gt.tail_wc,
+            il, gt.px, gt.py, gb, gt.centre, gb, halves.centre, gb16, axes, ax, gt.tail_perm, gt.tail_wc,
             gt.n_tail, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, gt.centre, gb, gt.centre, gb, gt.tail_perm, gt.tail_wc,
+            il, gt.px, gt.py, gb, gt.centre, gb, gt.centre, gb, axes, ax, gt.tail_perm, gt.tail_wc,
             gt.n_tail, out);
     });
   });

@@ -44,6 +44,12 @@ struct GridTiles {
   double cx0 = 0, cx1 = 0, cy0 = 0, cy1 = 0, z = 0;
   float rho_x = 0, rho_y = 0;
   double px = 0, py = 0;  // pitches of a uniform lattice (16 x 16 tiles)
+  // a uniform lattice's row x's (16 per row of 16 x 16 tiles) and column y's
+  // (16 per column of tiles), tj tile columns, nx x ny sources (16 x 16
+  // tiles; xs = nullptr: the exact path reads positions through perm)
+  const double* xs = nullptr;
+  const double* ys = nullptr;
+  uint32_t tj = 0, nx = 0, ny = 0;
   // sources after a padded lattice (16 x 16 tiles): slots, weights, count
   const int32_t* tail_perm = nullptr;
   const float2* tail_wc = nullptr;
@@ -64,6 +70,7 @@ struct Tuning {
   bool grid64 = true;     // ... as 16 x 64 tiles where they form quads
   bool grid_series = true; // 16-row kernels: near-field pairs by the q-series, not exactly
   bool grid_near = true;   // 16-row kernels: waves off the far-field bounds take the near-field GEMM
+  bool grid_rows = true;   // 16-row kernels: exact pairs read row x's and column y's from tables
   bool mc_lattice = true; // per-target FP32-split MC: table gather on product grids
   int mc_passes = 0;      // per-target MC: source ranges, one launch each (0 = automatic)
 };

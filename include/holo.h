@@ -122,7 +122,10 @@ holo_status holo_synchronize(holo_ctx *ctx);
  * third order where its bounds hold, else exactly; default 1), "grid_near"
  * (0/1: waves of targets off the 16-row kernels' far-field bounds take the
  * near-field GEMM, its cross factor to second order, before the q-series and
- * exact paths; default 1), "mc_passes"
+ * exact paths; default 1), "grid_rows" (0/1: the 16-row kernels' exact
+ * pairs take the x of their row and the y of their column from tables of
+ * the lattice's rows and columns, dx^2 once per row, instead of each
+ * source's position; same bits; default 1), "mc_passes"
  * (per-target Monte Carlo: source ranges, one launch each, 0 = automatic,
  * 1 = one launch), "mc_lattice" (0/1: per-target
  * FP32-split Monte Carlo on product-grid sources reads positions from row

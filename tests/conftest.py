@@ -49,6 +49,6 @@ def _reset_options():
         for k, v in (("r_f64", 4), ("r_f32", 2), ("splits", 0), ("target_waves", 16384),
                      ("f64_libm", 0), ("f64_planar", 1), ("f64_rows", 1),
                      ("grid", 1), ("grid16", 1),
-                     ("grid32", 1), ("grid64", 1), ("grid_series", 1), ("grid_near", 1), ("mc_lattice", 1),
+                     ("grid32", 1), ("grid64", 1), ("grid_series", 1), ("grid_near", 1), ("grid_rows", 1), ("mc_lattice", 1),
                      ("mc_passes", 0)):
             holo.set_option(k, v)

@@ -53,7 +53,7 @@ def test_options(pkg):
         pkg.set_option("mc_passes", 65)
     pkg.set_option("mc_passes", 0)
     for name in ("f64_libm", "f64_planar", "f64_rows", "grid", "grid16", "grid32", "grid64", "grid_series",
-                 "grid_near", "mc_lattice"):
+                 "grid_near", "grid_rows", "mc_lattice"):
         old = pkg.get_option(name)
         assert old in (0, 1)
         pkg.set_option(name, 1 - old)

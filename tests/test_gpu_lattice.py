@@ -528,6 +528,40 @@ def test_lattice_near_field_full_plane(pkg, ctx, oracle, workloads, tiles):
     assert err_report <= 1e-6
 
 
+@pytest.mark.parametrize("z", [2e-3, 2e-2])
+@pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
+def test_lattice_exact_rows_same_bits(pkg, ctx, oracle, workloads, z, sign, obliquity, tiles):
+    """Targets 2 mm and 2 cm from the source plane: the (target, tile) pairs
+    fail every expansion's bounds and are summed exactly.  The 16-row kernels
+    take each pair's x and y from the lattice's row and column tables (option
+    grid_rows); with the option off they gather the source's position instead.
+    Both give the same bits: on a 40 x 72 lattice (16-row tiles of 8 rows and
+    of 8 columns, and one tile column of padding only, since 5 tile columns
+    become 6), on the generator's padded plane of 3001 points (54 x 55 and 31
+    copies of point 0), with source splits, and within 1e-6 of the oracle."""
+    planes = [workloads.grid(40, 72, 10e-6, 0.0),
+              workloads.reference_plane(3001, 0.6e-3, 0.0, 1.0, (0.0, 0.0))]
+    for u in planes:
+        g = np.random.default_rng(len(u))
+        amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+        v = targets(len(u) % 89, 320, z, half=1.5e-3)
+        plan = ctx.plan(u)
+        assert plan.lattice() in ((40, 72), (54, 55))
+        ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=obliquity)
+        for splits in (0, 3):
+            pkg.set_option("splits", splits)
+            ys = []
+            for rows in (1, 0):
+                pkg.set_option("grid_rows", rows)
+                ys.append(plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT,
+                                         obliquity=obliquity))
+            pkg.set_option("grid_rows", 1)
+            err = rel_l2(ys[0], ref)
+            print(f"{len(u)} sources, z={z}, splits {splits}: rel L2 {err:.3e}")
+            assert np.array_equal(ys[0], ys[1]) and err <= 1e-6
+        pkg.set_option("splits", 0)
+
+
 @pytest.mark.parametrize("splits", [1, 3, 7])
 def test_lattice_tail_of_arbitrary_points(pkg, ctx, oracle, workloads, splits, tiles):
     """A uniform lattice followed by fewer than max(nx, ny) other sources at
