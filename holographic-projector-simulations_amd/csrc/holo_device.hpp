@@ -191,6 +191,18 @@ __device__ __forceinline__ void mc_pair_f32s(const double4& p, double vx, double
 // sin/cos and weights.
 constexpr float kGuardCycles = 2e-7f;
 
+// One exact pair of the FP32-split formulation at the source (ux, uy, uz):
+// d^2 = dy^2 + (dz^2 + dx^2), in that order, so that a lattice row's
+// dz^2 + dx^2 can be formed once (exact_lattice_tile) with the same bits.
+template <int SIGN, bool OBL>
+__device__ __forceinline__ void exact_pair(double ux, double uy, double uz, float2 w, double vx,
+                                           double vy, double vz, double inv_lambda, float& fr,
+                                           float& fi) {
+  const double dx = vx - ux, dy = vy - uy, dz = vz - uz;
+  pair_f32s_s<SIGN, OBL>(__builtin_fma(dy, dy, __builtin_fma(dz, dz, dx * dx)), dz, w,
+                         inv_lambda, fr, fi);
+}
+
 // POLAR_W: w holds (|w|, arg w in cycles) rather than (re, im).  Sources
 // come 8 at a time, their (wave-uniform) loads issued before any of them is
 // evaluated, so that the dependent loads through perm overlap; padding slots
@@ -223,8 +235,7 @@ __device__ __forceinline__ void exact_tile_pairs(
         sincos_rev_f32(ww.y, sn, cs);
         ww = make_float2(ww.x * cs, ww.x * sn);
       }
-      p[h].w = __builtin_bit_cast(double, ww);
-      mc_pair_f32s<SIGN, OBL>(p[h], vx, vy, vz, inv_lambda, fr, fi);
+      exact_pair<SIGN, OBL>(p[h].x, p[h].y, p[h].z, ww, vx, vy, vz, inv_lambda, fr, fi);
     }
   }
   for (int j = j0; j < T; ++j) {
@@ -236,9 +247,8 @@ __device__ __forceinline__ void exact_tile_pairs(
       sincos_rev_f32(wj.y, sn, cs);
       wj = make_float2(wj.x * cs, wj.x * sn);
     }
-    const double4 p = make_double4(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2],
-                                   __builtin_bit_cast(double, wj));
-    mc_pair_f32s<SIGN, OBL>(p, vx, vy, vz, inv_lambda, fr, fi);
+    exact_pair<SIGN, OBL>(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2], wj, vx,
+                          vy, vz, inv_lambda, fr, fi);
   }
   ar += fr;
   ai += fi;
@@ -249,9 +259,9 @@ __device__ __forceinline__ void exact_tile_pairs(
 // ys its 16 column y's (entries beyond the lattice finite and unused), z the
 // lattice's z, w the tile's 256 weights (slot 16 i + j = row i, column j).
 // Rows below ni and columns below nj are sources, the other slots padding.
-// dx^2 is formed once per row and dy once per column; each pair then takes
-// the operations of mc_pair_f32s, in the same order, and the pairs come in
-// slot order, so the sum has the bits of exact_tile_pairs over the tile's
+// dz^2 + dx^2 is formed once per row and dy once per column; each pair then
+// takes the operations of exact_pair, in the same order, and the pairs come
+// in slot order, so the sum has the bits of exact_tile_pairs over the tile's
 // slots, without the per-pair gathers of three positions through perm.
 template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_lattice_tile(
@@ -266,21 +276,21 @@ __device__ __forceinline__ void exact_lattice_tile(
   if (nj == 16) {  // every column a source: straight-line rows
     for (int i = 0; i < ni; ++i) {
       const double dx = vx - xs[i];
-      const double dx2 = dx * dx;
+      const double a = __builtin_fma(dz, dz, dx * dx);
 #pragma unroll
       for (int j = 0; j < 16; ++j)
-        pair_f32s_s<SIGN, OBL>(__builtin_fma(dz, dz, __builtin_fma(dy[j], dy[j], dx2)), dz,
-                               w[16 * i + j], inv_lambda, fr, fi);
+        pair_f32s_s<SIGN, OBL>(__builtin_fma(dy[j], dy[j], a), dz, w[16 * i + j], inv_lambda,
+                               fr, fi);
     }
   } else {
     for (int i = 0; i < ni; ++i) {
       const double dx = vx - xs[i];
-      const double dx2 = dx * dx;
+      const double a = __builtin_fma(dz, dz, dx * dx);
 #pragma unroll
       for (int j = 0; j < 16; ++j)
         if (j < nj)
-          pair_f32s_s<SIGN, OBL>(__builtin_fma(dz, dz, __builtin_fma(dy[j], dy[j], dx2)), dz,
-                                 w[16 * i + j], inv_lambda, fr, fi);
+          pair_f32s_s<SIGN, OBL>(__builtin_fma(dy[j], dy[j], a), dz, w[16 * i + j], inv_lambda,
+                                 fr, fi);
     }
   }
   ar += fr;
