@@ -150,7 +150,7 @@ def main():
                     help="comma list of chain, fp32, mc, adjoint (profiling runs pick one), "
                          "and lattices and distance (not run by default: the FP32-split "
                          "lattice kernel's tile shapes on narrow and square lattices, and its "
-                         "speed against FP64 from 0.05 to 0.35 m)")
+                         "speed against FP64 from 0.02 to 0.35 m)")
     ap.add_argument("--hologram-steps", type=int, default=20,
                     help="Adam steps of the hologram section (the full optimisation is 200)")
     ap.add_argument("--lattice-tiles", type=int, choices=(0, 8, 16, 32, 64), default=64,
@@ -176,6 +176,10 @@ def main():
     ap.add_argument("--no-near", action="store_true",
                     help="library option grid_near = 0: waves off the lattice kernels' "
                          "far-field bounds take the q-series and exact paths per pair")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="library option grid_rows = 0: the lattice kernels' exact pairs gather "
+                         "each source's position instead of reading the lattice's row and "
+                         "column tables (same bits)")
     ap.add_argument("--target-waves", type=int, default=0,
                     help="library option target_waves (waves a launch aims for when it splits "
                          "sources; 0 = the library default)")
@@ -205,6 +209,8 @@ def main():
         holo.set_option("grid_series", 0)
     if a.no_near:
         holo.set_option("grid_near", 0)
+    if a.no_rows:
+        holo.set_option("grid_rows", 0)
     import importlib
     wl = importlib.import_module("holo_amd.workloads")
     w = wl.chain_xyz(0)
