@@ -1029,11 +1029,12 @@ __device__ __forceinline__ void grid16_tiles(
           // tile t = I tj + J holds rows 16 I .. and columns 16 J .. of the lattice
           // (wave-uniform, kept in scalar registers so that xs and ys take scalar loads)
           const uint32_t I = __builtin_amdgcn_readfirstlane(t / ax.tj), J = t - I * ax.tj;
-          const int ni = min(16, (int)ax.nx - 16 * (int)I), nj = min(16, (int)ax.ny - 16 * (int)J);
+          const int ni = min(16, (int)ax.nx - 16 * (int)I);
+          const int nj = min(16, (int)ax.ny - 16 * (int)J);
           if (nj > 0)
-            exact_lattice_tile<SIGN, OBL>(axes + 16 * (size_t)I, axes + ax.ys + 16 * (size_t)J, gb.z,
-                                          ni, nj, wc + (size_t)t * 256, vx, vy, vz, inv_lambda,
-                                          ex_r, ex_i);
+            exact_lattice_tile<SIGN, OBL>(axes + 16 * (size_t)I, axes + ax.ys + 16 * (size_t)J,
+                                          gb.z, ni, nj, wc + (size_t)t * 256, vx, vy, vz,
+                                          inv_lambda, ex_r, ex_i);
         } else
           exact_tile_pairs<SIGN, OBL>(u, perm + (size_t)t * 256 * NKC,
                                       wc + (size_t)t * 256 * NKC, 256 * NKC, vx, vy, vz,
@@ -1190,18 +1191,18 @@ hipError_t launch_superpose_grid16(int sign, bool obliquity, int nkc, const Grid
       if (nkc == 4)
         k_superpose_grid16<sg.value, (bool)obl.value, 4><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, pairs.centre, gb32, halves.centre, gb16, axes, ax, gt.tail_perm,
-            gt.tail_wc, gt.n_tail, out);
+            il, gt.px, gt.py, gb, pairs.centre, gb32, halves.centre, gb16, axes, ax,
+            gt.tail_perm, gt.tail_wc, gt.n_tail, out);
       else if (nkc == 2)
         k_superpose_grid16<sg.value, (bool)obl.value, 2><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, gt.centre, gb, halves.centre, gb16, axes, ax, gt.tail_perm, gt.tail_wc,
-            gt.n_tail, out);
+            il, gt.px, gt.py, gb, gt.centre, gb, halves.centre, gb16, axes, ax, gt.tail_perm,
+            gt.tail_wc, gt.n_tail, out);
       else
         k_superpose_grid16<sg.value, (bool)obl.value, 1><<<grid, kBlock, 0, stream>>>(
             gt.centre, gt.perm, gt.u, wc, wq, wqn, tscale, gt.n_tiles, chunk, v, (uint32_t)M,
-            il, gt.px, gt.py, gb, gt.centre, gb, gt.centre, gb, axes, ax, gt.tail_perm, gt.tail_wc,
-            gt.n_tail, out);
+            il, gt.px, gt.py, gb, gt.centre, gb, gt.centre, gb, axes, ax, gt.tail_perm,
+            gt.tail_wc, gt.n_tail, out);
     });
   });
   hipError_t e = hipGetLastError();
