@@ -203,11 +203,10 @@ __device__ __forceinline__ void exact_pair(double ux, double uy, double uz, floa
                          inv_lambda, fr, fi);
 }
 
-// POLAR_W: w holds (|w|, arg w in cycles) rather than (re, im).  Sources
-// come 8 at a time, their (wave-uniform) loads issued before any of them is
-// evaluated, so that the dependent loads through perm overlap; padding slots
-// (perm < 0) are skipped as before, in the same order.
-template <int SIGN, bool OBL, bool POLAR_W = false>
+// Sources come 8 at a time, their (wave-uniform) loads issued before any of
+// them is evaluated, so that the dependent loads through perm overlap;
+// padding slots (perm < 0) are skipped.
+template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_tile_pairs(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
     const float2* __restrict__ w, int T, double vx, double vy, double vz,
@@ -229,29 +228,91 @@ __device__ __forceinline__ void exact_tile_pairs(
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
       if (n[h] < 0) continue;
-      float2 ww = wj[h];
-      if constexpr (POLAR_W) {
-        float sn, cs;
-        sincos_rev_f32(ww.y, sn, cs);
-        ww = make_float2(ww.x * cs, ww.x * sn);
-      }
-      exact_pair<SIGN, OBL>(p[h].x, p[h].y, p[h].z, ww, vx, vy, vz, inv_lambda, fr, fi);
+      exact_pair<SIGN, OBL>(p[h].x, p[h].y, p[h].z, wj[h], vx, vy, vz, inv_lambda, fr, fi);
     }
   }
   for (int j = j0; j < T; ++j) {
     const int32_t n = perm[j];
     if (n < 0) continue;
-    float2 wj = w[j];
-    if constexpr (POLAR_W) {
-      float sn, cs;
-      sincos_rev_f32(wj.y, sn, cs);
-      wj = make_float2(wj.x * cs, wj.x * sn);
-    }
-    exact_pair<SIGN, OBL>(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2], wj, vx,
-                          vy, vz, inv_lambda, fr, fi);
+    exact_pair<SIGN, OBL>(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2], w[j],
+                          vx, vy, vz, inv_lambda, fr, fi);
   }
   ar += fr;
   ai += fi;
+}
+
+// One exact pair with the weight in polar form, w = (|w|, arg w in cycles):
+// the weight's phase joins the pair's fractional cycle before sin/cos and
+// its modulus the amplitude (FP64 distance and cycle fraction as in
+// exact_pair, the sum of the two fractions in FP32, within 6e-8 cycles).
+template <int SIGN, bool OBL>
+__device__ __forceinline__ void exact_pair_polar(double ux, double uy, double uz, float2 w,
+                                                 double vx, double vy, double vz,
+                                                 double inv_lambda, float& fr, float& fi) {
+  const double dx = vx - ux, dy = vy - uy, dz = vz - uz;
+  double d, id;
+  hm_sqrt_inv_fast(__builtin_fma(dy, dy, __builtin_fma(dz, dz, dx * dx)), d, id);
+  const float f = (float)__builtin_amdgcn_fract(d * inv_lambda);
+  float sn, cs;
+  sincos_rev_f32(SIGN > 0 ? w.y + f : w.y - f, sn, cs);
+  const float idf = OBL ? (float)(id * (__builtin_fabs(dz) * id)) : (float)id;
+  const float amp = w.x * idf;
+  fr = __builtin_fmaf(amp, cs, fr);
+  fi = __builtin_fmaf(amp, sn, fi);
+}
+
+// The exact pairs of one tile for those of a lane's R targets whose flag
+// near[r] is set, with polar weights (the tiled kernel's).  Each source is
+// loaded once for all R targets, 8 sources at a time as in exact_tile_pairs;
+// padding slots are skipped.  Every target's pairs come in slot order.
+template <int SIGN, bool OBL, int R>
+__device__ __forceinline__ void exact_tile_pairs_polar(
+    const double* __restrict__ u, const int32_t* __restrict__ perm,
+    const float2* __restrict__ w, int T, const double (&vx)[R], const double (&vy)[R],
+    const double (&vz)[R], const bool (&near)[R], double inv_lambda, double (&ar)[R],
+    double (&ai)[R]) {
+  float fr[R], fi[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) fr[r] = fi[r] = 0.f;
+  int j0 = 0;
+  for (; j0 + 8 <= T; j0 += 8) {
+    int32_t n[8];
+    float2 wj[8];
+    double4 p[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) n[h] = perm[j0 + h];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      wj[h] = w[j0 + h];
+      const size_t m = n[h] < 0 ? 0 : (size_t)n[h];  // a padding slot reads source 0, unused
+      p[h] = make_double4(u[3 * m], u[3 * m + 1], u[3 * m + 2], 0.0);
+    }
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      if (n[h] < 0) continue;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (near[r])
+          exact_pair_polar<SIGN, OBL>(p[h].x, p[h].y, p[h].z, wj[h], vx[r], vy[r], vz[r],
+                                      inv_lambda, fr[r], fi[r]);
+    }
+  }
+  for (int j = j0; j < T; ++j) {
+    const int32_t n = perm[j];
+    if (n < 0) continue;
+    const double px = u[3 * (size_t)n], py = u[3 * (size_t)n + 1], pz = u[3 * (size_t)n + 2];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (near[r])
+        exact_pair_polar<SIGN, OBL>(px, py, pz, w[j], vx[r], vy[r], vz[r], inv_lambda, fr[r],
+                                    fi[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (near[r]) {
+      ar[r] += fr[r];
+      ai[r] += fi[r];
+    }
 }
 
 // The exact sum of one target over one 16 x 16 tile of a uniform lattice,

@@ -460,25 +460,31 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
     }
     const float4* g = geo + (size_t)t * T;
     const float2* w = wc + (size_t)t * T;
-#pragma unroll 4
-    for (int j = 0; j < T; ++j) {
-      const float4 gj = g[j];
-      const float2 wj = w[j];
+    // a wave whose every target is near this tile skips the expansion
+    bool all_near = true;
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        float q = __builtin_fmaf(m2ry[r], gj.y, gj.w);
-        if constexpr (!PLANAR) q = __builtin_fmaf(m2rz[r], gj.z, q);
-        q = __builtin_fmaf(m2rx[r], gj.x, q);
-        // w e^{i s 2 pi t} = |w| e^{2 pi i (arg w + s t)}: the weight's phase
-        // joins the pair's, its modulus the amplitude
-        const float pc = __builtin_fmaf(c2[r], q, alpha[r]);
-        const float tt = SIGN > 0 ? __builtin_fmaf(q, pc, wj.y) : __builtin_fmaf(-q, pc, wj.y);
-        float sn, cs;
-        sincos_rev_f32(tt, sn, cs);
-        float amp = wj.x * __builtin_fmaf(q, gq[r], invd[r]);
-        if constexpr (OBL && !PLANAR) amp *= __builtin_fabsf(rzf[r] - gj.z);
-        sr[r] = __builtin_fmaf(amp, cs, sr[r]);
-        si[r] = __builtin_fmaf(amp, sn, si[r]);
+    for (int r = 0; r < R; ++r) all_near &= near[r];
+    if (__builtin_amdgcn_ballot_w64(!all_near) != 0) {
+#pragma unroll 4
+      for (int j = 0; j < T; ++j) {
+        const float4 gj = g[j];
+        const float2 wj = w[j];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          float q = __builtin_fmaf(m2ry[r], gj.y, gj.w);
+          if constexpr (!PLANAR) q = __builtin_fmaf(m2rz[r], gj.z, q);
+          q = __builtin_fmaf(m2rx[r], gj.x, q);
+          // w e^{i s 2 pi t} = |w| e^{2 pi i (arg w + s t)}: the weight's phase
+          // joins the pair's, its modulus the amplitude
+          const float pc = __builtin_fmaf(c2[r], q, alpha[r]);
+          const float tt = SIGN > 0 ? __builtin_fmaf(q, pc, wj.y) : __builtin_fmaf(-q, pc, wj.y);
+          float sn, cs;
+          sincos_rev_f32(tt, sn, cs);
+          float amp = wj.x * __builtin_fmaf(q, gq[r], invd[r]);
+          if constexpr (OBL && !PLANAR) amp *= __builtin_fabsf(rzf[r] - gj.z);
+          sr[r] = __builtin_fmaf(amp, cs, sr[r]);
+          si[r] = __builtin_fmaf(amp, sn, si[r]);
+        }
       }
     }
 #pragma unroll
@@ -488,13 +494,9 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
       ar[r] += xr * cr - xi * cs;
       ai[r] += xr * cs + xi * cr;
     }
-    if (any_near) {  // rare: thick tiles or near field
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        if (near[r])
-          exact_tile_pairs<SIGN, OBL, true>(u, perm + (size_t)t * T, w, T, vx[r], vy[r], vz[r],
-                                            inv_lambda, ar[r], ai[r]);
-    }
+    if (any_near)  // thick tiles or near field
+      exact_tile_pairs_polar<SIGN, OBL, R>(u, perm + (size_t)t * T, w, T, vx, vy, vz, near,
+                                           inv_lambda, ar, ai);
   }
   double2* dst = out + (size_t)blockIdx.y * M;
 #pragma unroll
