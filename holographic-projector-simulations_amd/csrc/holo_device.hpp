@@ -241,17 +241,16 @@ __device__ __forceinline__ void exact_tile_pairs(
   ai += fi;
 }
 
-// One exact pair with the weight in polar form, w = (|w|, arg w in cycles):
-// the weight's phase joins the pair's fractional cycle before sin/cos and
-// its modulus the amplitude (FP64 distance and cycle fraction as in
-// exact_pair, the sum of the two fractions in FP32, within 6e-8 cycles).
+// One exact pair from its squared distance s and axial offset dz, with the
+// weight in polar form, w = (|w|, arg w in cycles): the weight's phase joins
+// the pair's fractional cycle before sin/cos and its modulus the amplitude
+// (FP64 distance and cycle fraction as in pair_f32s_s, the sum of the two
+// fractions in FP32, within 6e-8 cycles).
 template <int SIGN, bool OBL>
-__device__ __forceinline__ void exact_pair_polar(double ux, double uy, double uz, float2 w,
-                                                 double vx, double vy, double vz,
-                                                 double inv_lambda, float& fr, float& fi) {
-  const double dx = vx - ux, dy = vy - uy, dz = vz - uz;
+__device__ __forceinline__ void polar_pair_s(double s, double dz, float2 w, double inv_lambda,
+                                             float& fr, float& fi) {
   double d, id;
-  hm_sqrt_inv_fast(__builtin_fma(dy, dy, __builtin_fma(dz, dz, dx * dx)), d, id);
+  hm_sqrt_inv_fast(s, d, id);
   const float f = (float)__builtin_amdgcn_fract(d * inv_lambda);
   float sn, cs;
   sincos_rev_f32(SIGN > 0 ? w.y + f : w.y - f, sn, cs);
@@ -265,15 +264,34 @@ __device__ __forceinline__ void exact_pair_polar(double ux, double uy, double uz
 // near[r] is set, with polar weights (the tiled kernel's).  Each source is
 // loaded once for all R targets, 8 sources at a time as in exact_tile_pairs;
 // padding slots are skipped.  Every target's pairs come in slot order.
-template <int SIGN, bool OBL, int R>
+// PLANAR: every source of the tile has the z of its centre, cz, so that dz
+// and dz^2 are formed once per target and d^2 = dy^2 + (dx^2 + dz^2).
+template <int SIGN, bool OBL, int R, bool PLANAR>
 __device__ __forceinline__ void exact_tile_pairs_polar(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
-    const float2* __restrict__ w, int T, const double (&vx)[R], const double (&vy)[R],
-    const double (&vz)[R], const bool (&near)[R], double inv_lambda, double (&ar)[R],
-    double (&ai)[R]) {
+    const float2* __restrict__ w, int T, double cz, const double (&vx)[R],
+    const double (&vy)[R], const double (&vz)[R], const bool (&near)[R], double inv_lambda,
+    double (&ar)[R], double (&ai)[R]) {
   float fr[R], fi[R];
+  double dzp[R], dz2[R];  // PLANAR: vz - cz and its square
 #pragma unroll
-  for (int r = 0; r < R; ++r) fr[r] = fi[r] = 0.f;
+  for (int r = 0; r < R; ++r) {
+    fr[r] = fi[r] = 0.f;
+    dzp[r] = vz[r] - cz;
+    dz2[r] = dzp[r] * dzp[r];
+  }
+  const auto pair = [&](int r, double px, double py, double pz, float2 wj)
+                        __attribute__((always_inline)) {
+    const double dx = vx[r] - px, dy = vy[r] - py;
+    if constexpr (PLANAR) {
+      polar_pair_s<SIGN, OBL>(__builtin_fma(dy, dy, __builtin_fma(dx, dx, dz2[r])), dzp[r], wj,
+                              inv_lambda, fr[r], fi[r]);
+    } else {
+      const double dz = vz[r] - pz;
+      polar_pair_s<SIGN, OBL>(__builtin_fma(dy, dy, __builtin_fma(dz, dz, dx * dx)), dz, wj,
+                              inv_lambda, fr[r], fi[r]);
+    }
+  };
   int j0 = 0;
   for (; j0 + 8 <= T; j0 += 8) {
     int32_t n[8];
@@ -285,27 +303,24 @@ __device__ __forceinline__ void exact_tile_pairs_polar(
     for (int h = 0; h < 8; ++h) {
       wj[h] = w[j0 + h];
       const size_t m = n[h] < 0 ? 0 : (size_t)n[h];  // a padding slot reads source 0, unused
-      p[h] = make_double4(u[3 * m], u[3 * m + 1], u[3 * m + 2], 0.0);
+      p[h] = make_double4(u[3 * m], u[3 * m + 1], PLANAR ? 0.0 : u[3 * m + 2], 0.0);
     }
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
       if (n[h] < 0) continue;
 #pragma unroll
       for (int r = 0; r < R; ++r)
-        if (near[r])
-          exact_pair_polar<SIGN, OBL>(p[h].x, p[h].y, p[h].z, wj[h], vx[r], vy[r], vz[r],
-                                      inv_lambda, fr[r], fi[r]);
+        if (near[r]) pair(r, p[h].x, p[h].y, p[h].z, wj[h]);
     }
   }
   for (int j = j0; j < T; ++j) {
     const int32_t n = perm[j];
     if (n < 0) continue;
-    const double px = u[3 * (size_t)n], py = u[3 * (size_t)n + 1], pz = u[3 * (size_t)n + 2];
+    const double px = u[3 * (size_t)n], py = u[3 * (size_t)n + 1];
+    const double pz = PLANAR ? 0.0 : u[3 * (size_t)n + 2];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      if (near[r])
-        exact_pair_polar<SIGN, OBL>(px, py, pz, w[j], vx[r], vy[r], vz[r], inv_lambda, fr[r],
-                                    fi[r]);
+      if (near[r]) pair(r, px, py, pz, w[j]);
   }
 #pragma unroll
   for (int r = 0; r < R; ++r)

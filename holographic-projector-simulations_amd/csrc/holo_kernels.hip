@@ -495,8 +495,8 @@ __global__ __launch_bounds__(kBlock) void k_superpose_f32s(
       ai[r] += xr * cs + xi * cr;
     }
     if (any_near)  // thick tiles or near field
-      exact_tile_pairs_polar<SIGN, OBL, R>(u, perm + (size_t)t * T, w, T, vx, vy, vz, near,
-                                           inv_lambda, ar, ai);
+      exact_tile_pairs_polar<SIGN, OBL, R, PLANAR>(u, perm + (size_t)t * T, w, T, c.z, vx, vy,
+                                                   vz, near, inv_lambda, ar, ai);
   }
   double2* dst = out + (size_t)blockIdx.y * M;
 #pragma unroll
