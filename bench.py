@@ -163,6 +163,12 @@ def main():
                     help="fp32 and mc sections: this many sources from the reference's plane "
                          "generator (10.24 mm wide; 1105920 = 1024 x 1080, a 1051 x 1052 lattice "
                          "and 268 copies of point 0) instead of the 1024^2 lattice")
+    ap.add_argument("--source-jitter", type=float, default=0.0,
+                    help="fp32 and mc sections: move every point of the 1024^2 source lattice "
+                         "by up to this fraction of the pitch per axis (uniform, seed 5; the "
+                         "reference's -r gives 0.4875), which leaves no lattice: the tiled kernel")
+    ap.add_argument("--tile", type=int, choices=(8, 16, 32, 64), default=64,
+                    help="sources per tile of the tiled FP32-split kernel (the plan's tile)")
     ap.add_argument("--target-z", type=float, default=None,
                     help="distance of the 1024^2 target plane of the fp32 and mc sections "
                          "(default 0.35 m); nearer planes exercise the lattice kernel's "
@@ -256,8 +262,13 @@ def main():
                     if a.source_points else "1024^2")
         if a.target_z is not None:
             p3.v[:, 2] = a.target_z
+        if a.source_jitter and not a.source_points:
+            gj = np.random.default_rng(5)
+            p3.u[:, :2] += (gj.random((len(p3.u), 2)) - 0.5) * (2 * a.source_jitter * 10e-6)
+            src_desc = f"1024^2 jittered by up to {a.source_jitter:g} pitch"
         amp3, ph3, u3, v3 = T(p3.amp), T(p3.phase), T(p3.u), T(p3.v)
-        plan3 = holo.Plan(ctx, u3)
+        plan3 = holo.Plan(ctx, u3, tile=a.tile)
+        lattice3 = plan3.lattice() != (0, 0)
         y64 = torch.empty((v3.shape[0], 2), dtype=torch.float64, device=dev)
         y32 = torch.empty_like(y64)
         if "fp32" in sections:
@@ -282,7 +293,7 @@ def main():
                 "workload": f"{src_desc} -> 1024^2 (z 0 -> {p3.v[0, 2]:g} m, 10 / 7.56 um, amp U[0,1), "
                             "phase U[0,2pi)), " + (
                                 f"lattice kernel, {min(a.lattice_tiles, 16)}x{a.lattice_tiles} tiles"
-                                if a.lattice_tiles else "tiled kernel, tiles of 64"),
+                                if a.lattice_tiles and lattice3 else f"tiled kernel, tiles of {a.tile}"),
                 "pairs": pairs,
                 "fp64_pairs_per_s": pairs / (t64["median_ms"] / 1e3),
                 "fp64_ms": t64,
