@@ -464,6 +464,38 @@ def test_fp32_split_near_field(pkg, ctx, oracle, workloads, z):
     assert rel_l2(y, ref) <= 1e-6
 
 
+@pytest.mark.parametrize("z", [2e-3, 2e-2])
+@pytest.mark.parametrize("slab", [False, True])
+@pytest.mark.parametrize("sign, obliquity", [(+1, False), (-1, True)])
+def test_fp32_split_tiled_exact_pairs(pkg, ctx, oracle, workloads, z, slab, sign, obliquity):
+    """The tiled kernel's exact pairs, which take every (target, tile) here:
+    a 48 x 48 plane jittered by up to 0.4875 pitch in x and y (planar tiles:
+    dz once per target), or also by 50 um in z (a slab: dz per pair), 2 mm
+    and 2 cm from 777 targets (a ragged last wave), with 1, 2 and 4 targets
+    per lane (which share each source load) and tiles of 64 and 16: within
+    1e-6 of the oracle."""
+    g = np.random.default_rng(int(z * 1e4) + 7 * slab)
+    u = workloads.grid(48, 48, 10e-6, 0.0)
+    u[:, :2] += (g.random((len(u), 2)) - 0.5) * 2 * 0.4875 * 10e-6
+    if slab:
+        u[:, 2] = (g.random(len(u)) - 0.5) * 1e-4
+    amp, phase = g.random(len(u)), g.random(len(u)) * 2 * np.pi
+    v = np.zeros((777, 3))
+    v[:, :2] = (g.random((777, 2)) - 0.5) * 2e-3
+    v[:, 2] = z
+    ref = oracle.transform_full(sign, LAM, amp, phase, u, v, obliquity=obliquity)
+    for tile in (64, 16):
+        plan = ctx.plan(u, tile)
+        assert plan.lattice() == (0, 0) and pkg.plan_layout(u, tile)[3] == (not slab)
+        for r in (1, 2, 4):
+            pkg.set_option("r_f32", r)
+            y = plan.transform(amp, phase, v, sign, LAM, pkg.FP32_SPLIT, obliquity=obliquity)
+            err = rel_l2(y, ref)
+            print(f"z={z} slab={slab} tile={tile} R={r}: rel L2 {err:.3e}")
+            assert err <= 1e-6
+    pkg.set_option("r_f32", 2)
+
+
 def test_plan_info_grid(pkg, ctx, workloads):
     st = workloads.plane_1024(0).stages[0]
     info = ctx.plan(st.u, 16).info()
