@@ -203,8 +203,10 @@ __device__ __forceinline__ void exact_pair(double ux, double uy, double uz, floa
                          inv_lambda, fr, fi);
 }
 
-// Sources come 8 at a time, their (wave-uniform) loads issued before any of
-// them is evaluated, so that the dependent loads through perm overlap;
+// The exact sum of one target over the T slots of a tile (weights as
+// (re, im); slot j holds source perm[j]): exact_pair per source, in slot
+// order.  Sources come 8 at a time, their (wave-uniform) loads issued before
+// any of them is evaluated, so that the dependent loads through perm overlap;
 // padding slots (perm < 0) are skipped.
 template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_tile_pairs(
