@@ -203,11 +203,17 @@ __device__ __forceinline__ void exact_pair(double ux, double uy, double uz, floa
                          inv_lambda, fr, fi);
 }
 
+// exact_tile_pairs and exact_lattice_tile sum their pairs in FP32 over runs of
+// kExactRun slots (a lattice tile's row), each run's sum then added in FP64:
+// over a 16 x 16 tile that cuts the FP32 summation error about 3x (the
+// tiled kernel's tiles of at most 64 are summed whole).
+constexpr int kExactRun = 16;
+
 // The exact sum of one target over the T slots of a tile (weights as
 // (re, im); slot j holds source perm[j]): exact_pair per source, in slot
-// order.  Sources come 8 at a time, their (wave-uniform) loads issued before
-// any of them is evaluated, so that the dependent loads through perm overlap;
-// padding slots (perm < 0) are skipped.
+// order, in runs of kExactRun slots.  Sources come 8 at a time, their
+// (wave-uniform) loads issued before any of them is evaluated, so that the
+// dependent loads through perm overlap; padding slots (perm < 0) are skipped.
 template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_tile_pairs(
     const double* __restrict__ u, const int32_t* __restrict__ perm,
@@ -232,12 +238,22 @@ __device__ __forceinline__ void exact_tile_pairs(
       if (n[h] < 0) continue;
       exact_pair<SIGN, OBL>(p[h].x, p[h].y, p[h].z, wj[h], vx, vy, vz, inv_lambda, fr, fi);
     }
+    if ((j0 + 8) % kExactRun == 0) {
+      ar += fr;
+      ai += fi;
+      fr = fi = 0.f;
+    }
   }
   for (int j = j0; j < T; ++j) {
     const int32_t n = perm[j];
-    if (n < 0) continue;
-    exact_pair<SIGN, OBL>(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2], w[j],
-                          vx, vy, vz, inv_lambda, fr, fi);
+    if (n >= 0)
+      exact_pair<SIGN, OBL>(u[3 * (size_t)n], u[3 * (size_t)n + 1], u[3 * (size_t)n + 2], w[j],
+                            vx, vy, vz, inv_lambda, fr, fi);
+    if ((j + 1) % kExactRun == 0) {
+      ar += fr;
+      ai += fi;
+      fr = fi = 0.f;
+    }
   }
   ar += fr;
   ai += fi;
@@ -338,9 +354,10 @@ __device__ __forceinline__ void exact_tile_pairs_polar(
 // lattice's z, w the tile's 256 weights (slot 16 i + j = row i, column j).
 // Rows below ni and columns below nj are sources, the other slots padding.
 // dz^2 + dx^2 is formed once per row and dy once per column; each pair then
-// takes the operations of exact_pair, in the same order, and the pairs come
-// in slot order, so the sum has the bits of exact_tile_pairs over the tile's
-// slots, without the per-pair gathers of three positions through perm.
+// takes the operations of exact_pair, in the same order, the pairs come in
+// slot order and each row is one run of kExactRun slots, so the sum has the
+// bits of exact_tile_pairs over the tile's slots, without the per-pair
+// gathers of three positions through perm.
 template <int SIGN, bool OBL>
 __device__ __forceinline__ void exact_lattice_tile(
     const double* __restrict__ xs, const double* __restrict__ ys, double z, int ni, int nj,
@@ -359,6 +376,9 @@ __device__ __forceinline__ void exact_lattice_tile(
       for (int j = 0; j < 16; ++j)
         pair_f32s_s<SIGN, OBL>(__builtin_fma(dy[j], dy[j], a), dz, w[16 * i + j], inv_lambda,
                                fr, fi);
+      ar += fr;
+      ai += fi;
+      fr = fi = 0.f;
     }
   } else {
     for (int i = 0; i < ni; ++i) {
@@ -369,6 +389,9 @@ __device__ __forceinline__ void exact_lattice_tile(
         if (j < nj)
           pair_f32s_s<SIGN, OBL>(__builtin_fma(dy[j], dy[j], a), dz, w[16 * i + j], inv_lambda,
                                  fr, fi);
+      ar += fr;
+      ai += fi;
+      fr = fi = 0.f;
     }
   }
   ar += fr;
