@@ -560,6 +560,15 @@ def test_lattice_exact_rows_same_bits(pkg, ctx, oracle, workloads, z, sign, obli
             print(f"{len(u)} sources, z={z}, splits {splits}: rel L2 {err:.3e}")
             assert np.array_equal(ys[0], ys[1]) and err <= 1e-6
         pkg.set_option("splits", 0)
+        # complex weights (the adjoint's sources) through the same exact pairs
+        gz = g.standard_normal(len(u)) + 1j * g.standard_normal(len(u))
+        yc = []
+        for rows in (1, 0):
+            pkg.set_option("grid_rows", rows)
+            yc.append(plan.transform_complex(gz, v, sign, LAM, pkg.FP32_SPLIT))
+        pkg.set_option("grid_rows", 1)
+        refc = oracle.transform_cartesian(sign, LAM, gz, u, v)
+        assert np.array_equal(yc[0], yc[1]) and rel_l2(yc[0], refc) <= 1e-6
 
 
 @pytest.mark.parametrize("splits", [1, 3, 7])
