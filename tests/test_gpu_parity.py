@@ -494,6 +494,11 @@ def test_fp32_split_tiled_exact_pairs(pkg, ctx, oracle, workloads, z, slab, sign
             print(f"z={z} slab={slab} tile={tile} R={r}: rel L2 {err:.3e}")
             assert err <= 1e-6
     pkg.set_option("r_f32", 2)
+    # complex weights (the adjoint's sources): their modulus and argument
+    # join the exact pairs as for polar input
+    gz = g.standard_normal(len(u)) + 1j * g.standard_normal(len(u))
+    yc = ctx.plan(u).transform_complex(gz, v, sign, LAM, pkg.FP32_SPLIT)
+    assert rel_l2(yc, oracle.transform_cartesian(sign, LAM, gz, u, v)) <= 1e-6
 
 
 def test_plan_info_grid(pkg, ctx, workloads):
